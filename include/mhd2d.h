@@ -1,0 +1,152 @@
+/*
+ * mhd2d.h -- C ABI of the MI355X (gfx950) 2D resistive-MHD RK2 library.
+ *
+ * The state uses the reference's conventions: nx, ny count cells including
+ * 3 ghost layers per side; cell (i, j) has linear index k = j + i*ny; a cell
+ * holds 8 doubles (rho, rho*u, rho*v, rho*w, Bx, By, Bz, e) where Bx is the
+ * x-face value at (i+1/2, j) and By the y-face value at (i, j+1/2).  Host
+ * buffers passed through this ABI are in that array-of-structs layout
+ * ("AoS": 8*nx*ny doubles).  Device storage is private to the library.
+ *
+ * Every function returns 0 on success and a negative MHD_ERR_* code on
+ * failure; mhd_last_error() gives the message.  No exception crosses the ABI.
+ * A handle owns one HIP stream and its device buffers; handles are not
+ * internally synchronised (use one handle per host thread).
+ *
+ * Reference interface each entry point replaces (paths relative to the
+ * reference root, P = ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_periodicXY,
+ * S = ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_symmetricXY):
+ * cited at each declaration.
+ */
+#ifndef MHD2D_H
+#define MHD2D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHD2D_ABI_VERSION 1
+#define MHD_GHOST 3
+
+enum mhd_status {
+    MHD_OK = 0,
+    MHD_ERR_INVALID = -1,   /* bad argument or configuration */
+    MHD_ERR_HIP = -2,       /* a HIP runtime call failed */
+    MHD_ERR_NOMEM = -3,     /* device or host allocation failed */
+    MHD_ERR_IO = -4,        /* file could not be read or written */
+    MHD_ERR_STATE = -5      /* call not valid in the current state */
+};
+
+enum mhd_bc { MHD_BC_PERIODIC = 0, MHD_BC_SYMMETRIC = 1 };
+
+typedef struct mhd_config {
+    int nx, ny;          /* cells per axis including 3 ghost layers per side (P/const.hpp:10,15) */
+    double dx, dy;       /* P/const.hpp:7,12 */
+    double xmin, ymin;   /* lower corner of the interior (used by the initial-state setups) */
+    double gamma;        /* gamma_mhd, P/const.hpp:18 */
+    double cfl;          /* CFL, P/const.hpp:17 */
+    double eps;          /* EPS, P/const.hpp:4 (1e-20 in every FP64 example) */
+    double eta;          /* eta: uniform resistivity and the dt diffusion limit, P/const.hpp:25 */
+    int bc_x, bc_y;      /* enum mhd_bc per axis: P/ (periodic x and y), S/ (symmetric x and y),
+                            periodic x with symmetric y as ResistiveMHD_singleGPU_thrust/
+                            lib_ResistiveMHD_2D_GPU_single/resistiveMHD_2D.cu:198-259 */
+} mhd_config;
+
+typedef struct mhd_handle mhd_handle;
+
+/* Fills *cfg with the reference example defaults (gamma 5/3, CFL 0.7,
+ * EPS 1e-20, eta 0, periodic x and y) for an interior of nx_int x ny_int
+ * cells on [xmin, xmin+lx] x [ymin, ymin+ly].  Host only. */
+int mhd_default_config(mhd_config *cfg, int nx_int, int ny_int, double xmin, double lx,
+                       double ymin, double ly);
+
+/* ResistiveMHD2D::ResistiveMHD2D() (P/resistiveMHD_2D.cu:11-22) together with
+ * initializeDeviceConstants() (P/const.cu:4-24): allocates the device state
+ * and copies the configuration.  The state starts all-zero, as the
+ * reference's device_vector members do. */
+int mhd_create(const mhd_config *cfg, mhd_handle **out);
+int mhd_destroy(mhd_handle *h);
+const char *mhd_last_error(const mhd_handle *h);   /* h may be NULL: last global error */
+int mhd_get_config(const mhd_handle *h, mhd_config *cfg);
+
+/* Replaces the user-written ResistiveMHD2D::initializeU body
+ * (P/resistiveMHD_2D.hpp:28; e.g. EX/plasmoid/main_plasmoid.cu:85-156
+ * writes U then applies the boundary): uploads an AoS host state of all
+ * nx*ny cells as is.  The boundary is not applied; call mhd_apply_boundary. */
+int mhd_set_state(mhd_handle *h, const double *host_aos);
+/* ResistiveMHD2D::getU (P/resistiveMHD_2D.cu:431-434) as a host copy. */
+int mhd_get_state(mhd_handle *h, double *host_aos);
+
+/* Per-cell resistivity for the resistive flux (nx*ny doubles, cell (i, j)
+ * at j + i*ny).  Replaces the example plugin getEta(x, y) evaluated at
+ * (i*dx, j*dy) (EX/plasmoid/main_plasmoid.cu:159-172).  NULL clears it, after
+ * which the uniform cfg.eta is used (EX/bhattacharjee/main_bhattacharjee.cu:131-139).
+ * The dt diffusion limit always uses cfg.eta. */
+int mhd_set_eta_field(mhd_handle *h, const double *host_eta);
+int mhd_set_eta(mhd_handle *h, double eta);
+
+/* Boundary::periodicBoundaryX2nd / Y2nd (P/boundary.cu:61-71,131-141) and
+ * Boundary::symmetricBoundaryX2nd / Y2nd (S/boundary.cu:90-104,191-205)
+ * applied to the current state.  axis 0 = x, 1 = y; kind = enum mhd_bc. */
+int mhd_apply_boundary(mhd_handle *h, int axis, int kind);
+/* Sets the boundary kinds oneStepRK2 applies after each stage (x first). */
+int mhd_set_boundary(mhd_handle *h, int bc_x, int bc_y);
+
+/* ResistiveMHD2D::calculateDt (P/resistiveMHD_2D.cu:277-291): CFL time step
+ * of the current state over all nx*ny cells, with the eta diffusion limit. */
+int mhd_calculate_dt(mhd_handle *h, double *dt);
+
+/* ResistiveMHD2D::oneStepRK2 (P/resistiveMHD_2D.cu:133-195; S/ identical
+ * but for the boundary): one RK2 step.  mhd_step_rk2 runs nsteps steps with
+ * no host synchronisation between them. */
+int mhd_one_step_rk2(mhd_handle *h);
+int mhd_step_rk2(mhd_handle *h, int nsteps);
+
+/* dt of the last step and the accumulated time (the examples' host loop
+ * totalTime += dt, EX/plasmoid/main_plasmoid.cu:325). */
+int mhd_get_dt(mhd_handle *h, double *dt);
+int mhd_get_time(mhd_handle *h, double *t);
+int mhd_set_time(mhd_handle *h, double t);
+
+/* ResistiveMHD2D::checkCalculationIsCrashed (P/resistiveMHD_2D.cu:294-309):
+ * *crashed = 1 if any cell's e is NaN after the last step. */
+int mhd_check_crashed(mhd_handle *h, int *crashed);
+
+/* ResistiveMHD2D::save (P/resistiveMHD_2D.cu:198-244): writes
+ * {dir}/{name}_{step}.bin, raw little-endian FP64, all nx*ny cells in index
+ * order k = j + i*ny, 8 doubles per cell, no header.  mhd_load reads such a
+ * file back into the state (the reference has no reader). */
+int mhd_save(mhd_handle *h, const char *dir, const char *name, int step);
+int mhd_load(mhd_handle *h, const char *path);
+
+int mhd_synchronize(mhd_handle *h);
+
+/* Initial-state setups (host only, no GPU needed): fill an AoS buffer of
+ * nx*ny cells, ghosts included, before the boundary is applied.  They replace
+ * the examples' initializeU kernels (face B sampled at half positions as
+ * EX/plasmoid/main_plasmoid.cu:97-126 does).
+ *  - Orszag-Tang vortex: rho = gamma^2, p = gamma, v = (-sin y, sin x),
+ *    B = (-sin y, sin 2x).
+ *  - Harris sheet: Bx = tanh(y), rho = sech^2(y) + 0.2, p = rho/2, vy white
+ *    noise of amplitude noise_amp from a counter hash of (seed, i, j).
+ *  - Harris resistivity: eta0 + eta1 * exp(-(x^2 + y^2) / 4) per cell. */
+int mhd_fill_orszag_tang(const mhd_config *cfg, double *host_aos);
+int mhd_fill_harris(const mhd_config *cfg, unsigned long long seed, double noise_amp, double *host_aos);
+int mhd_fill_harris_eta(const mhd_config *cfg, double eta0, double eta1, double *host_eta);
+
+/* Device-side helpers built on the setups: fill, upload, apply the
+ * configured boundary (as initializeU does). */
+int mhd_init_orszag_tang(mhd_handle *h);
+int mhd_init_harris(mhd_handle *h, unsigned long long seed, double noise_amp, double eta0, double eta1);
+
+/* Launch-shape control (segment length along x of one wave's sweep; 0 picks
+ * the built-in default).  Exposed for the autotuner and the tests. */
+int mhd_set_sweep_length(mhd_handle *h, int columns);
+int mhd_get_sweep_length(const mhd_handle *h, int *columns);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHD2D_H */

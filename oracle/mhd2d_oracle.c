@@ -1,0 +1,820 @@
+/*
+ * mhd2d_oracle.c -- CPU restatement of the reference's single-GPU FP64
+ * resistive-MHD RK2 step.  TEST INFRASTRUCTURE ONLY: this file is the
+ * checker that tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg compare the HIP library against.  It is never linked into, called by
+ * or shipped as part of the product library.
+ *
+ * It follows the reference's pass structure literally: array-of-structs
+ * state of nx*ny cells (3 ghost layers per side included in nx, ny), index
+ * k = j + i*ny (y fastest), the same persistent intermediate arrays and the
+ * same element ranges for every pass, so that values the reference leaves in
+ * ghost cells (stale intermediates, wrap-around at column ends) are the same
+ * here.  Build with -O2 -ffp-contract=off so every operation rounds exactly
+ * as written.
+ *
+ * Reference files (paths relative to the reference root):
+ *   P = ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_periodicXY
+ *   S = ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_symmetricXY
+ *   F32 = ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_single
+ *   EX = ResistiveMHD_singleGPU_thrust/examples2D
+ *
+ * Pinning (which independent check pins each function) is listed in the
+ * header comment of each function; tests/test_oracle.py holds the checks.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "mhd2d_oracle.h"
+
+typedef struct { double rho, rhoU, rhoV, rhoW, bX, bY, bZ, e; } Cons;   /* P/conservation_parameter_struct.hpp:5-27 */
+typedef struct { double rho, u, v, w, bX, bY, bZ, p; } Prim;            /* P/basic_parameter_struct.hpp:5-27 */
+typedef struct { double f0, f1, f2, f3, f4, f5, f6, f7; } Flux;         /* P/flux_struct.hpp:5-27 */
+
+/* thrust::min / thrust::max argument order (lhs, rhs): min returns rhs only
+ * if rhs < lhs, max returns rhs only if lhs < rhs. */
+static inline double tmin(double a, double b) { return (b < a) ? b : a; }
+static inline double tmax(double a, double b) { return (a < b) ? b : a; }
+
+struct oracle_sim {
+    int nx, ny, N;
+    double dx, dy, gamma, cfl, eps, eta;
+    int bc_x, bc_y;
+    double dt, time;
+    long steps;
+    /* ResistiveMHD2D members, P/resistiveMHD_2D.hpp:11-23 */
+    Cons *U, *UBar;
+    Flux *fluxF, *fluxG;
+    double *dtVector, *bXOld, *bYOld, *tmpVector;
+    /* FluxSolver / HLLD members, P/flux_solver.hpp:9-10, P/hlld.hpp:7-23 */
+    Flux *solverFlux, *hlldFlux;
+    Prim *dQCenter, *dQLeft, *dQRight;
+    Cons *tmpUForFluxG;
+    /* CT members, P/ct.hpp:9-12 */
+    Flux *oldFluxF, *oldFluxG;
+    double *eZVector;
+    /* per-cell resistivity sampled at cell (i, j): plays the role of the
+     * example's getEta(i*dx, j*dy) (EX/plasmoid/main_plasmoid.cu:159-172) */
+    double *etaField;
+};
+
+/* ---------------------------------------------------------------- minmod */
+/* P/minmod.hpp:6-16.  Pinned by the closed-form truth table (Q1) in tests. */
+static inline double minmod(double x, double y, double eps)
+{
+    int sign_x = (x > 0) - (x < 0);
+    double abs_x = fabs(x);
+    return sign_x * tmax(tmin(abs_x, sign_x * y), eps);
+}
+
+double oracle_minmod(double x, double y, double eps) { return minmod(x, y, eps); }
+
+/* --------------------------------------------------- primitive variables */
+/* P/calculate_half_Q.cu:4-38 (GetBasicParamterFunctor).  Pinned by the
+ * conservative->primitive round trip test. */
+static inline Prim basic_param(const Cons *c, const Cons *cp1, double gamma)
+{
+    Prim q;
+    double rho = c->rho;
+    double u = c->rhoU / rho;
+    double v = c->rhoV / rho;
+    double w = c->rhoW / rho;
+    double bX = c->bX, bY = c->bY, bZ = c->bZ, e = c->e;
+    double p = (gamma - 1.0)
+             * (e - 0.5 * (rho * (u * u + v * v + w * w))
+             - 0.5 * (bX * bX + bY * bY + bZ * bZ));
+    q.rho = rho; q.u = u; q.v = v; q.w = w;
+    q.bX = 0.5 * (bX + cp1->bX);
+    q.bY = bY; q.bZ = bZ; q.p = p;
+    return q;
+}
+
+/* --------------------------------------------------------------- MUSCL */
+/* P/muscl.cu:6-29 LeftParameterFunctor */
+static inline Prim left_param(const Prim *m1, const Prim *q, const Prim *p1, double eps)
+{
+    Prim l;
+    l.rho = q->rho + 0.5 * minmod(q->rho - m1->rho, p1->rho - q->rho, eps);
+    l.u   = q->u   + 0.5 * minmod(q->u   - m1->u,   p1->u   - q->u,   eps);
+    l.v   = q->v   + 0.5 * minmod(q->v   - m1->v,   p1->v   - q->v,   eps);
+    l.w   = q->w   + 0.5 * minmod(q->w   - m1->w,   p1->w   - q->w,   eps);
+    l.bY  = q->bY  + 0.5 * minmod(q->bY  - m1->bY,  p1->bY  - q->bY,  eps);
+    l.bZ  = q->bZ  + 0.5 * minmod(q->bZ  - m1->bZ,  p1->bZ  - q->bZ,  eps);
+    l.p   = q->p   + 0.5 * minmod(q->p   - m1->p,   p1->p   - q->p,   eps);
+    l.bX  = q->bX;
+    return l;
+}
+
+/* P/muscl.cu:67-90 RightParameterFunctor */
+static inline Prim right_param(const Prim *q, const Prim *p1, const Prim *p2, double eps)
+{
+    Prim r;
+    r.rho = p1->rho - 0.5 * minmod(p1->rho - q->rho, p2->rho - p1->rho, eps);
+    r.u   = p1->u   - 0.5 * minmod(p1->u   - q->u,   p2->u   - p1->u,   eps);
+    r.v   = p1->v   - 0.5 * minmod(p1->v   - q->v,   p2->v   - p1->v,   eps);
+    r.w   = p1->w   - 0.5 * minmod(p1->w   - q->w,   p2->w   - p1->w,   eps);
+    r.bY  = p1->bY  - 0.5 * minmod(p1->bY  - q->bY,  p2->bY  - p1->bY,  eps);
+    r.bZ  = p1->bZ  - 0.5 * minmod(p1->bZ  - q->bZ,  p2->bZ  - p1->bZ,  eps);
+    r.p   = p1->p   - 0.5 * minmod(p1->p   - q->p,   p2->p   - p1->p,   eps);
+    r.bX  = q->bX;
+    return r;
+}
+
+/* ---------------------------------------------------------------- HLLD */
+/* P/hlld.cu:26-32 */
+static inline double sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
+
+/* P/hlld.cu:448-467 getOneFlux */
+static inline Flux one_flux(double rho, double u, double v, double w,
+                            double bX, double bY, double bZ, double e, double pT)
+{
+    Flux f;
+    f.f0 = rho * u;
+    f.f1 = rho * u * u + pT - bX * bX;
+    f.f2 = rho * u * v - bX * bY;
+    f.f3 = rho * u * w - bX * bZ;
+    f.f4 = 0.0;
+    f.f5 = u * bY - v * bX;
+    f.f6 = u * bZ - w * bX;
+    f.f7 = (e + pT) * u - bX * (bX * u + bY * v + bZ * w);
+    return f;
+}
+
+/* One HLLD interface flux from MUSCL states L and R.
+ * Parameters: P/hlld.cu:256-433 (calculateHLLDParameterFunctor, with the
+ * SL<=0<=SR clamp at :315-318 and EPS in every denominator).  Candidates:
+ * P/hlld.cu:469-549 (setFluxFunctor).  Selection: P/hlld.cu:35-107 (sum of
+ * the six indicator-weighted candidates).  The reference stores the
+ * parameters and candidates in arrays between passes; every element is
+ * computed from (dQLeft[k], dQRight[k]) alone, so computing them here in one
+ * function gives the same values.  Pinned by consistency F(U,U) = F_phys(U),
+ * x<->y rotation equivariance, the Bx=0 reduction and mirror symmetry. */
+static Flux hlld_flux(const Prim *L, const Prim *R, double gamma, double eps)
+{
+    double rhoL = L->rho, uL = L->u, vL = L->v, wL = L->w, bXL = L->bX, bYL = L->bY, bZL = L->bZ, pL = L->p;
+    double rhoR = R->rho, uR = R->u, vR = R->v, wR = R->w, bXR = R->bX, bYR = R->bY, bZR = R->bZ, pR = R->p;
+    double eL = pL / (gamma - 1.0)
+              + 0.5 * rhoL * (uL * uL + vL * vL + wL * wL)
+              + 0.5 * (bXL * bXL + bYL * bYL + bZL * bZL);
+    double pTL = pL + 0.5 * (bXL * bXL + bYL * bYL + bZL * bZL);
+    double eR = pR / (gamma - 1.0)
+              + 0.5 * rhoR * (uR * uR + vR * vR + wR * wR)
+              + 0.5 * (bXR * bXR + bYR * bYR + bZR * bZR);
+    double pTR = pR + 0.5 * (bXR * bXR + bYR * bYR + bZR * bZR);
+
+    double csL = sqrt(gamma * pL / rhoL);
+    double caL = sqrt((bXL * bXL + bYL * bYL + bZL * bZL) / rhoL);
+    double vaL = sqrt(bXL * bXL / rhoL);
+    double cfL = sqrt(0.5 * (csL * csL + caL * caL
+               + sqrt((csL * csL + caL * caL) * (csL * csL + caL * caL)
+               - 4.0 * csL * csL * vaL * vaL)));
+    double csR = sqrt(gamma * pR / rhoR);
+    double caR = sqrt((bXR * bXR + bYR * bYR + bZR * bZR) / rhoR);
+    double vaR = sqrt(bXR * bXR / rhoR);
+    double cfR = sqrt(0.5 * (csR * csR + caR * caR
+               + sqrt((csR * csR + caR * caR) * (csR * csR + caR * caR)
+               - 4.0 * csR * csR * vaR * vaR)));
+
+    double SL = tmin(uL, uR) - tmax(cfL, cfR);
+    double SR = tmax(uL, uR) + tmax(cfL, cfR);
+    SL = tmin(SL, 0.0);
+    SR = tmax(SR, 0.0);
+
+    double SM = ((SR - uR) * rhoR * uR - (SL - uL) * rhoL * uL - pTR + pTL)
+              / ((SR - uR) * rhoR - (SL - uL) * rhoL + eps);
+    double pT1 = ((SR - uR) * rhoR * pTL - (SL - uL) * rhoL * pTR
+               + rhoL * rhoR * (SR - uR) * (SL - uL) * (uR - uL))
+               / ((SR - uR) * rhoR - (SL - uL) * rhoL + eps);
+    double pT1L = pT1, pT1R = pT1;
+
+    double rho1L = rhoL * (SL - uL) / (SL - SM + eps);
+    double u1L = SM;
+    double v1L = vL - bXL * bYL * (SM - uL) / (rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps);
+    double w1L = wL - bXL * bZL * (SM - uL) / (rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps);
+    double bX1L = bXL;
+    double bY1L = bYL * (rhoL * (SL - uL) * (SL - uL) - bXL * bXL)
+                / (rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps);
+    double bZ1L = bZL * (rhoL * (SL - uL) * (SL - uL) - bXL * bXL)
+                / (rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps);
+    double e1L = ((SL - uL) * eL - pTL * uL + pT1 * SM
+               + bXL * ((uL * bXL + vL * bYL + wL * bZL) - (u1L * bX1L + v1L * bY1L + w1L * bZ1L)))
+               / (SL - SM + eps);
+
+    double rho1R = rhoR * (SR - uR) / (SR - SM + eps);
+    double u1R = SM;
+    double v1R = vR - bXR * bYR * (SM - uR) / (rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps);
+    double w1R = wR - bXR * bZR * (SM - uR) / (rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps);
+    double bX1R = bXR;
+    double bY1R = bYR * (rhoR * (SR - uR) * (SR - uR) - bXR * bXR)
+                / (rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps);
+    double bZ1R = bZR * (rhoR * (SR - uR) * (SR - uR) - bXR * bXR)
+                / (rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps);
+    double e1R = ((SR - uR) * eR - pTR * uR + pT1 * SM
+               + bXR * ((uR * bXR + vR * bYR + wR * bZR) - (u1R * bX1R + v1R * bY1R + w1R * bZ1R)))
+               / (SR - SM + eps);
+
+    double S1L = SM - sqrt(bX1L * bX1L / rho1L);
+    double S1R = SM + sqrt(bX1R * bX1R / rho1R);
+
+    double rho2L = rho1L, rho2R = rho1R, u2 = SM;
+    double v2 = (sqrt(rho1L) * v1L + sqrt(rho1R) * v1R + (bY1R - bY1L) * sgn(bX1L))
+              / (sqrt(rho1L) + sqrt(rho1R) + eps);
+    double w2 = (sqrt(rho1L) * w1L + sqrt(rho1R) * w1R + (bZ1R - bZ1L) * sgn(bX1L))
+              / (sqrt(rho1L) + sqrt(rho1R) + eps);
+    double bY2 = (sqrt(rho1L) * bY1R + sqrt(rho1R) * bY1L + sqrt(rho1L * rho1R) * (v1R - v1L) * sgn(bX1L))
+               / (sqrt(rho1L) + sqrt(rho1R) + eps);
+    double bZ2 = (sqrt(rho1L) * bZ1R + sqrt(rho1R) * bZ1L + sqrt(rho1L * rho1R) * (w1R - w1L) * sgn(bX1L))
+               / (sqrt(rho1L) + sqrt(rho1R) + eps);
+    double e2L = e1L - sqrt(rho1L)
+               * ((u1L * bX1L + v1L * bY1L + w1L * bZ1L) - (u2 * bXL + v2 * bY2 + w2 * bZ2))
+               * sgn(bXL);
+    double e2R = e1R + sqrt(rho1R)
+               * ((u1R * bX1R + v1R * bY1R + w1R * bZ1R) - (u2 * bXR + v2 * bY2 + w2 * bZ2))
+               * sgn(bXR);
+    double pT2L = pT1L, pT2R = pT1R;
+
+    Flux oL = one_flux(rhoL, uL, vL, wL, bXL, bYL, bZL, eL, pTL);
+    Flux mL = one_flux(rho1L, u1L, v1L, w1L, bXL, bY1L, bZ1L, e1L, pT1L);
+    Flux iL = one_flux(rho2L, u2, v2, w2, bXL, bY2, bZ2, e2L, pT2L);
+    Flux oR = one_flux(rhoR, uR, vR, wR, bXR, bYR, bZR, eR, pTR);
+    Flux mR = one_flux(rho1R, u1R, v1R, w1R, bXR, bY1R, bZ1R, e1R, pT1R);
+    Flux iR = one_flux(rho2R, u2, v2, w2, bXR, bY2, bZ2, e2R, pT2R);
+
+    double w_oL = (double)(SL > 0.0);
+    double w_mL = (double)((SL <= 0.0) && (0.0 < S1L));
+    double w_iL = (double)((S1L <= 0.0) && (0.0 < SM));
+    double w_oR = (double)(SR <= 0.0);
+    double w_mR = (double)((S1R <= 0.0) && (0.0 < SR));
+    double w_iR = (double)((SM <= 0.0) && (0.0 < S1R));
+#define SEL(c) (oL.c * w_oL + mL.c * w_mL + iL.c * w_iL + oR.c * w_oR + mR.c * w_mR + iR.c * w_iR)
+    Flux f;
+    f.f0 = SEL(f0); f.f1 = SEL(f1); f.f2 = SEL(f2); f.f3 = SEL(f3);
+    f.f4 = SEL(f4); f.f5 = SEL(f5); f.f6 = SEL(f6); f.f7 = SEL(f7);
+#undef SEL
+    return f;
+}
+
+void oracle_hlld_flux(const double *left8, const double *right8, double gamma, double eps, double *flux8)
+{
+    Prim L, R;
+    memcpy(&L, left8, sizeof L);
+    memcpy(&R, right8, sizeof R);
+    Flux f = hlld_flux(&L, &R, gamma, eps);
+    memcpy(flux8, &f, sizeof f);
+}
+
+/* ------------------------------------------------------ flux assembly */
+/* HLLD::setQX + calculateFluxF: P/hlld.cu:110-131,236-243 with the ranges
+ * of P/calculate_half_Q.cu:41-53 and P/muscl.cu:32-42,88-102. */
+static void hlld_flux_x(struct oracle_sim *s, const Cons *U)
+{
+    int N = s->N, ny = s->ny;
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < N - ny; ++k)
+        s->dQCenter[k] = basic_param(&U[k], &U[k + ny], s->gamma);
+    #pragma omp parallel for schedule(static)
+    for (int k = ny; k < N - ny; ++k)
+        s->dQLeft[k] = left_param(&s->dQCenter[k - ny], &s->dQCenter[k], &s->dQCenter[k + ny], s->eps);
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < N - 2 * ny; ++k)
+        s->dQRight[k] = right_param(&s->dQCenter[k], &s->dQCenter[k + ny], &s->dQCenter[k + 2 * ny], s->eps);
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < N; ++k)
+        s->hlldFlux[k] = hlld_flux(&s->dQLeft[k], &s->dQRight[k], s->gamma, s->eps);
+}
+
+/* HLLD::calculateFluxG: P/hlld.cu:134-233.  The state is rotated
+ * (u,v,w)->(v,w,u), (Bx,By,Bz)->(By,Bz,Bx), swept with stride 1 (which wraps
+ * across columns at j = 0 and ny-1, as in the reference), and the flux is
+ * rotated back. */
+static void hlld_flux_y(struct oracle_sim *s, const Cons *U)
+{
+    int N = s->N;
+    Cons *t = s->tmpUForFluxG;
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < N; ++k) {                         /* P/hlld.cu:134-152 */
+        t[k].rho = U[k].rho;
+        t[k].rhoU = U[k].rhoV; t[k].rhoV = U[k].rhoW; t[k].rhoW = U[k].rhoU;
+        t[k].bX = U[k].bY; t[k].bY = U[k].bZ; t[k].bZ = U[k].bX;
+        t[k].e = U[k].e;
+    }
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < N - 1; ++k)                       /* P/calculate_half_Q.cu:56-67 */
+        s->dQCenter[k] = basic_param(&t[k], &t[k + 1], s->gamma);
+    #pragma omp parallel for schedule(static)
+    for (int k = 1; k < N - 1; ++k)                       /* P/muscl.cu:45-59 */
+        s->dQLeft[k] = left_param(&s->dQCenter[k - 1], &s->dQCenter[k], &s->dQCenter[k + 1], s->eps);
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < N - 2; ++k)                       /* P/muscl.cu:106-126 */
+        s->dQRight[k] = right_param(&s->dQCenter[k], &s->dQCenter[k + 1], &s->dQCenter[k + 2], s->eps);
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < N; ++k) {
+        Flux f = hlld_flux(&s->dQLeft[k], &s->dQRight[k], s->gamma, s->eps);
+        Flux g;                                          /* P/hlld.cu:170-194 */
+        g.f0 = f.f0;
+        g.f1 = f.f3; g.f2 = f.f1; g.f3 = f.f2;
+        g.f4 = f.f6; g.f5 = f.f4; g.f6 = f.f5;
+        g.f7 = f.f7;
+        s->hlldFlux[k] = g;
+    }
+}
+
+static inline double eta_at(const struct oracle_sim *s, int k)
+{
+    return s->etaField ? s->etaField[k] : s->eta;
+}
+
+/* Resistive eta*J flux added to F: EX/bhattacharjee/main_bhattacharjee.cu:142-182
+ * (identical formula EX/plasmoid/main_plasmoid.cu:175-224), centred B.
+ * Pinned by the linear-B closed form test. */
+static void resistive_x(struct oracle_sim *s, const Cons *U, Flux *flux)
+{
+    int nx = s->nx, ny = s->ny;
+    double dx = s->dx, dy = s->dy;
+    #pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx - 2; ++i) {
+        for (int j = 1; j < ny - 2; ++j) {
+            int k = j + i * ny;
+            double jY, jZ, eta;
+            double etaJY, etaJYPlus1, etaJZ, etaJZPlus1;
+            double etaJYBZ, etaJYBZPlus1, etaJZBY, etaJZBYPlus1;
+            jY = -(U[j + (i + 1) * ny].bZ - U[j + (i - 1) * ny].bZ) / (2.0 * dx);
+            jZ = (U[j + (i + 1) * ny].bY - U[j + (i - 1) * ny].bY) / (2.0 * dx)
+               - (U[j + 1 + i * ny].bX - U[j - 1 + i * ny].bX) / (2.0 * dy);
+            eta = eta_at(s, k);
+            etaJY = eta * jY;
+            etaJZ = eta * jZ;
+            etaJYBZ = etaJY * U[j + i * ny].bZ;
+            etaJZBY = etaJZ * U[j + i * ny].bY;
+            jY = -(U[j + (i + 2) * ny].bZ - U[j + i * ny].bZ) / (2.0 * dx);
+            jZ = (U[j + (i + 2) * ny].bY - U[j + i * ny].bY) / (2.0 * dx)
+               - (U[j + 1 + (i + 1) * ny].bX - U[j - 1 + (i + 1) * ny].bX) / (2.0 * dy);
+            eta = eta_at(s, k + ny);
+            etaJYPlus1 = eta * jY;
+            etaJZPlus1 = eta * jZ;
+            etaJYBZPlus1 = etaJYPlus1 * U[j + (i + 1) * ny].bZ;
+            etaJZBYPlus1 = etaJZPlus1 * U[j + (i + 1) * ny].bY;
+            flux[k].f5 -= 0.5 * (etaJZ + etaJZPlus1);
+            flux[k].f6 += 0.5 * (etaJY + etaJYPlus1);
+            flux[k].f7 += 0.5 * (etaJYBZ + etaJYBZPlus1)
+                        - 0.5 * (etaJZBY + etaJZBYPlus1);
+        }
+    }
+}
+
+/* Resistive eta*J flux added to G: EX/bhattacharjee/main_bhattacharjee.cu:201-241 */
+static void resistive_y(struct oracle_sim *s, const Cons *U, Flux *flux)
+{
+    int nx = s->nx, ny = s->ny;
+    double dx = s->dx, dy = s->dy;
+    #pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx - 2; ++i) {
+        for (int j = 1; j < ny - 2; ++j) {
+            int k = j + i * ny;
+            double jX, jZ, eta;
+            double etaJX, etaJXPlus1, etaJZ, etaJZPlus1;
+            double etaJZBX, etaJZBXPlus1, etaJXBZ, etaJXBZPlus1;
+            jX = (U[j + 1 + i * ny].bZ - U[j - 1 + i * ny].bZ) / (2.0 * dy);
+            jZ = (U[j + (i + 1) * ny].bY - U[j + (i - 1) * ny].bY) / (2.0 * dx)
+               - (U[j + 1 + i * ny].bX - U[j - 1 + i * ny].bX) / (2.0 * dy);
+            eta = eta_at(s, k);
+            etaJX = eta * jX;
+            etaJZ = eta * jZ;
+            etaJXBZ = etaJX * U[j + i * ny].bZ;
+            etaJZBX = etaJZ * U[j + i * ny].bX;
+            jX = (U[j + 2 + i * ny].bZ - U[j + i * ny].bZ) / (2.0 * dy);
+            jZ = (U[j + 1 + (i + 1) * ny].bY - U[j + 1 + (i - 1) * ny].bY) / (2.0 * dx)
+               - (U[j + 2 + i * ny].bX - U[j + i * ny].bX) / (2.0 * dy);
+            eta = eta_at(s, k + 1);
+            etaJXPlus1 = eta * jX;
+            etaJZPlus1 = eta * jZ;
+            etaJXBZPlus1 = etaJXPlus1 * U[j + 1 + i * ny].bZ;
+            etaJZBXPlus1 = etaJZPlus1 * U[j + 1 + i * ny].bX;
+            flux[k].f4 += 0.5 * (etaJZ + etaJZPlus1);
+            flux[k].f6 -= 0.5 * (etaJX + etaJXPlus1);
+            flux[k].f7 += 0.5 * (etaJZBX + etaJZBXPlus1)
+                        - 0.5 * (etaJXBZ + etaJXBZPlus1);
+        }
+    }
+}
+
+/* FluxSolver::getFluxF / getFluxG: P/flux_solver.cu:10-33 */
+static void get_flux_f(struct oracle_sim *s, const Cons *U, Flux *out)
+{
+    hlld_flux_x(s, U);
+    memcpy(s->solverFlux, s->hlldFlux, sizeof(Flux) * s->N);
+    resistive_x(s, U, s->solverFlux);
+    memcpy(out, s->solverFlux, sizeof(Flux) * s->N);
+}
+
+static void get_flux_g(struct oracle_sim *s, const Cons *U, Flux *out)
+{
+    hlld_flux_y(s, U);
+    memcpy(s->solverFlux, s->hlldFlux, sizeof(Flux) * s->N);
+    resistive_y(s, U, s->solverFlux);
+    memcpy(out, s->solverFlux, sizeof(Flux) * s->N);
+}
+
+/* ------------------------------------------- face <-> centre averaging */
+/* shiftUToCenterForCT: P/resistiveMHD_2D.cu:313-368 */
+static void shift_to_center(struct oracle_sim *s, Cons *U)
+{
+    int nx = s->nx, ny = s->ny;
+    double *tmp = s->tmpVector;
+    for (int k = 0; k < s->N; ++k) tmp[k] = U[k].bX;
+    #pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            U[j + i * ny].bX = 0.5 * (tmp[j + i * ny] + tmp[j + (i - 1) * ny]);
+    for (int k = 0; k < s->N; ++k) tmp[k] = U[k].bY;
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 1; j < ny; ++j)
+            U[j + i * ny].bY = 0.5 * (tmp[j + i * ny] + tmp[j - 1 + i * ny]);
+}
+
+/* backUToCenterHalfForCT: P/resistiveMHD_2D.cu:371-426 */
+static void back_to_center_half(struct oracle_sim *s, Cons *U)
+{
+    int nx = s->nx, ny = s->ny;
+    double *tmp = s->tmpVector;
+    for (int k = 0; k < s->N; ++k) tmp[k] = U[k].bX;
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx - 1; ++i)
+        for (int j = 0; j < ny; ++j)
+            U[j + i * ny].bX = 0.5 * (tmp[j + i * ny] + tmp[j + (i + 1) * ny]);
+    for (int k = 0; k < s->N; ++k) tmp[k] = U[k].bY;
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny - 1; ++j)
+            U[j + i * ny].bY = 0.5 * (tmp[j + i * ny] + tmp[j + 1 + i * ny]);
+}
+
+/* ------------------------------------------------------------------ CT */
+/* CT::divBClean: P/ct.cu:22-85 (getEZVector_kernel, CT_kernel).  The EMF
+ * array persists between calls; entries at i = nx-1 or j = ny-1 are never
+ * written (they stay 0 as the reference's zero-initialised device_vector). */
+static void div_b_clean(struct oracle_sim *s, const double *bXOld, const double *bYOld, Cons *U)
+{
+    int nx = s->nx, ny = s->ny;
+    const Flux *F = s->oldFluxF, *G = s->oldFluxG;
+    double *eZ = s->eZVector;
+    double dt = s->dt;
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx - 1; ++i) {
+        for (int j = 0; j < ny - 1; ++j) {
+            double eZG1 = G[j + i * ny].f4;
+            double eZG2 = G[j + (i + 1) * ny].f4;
+            double eZF1 = -1.0 * F[j + i * ny].f5;
+            double eZF2 = -1.0 * F[j + 1 + i * ny].f5;
+            eZ[j + i * ny] = 0.25 * (eZG1 + eZG2 + eZF1 + eZF2);
+        }
+    }
+    #pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx; ++i) {
+        for (int j = 1; j < ny; ++j) {
+            int k = j + i * ny;
+            U[k].bX = bXOld[k] - dt / s->dy * (eZ[k] - eZ[k - 1]);
+            U[k].bY = bYOld[k] + dt / s->dx * (eZ[k] - eZ[k - ny]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------ boundary */
+static void copy_cell(Cons *dst, const Cons *src) { *dst = *src; }
+
+/* P/boundary.cu:4-71 periodicBoundaryX2nd */
+static void periodic_x(struct oracle_sim *s, Cons *U)
+{
+    int nx = s->nx, ny = s->ny;
+    for (int j = 0; j < ny; ++j) {
+        copy_cell(&U[j + 0 * ny], &U[j + (nx - 6) * ny]);
+        copy_cell(&U[j + 1 * ny], &U[j + (nx - 5) * ny]);
+        copy_cell(&U[j + 2 * ny], &U[j + (nx - 4) * ny]);
+        copy_cell(&U[j + (nx - 3) * ny], &U[j + 3 * ny]);
+        copy_cell(&U[j + (nx - 2) * ny], &U[j + 4 * ny]);
+        copy_cell(&U[j + (nx - 1) * ny], &U[j + 5 * ny]);
+    }
+}
+
+/* P/boundary.cu:74-141 periodicBoundaryY2nd */
+static void periodic_y(struct oracle_sim *s, Cons *U)
+{
+    int nx = s->nx, ny = s->ny;
+    for (int i = 0; i < nx; ++i) {
+        copy_cell(&U[0 + i * ny], &U[ny - 6 + i * ny]);
+        copy_cell(&U[1 + i * ny], &U[ny - 5 + i * ny]);
+        copy_cell(&U[2 + i * ny], &U[ny - 4 + i * ny]);
+        copy_cell(&U[ny - 3 + i * ny], &U[3 + i * ny]);
+        copy_cell(&U[ny - 2 + i * ny], &U[4 + i * ny]);
+        copy_cell(&U[ny - 1 + i * ny], &U[5 + i * ny]);
+    }
+}
+
+static void mirror_no_bx(Cons *dst, const Cons *src)
+{
+    dst->rho = src->rho; dst->rhoU = src->rhoU; dst->rhoV = src->rhoV; dst->rhoW = src->rhoW;
+    dst->bY = src->bY; dst->bZ = src->bZ; dst->e = src->e;
+}
+
+static void mirror_no_by(Cons *dst, const Cons *src)
+{
+    dst->rho = src->rho; dst->rhoU = src->rhoU; dst->rhoV = src->rhoV; dst->rhoW = src->rhoW;
+    dst->bX = src->bX; dst->bZ = src->bZ; dst->e = src->e;
+}
+
+/* S/boundary.cu:4-104 symmetricBoundaryX2nd: mirror everything but bX (the
+ * normal momentum is not flipped), then rebuild the ghost bX from div B = 0. */
+static void symmetric_x(struct oracle_sim *s, Cons *U)
+{
+    int nx = s->nx, ny = s->ny;
+    double dx = s->dx, dy = s->dy;
+    for (int j = 0; j < ny; ++j) {
+        mirror_no_bx(&U[j + 0 * ny], &U[j + 5 * ny]);
+        mirror_no_bx(&U[j + 1 * ny], &U[j + 4 * ny]);
+        mirror_no_bx(&U[j + 2 * ny], &U[j + 3 * ny]);
+        mirror_no_bx(&U[j + (nx - 1) * ny], &U[j + (nx - 6) * ny]);
+        mirror_no_bx(&U[j + (nx - 2) * ny], &U[j + (nx - 5) * ny]);
+        mirror_no_bx(&U[j + (nx - 3) * ny], &U[j + (nx - 4) * ny]);
+    }
+    for (int j = 0; j < ny; ++j) {
+        if (0 < j) {
+            U[j + 2 * ny].bX = U[j + 3 * ny].bX + (U[j + 3 * ny].bY - U[j - 1 + 3 * ny].bY) / dy * dx;
+            U[j + 1 * ny].bX = U[j + 2 * ny].bX + (U[j + 2 * ny].bY - U[j - 1 + 2 * ny].bY) / dy * dx;
+            U[j + 0 * ny].bX = U[j + 1 * ny].bX + (U[j + 1 * ny].bY - U[j - 1 + 1 * ny].bY) / dy * dx;
+            U[j + (nx - 3) * ny].bX = U[j + (nx - 4) * ny].bX
+                - (U[j + (nx - 3) * ny].bY - U[j - 1 + (nx - 3) * ny].bY) / dy * dx;
+            U[j + (nx - 2) * ny].bX = U[j + (nx - 3) * ny].bX
+                - (U[j + (nx - 2) * ny].bY - U[j - 1 + (nx - 2) * ny].bY) / dy * dx;
+            U[j + (nx - 1) * ny].bX = U[j + (nx - 2) * ny].bX
+                - (U[j + (nx - 1) * ny].bY - U[j - 1 + (nx - 1) * ny].bY) / dy * dx;
+        }
+        if (j == 0) {
+            U[j + 2 * ny].bX = U[j + 3 * ny].bX;
+            U[j + 1 * ny].bX = U[j + 2 * ny].bX;
+            U[j + 0 * ny].bX = U[j + 1 * ny].bX;
+            U[j + (nx - 3) * ny].bX = U[j + (nx - 4) * ny].bX;
+            U[j + (nx - 2) * ny].bX = U[j + (nx - 3) * ny].bX;
+            U[j + (nx - 1) * ny].bX = U[j + (nx - 2) * ny].bX;
+        }
+    }
+}
+
+/* S/boundary.cu:107-205 symmetricBoundaryY2nd */
+static void symmetric_y(struct oracle_sim *s, Cons *U)
+{
+    int nx = s->nx, ny = s->ny;
+    double dx = s->dx, dy = s->dy;
+    for (int i = 0; i < nx; ++i) {
+        mirror_no_by(&U[0 + i * ny], &U[5 + i * ny]);
+        mirror_no_by(&U[1 + i * ny], &U[4 + i * ny]);
+        mirror_no_by(&U[2 + i * ny], &U[3 + i * ny]);
+        mirror_no_by(&U[ny - 1 + i * ny], &U[ny - 6 + i * ny]);
+        mirror_no_by(&U[ny - 2 + i * ny], &U[ny - 5 + i * ny]);
+        mirror_no_by(&U[ny - 3 + i * ny], &U[ny - 4 + i * ny]);
+    }
+    for (int i = 0; i < nx; ++i) {
+        if (0 < i) {
+            U[2 + i * ny].bY = U[3 + i * ny].bY + (U[3 + i * ny].bX - U[3 + (i - 1) * ny].bX) / dx * dy;
+            U[1 + i * ny].bY = U[2 + i * ny].bY + (U[2 + i * ny].bX - U[2 + (i - 1) * ny].bX) / dx * dy;
+            U[0 + i * ny].bY = U[1 + i * ny].bY + (U[1 + i * ny].bX - U[1 + (i - 1) * ny].bX) / dx * dy;
+            U[ny - 3 + i * ny].bY = U[ny - 4 + i * ny].bY
+                - (U[ny - 3 + i * ny].bX - U[ny - 3 + (i - 1) * ny].bX) / dx * dy;
+            U[ny - 2 + i * ny].bY = U[ny - 3 + i * ny].bY
+                - (U[ny - 2 + i * ny].bX - U[ny - 2 + (i - 1) * ny].bX) / dx * dy;
+            U[ny - 1 + i * ny].bY = U[ny - 2 + i * ny].bY
+                - (U[ny - 1 + i * ny].bX - U[ny - 1 + (i - 1) * ny].bX) / dx * dy;
+        }
+        if (i == 0) {
+            U[2 + i * ny].bY = U[3 + i * ny].bY;
+            U[1 + i * ny].bY = U[2 + i * ny].bY;
+            U[0 + i * ny].bY = U[1 + i * ny].bY;
+            U[ny - 3 + i * ny].bY = U[ny - 4 + i * ny].bY;
+            U[ny - 2 + i * ny].bY = U[ny - 3 + i * ny].bY;
+            U[ny - 1 + i * ny].bY = U[ny - 2 + i * ny].bY;
+        }
+    }
+}
+
+/* Boundary order: x first, then y over full columns (which fills the
+ * corners), as P/resistiveMHD_2D.cu:171-172 and, for periodic x with
+ * symmetric y, F32/resistiveMHD_2D.cu:239-240,257-258. */
+static void apply_bc(struct oracle_sim *s, Cons *U)
+{
+    if (s->bc_x == ORACLE_BC_PERIODIC) periodic_x(s, U); else symmetric_x(s, U);
+    if (s->bc_y == ORACLE_BC_PERIODIC) periodic_y(s, U); else symmetric_y(s, U);
+}
+
+/* ------------------------------------------------------------------ dt */
+/* calculateDtFunctor: P/resistiveMHD_2D.cu:247-274 (stored face B is used as
+ * if centred).  Pinned by the uniform-state closed form. */
+static inline double cell_dt(const Cons *c, double gamma, double dx, double dy, double eps)
+{
+    double rho = c->rho;
+    double u = c->rhoU / rho, v = c->rhoV / rho, w = c->rhoW / rho;
+    double bX = c->bX, bY = c->bY, bZ = c->bZ, e = c->e;
+    double p = (gamma - 1.0)
+             * (e - 0.5 * rho * (u * u + v * v + w * w)
+             - 0.5 * (bX * bX + bY * bY + bZ * bZ));
+    double cs = sqrt(gamma * p / rho);
+    double ca = sqrt((bX * bX + bY * bY + bZ * bZ) / rho);
+    double maxSpeedX = fabs(u) + sqrt(cs * cs + ca * ca);
+    double maxSpeedY = fabs(v) + sqrt(cs * cs + ca * ca);
+    return 1.0 / (maxSpeedX / dx + maxSpeedY / dy + eps);
+}
+
+/* ResistiveMHD2D::calculateDt: P/resistiveMHD_2D.cu:277-291, with
+ * thrust::min_element's sequential semantics (first element, replaced only
+ * by a strictly smaller one). */
+static double calculate_dt(struct oracle_sim *s)
+{
+    #pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->N; ++k)
+        s->dtVector[k] = cell_dt(&s->U[k], s->gamma, s->dx, s->dy, s->eps);
+    double m = s->dtVector[0];
+    for (int k = 1; k < s->N; ++k)
+        if (s->dtVector[k] < m) m = s->dtVector[k];
+    double lim = fmin(s->dx * s->dx / s->eta, s->dy * s->dy / s->eta);
+    s->dt = fmin(m, lim) * s->cfl;
+    return s->dt;
+}
+
+/* ----------------------------------------------------------- RK stages */
+/* oneStepFirstFunctor: P/resistiveMHD_2D.cu:53-86 */
+static inline Cons stage_first(const Cons *U, const Flux *F, const Flux *Fm, const Flux *G, const Flux *Gm,
+                               double dt, double dx, double dy)
+{
+    Cons n;
+    n.rho  = U->rho  - dt / dx * (F->f0 - Fm->f0) - dt / dy * (G->f0 - Gm->f0);
+    n.rhoU = U->rhoU - dt / dx * (F->f1 - Fm->f1) - dt / dy * (G->f1 - Gm->f1);
+    n.rhoV = U->rhoV - dt / dx * (F->f2 - Fm->f2) - dt / dy * (G->f2 - Gm->f2);
+    n.rhoW = U->rhoW - dt / dx * (F->f3 - Fm->f3) - dt / dy * (G->f3 - Gm->f3);
+    n.bX   = U->bX   - dt / dx * (F->f4 - Fm->f4) - dt / dy * (G->f4 - Gm->f4);
+    n.bY   = U->bY   - dt / dx * (F->f5 - Fm->f5) - dt / dy * (G->f5 - Gm->f5);
+    n.bZ   = U->bZ   - dt / dx * (F->f6 - Fm->f6) - dt / dy * (G->f6 - Gm->f6);
+    n.e    = U->e    - dt / dx * (F->f7 - Fm->f7) - dt / dy * (G->f7 - Gm->f7);
+    return n;
+}
+
+/* oneStepSecondFunctor: P/resistiveMHD_2D.cu:88-130 */
+static inline Cons stage_second(const Cons *U, const Cons *B, const Flux *F, const Flux *Fm,
+                                const Flux *G, const Flux *Gm, double dt, double dx, double dy)
+{
+    Cons n;
+    n.rho  = 0.5 * (U->rho  + B->rho  - dt / dx * (F->f0 - Fm->f0) - dt / dy * (G->f0 - Gm->f0));
+    n.rhoU = 0.5 * (U->rhoU + B->rhoU - dt / dx * (F->f1 - Fm->f1) - dt / dy * (G->f1 - Gm->f1));
+    n.rhoV = 0.5 * (U->rhoV + B->rhoV - dt / dx * (F->f2 - Fm->f2) - dt / dy * (G->f2 - Gm->f2));
+    n.rhoW = 0.5 * (U->rhoW + B->rhoW - dt / dx * (F->f3 - Fm->f3) - dt / dy * (G->f3 - Gm->f3));
+    n.bX   = 0.5 * (U->bX   + B->bX   - dt / dx * (F->f4 - Fm->f4) - dt / dy * (G->f4 - Gm->f4));
+    n.bY   = 0.5 * (U->bY   + B->bY   - dt / dx * (F->f5 - Fm->f5) - dt / dy * (G->f5 - Gm->f5));
+    n.bZ   = 0.5 * (U->bZ   + B->bZ   - dt / dx * (F->f6 - Fm->f6) - dt / dy * (G->f6 - Gm->f6));
+    n.e    = 0.5 * (U->e    + B->e    - dt / dx * (F->f7 - Fm->f7) - dt / dy * (G->f7 - Gm->f7));
+    return n;
+}
+
+/* ResistiveMHD2D::oneStepRK2: P/resistiveMHD_2D.cu:133-195 (S/ identical
+ * but for the boundary calls).  Note the second CT call reuses the stage-1
+ * fluxes saved by setOldFlux2D (:167-168, :190). */
+static void one_step_rk2(struct oracle_sim *s)
+{
+    int N = s->N, ny = s->ny;
+    Cons *U = s->U, *UBar = s->UBar;
+    for (int k = 0; k < N; ++k) { s->bXOld[k] = U[k].bX; s->bYOld[k] = U[k].bY; }
+    memcpy(UBar, U, sizeof(Cons) * N);
+
+    calculate_dt(s);
+
+    shift_to_center(s, U);
+    get_flux_f(s, U, s->fluxF);
+    get_flux_g(s, U, s->fluxG);
+    back_to_center_half(s, U);
+
+    #pragma omp parallel for schedule(static)
+    for (int k = ny; k < N; ++k)
+        UBar[k] = stage_first(&U[k], &s->fluxF[k], &s->fluxF[k - ny], &s->fluxG[k], &s->fluxG[k - 1],
+                              s->dt, s->dx, s->dy);
+
+    memcpy(s->oldFluxF, s->fluxF, sizeof(Flux) * N);     /* CT::setOldFlux2D, P/ct.cu:12-19 */
+    memcpy(s->oldFluxG, s->fluxG, sizeof(Flux) * N);
+    div_b_clean(s, s->bXOld, s->bYOld, UBar);
+
+    apply_bc(s, UBar);
+
+    shift_to_center(s, UBar);
+    get_flux_f(s, UBar, s->fluxF);
+    get_flux_g(s, UBar, s->fluxG);
+    back_to_center_half(s, UBar);
+
+    #pragma omp parallel for schedule(static)
+    for (int k = ny; k < N; ++k)
+        U[k] = stage_second(&U[k], &UBar[k], &s->fluxF[k], &s->fluxF[k - ny], &s->fluxG[k], &s->fluxG[k - 1],
+                            s->dt, s->dx, s->dy);
+
+    div_b_clean(s, s->bXOld, s->bYOld, U);
+
+    apply_bc(s, U);
+    s->time += s->dt;
+    s->steps += 1;
+}
+
+/* -------------------------------------------------------------- C API */
+struct oracle_sim *oracle_create(int nx, int ny, double dx, double dy, double gamma, double cfl,
+                                 double eps, double eta, int bc_x, int bc_y)
+{
+    if (nx < 12 || ny < 12) return NULL;
+    struct oracle_sim *s = (struct oracle_sim *)calloc(1, sizeof *s);
+    if (!s) return NULL;
+    s->nx = nx; s->ny = ny; s->N = nx * ny;
+    s->dx = dx; s->dy = dy; s->gamma = gamma; s->cfl = cfl; s->eps = eps; s->eta = eta;
+    s->bc_x = bc_x; s->bc_y = bc_y;
+    size_t N = (size_t)s->N;
+    /* zero-initialised, as thrust::device_vector of these structs */
+    s->U = calloc(N, sizeof(Cons)); s->UBar = calloc(N, sizeof(Cons));
+    s->fluxF = calloc(N, sizeof(Flux)); s->fluxG = calloc(N, sizeof(Flux));
+    s->dtVector = calloc(N, sizeof(double)); s->bXOld = calloc(N, sizeof(double));
+    s->bYOld = calloc(N, sizeof(double)); s->tmpVector = calloc(N, sizeof(double));
+    s->solverFlux = calloc(N, sizeof(Flux)); s->hlldFlux = calloc(N, sizeof(Flux));
+    s->dQCenter = calloc(N, sizeof(Prim)); s->dQLeft = calloc(N, sizeof(Prim)); s->dQRight = calloc(N, sizeof(Prim));
+    s->tmpUForFluxG = calloc(N, sizeof(Cons));
+    s->oldFluxF = calloc(N, sizeof(Flux)); s->oldFluxG = calloc(N, sizeof(Flux));
+    s->eZVector = calloc(N, sizeof(double));
+    if (!s->U || !s->UBar || !s->fluxF || !s->fluxG || !s->dtVector || !s->bXOld || !s->bYOld ||
+        !s->tmpVector || !s->solverFlux || !s->hlldFlux || !s->dQCenter || !s->dQLeft ||
+        !s->dQRight || !s->tmpUForFluxG || !s->oldFluxF || !s->oldFluxG || !s->eZVector) {
+        oracle_destroy(s);
+        return NULL;
+    }
+    return s;
+}
+
+void oracle_destroy(struct oracle_sim *s)
+{
+    if (!s) return;
+    free(s->U); free(s->UBar); free(s->fluxF); free(s->fluxG); free(s->dtVector); free(s->bXOld);
+    free(s->bYOld); free(s->tmpVector); free(s->solverFlux); free(s->hlldFlux); free(s->dQCenter);
+    free(s->dQLeft); free(s->dQRight); free(s->tmpUForFluxG); free(s->oldFluxF); free(s->oldFluxG);
+    free(s->eZVector); free(s->etaField);
+    free(s);
+}
+
+void oracle_set_state(struct oracle_sim *s, const double *aos) { memcpy(s->U, aos, sizeof(Cons) * s->N); }
+void oracle_get_state(const struct oracle_sim *s, double *aos) { memcpy(aos, s->U, sizeof(Cons) * s->N); }
+
+int oracle_set_eta_field(struct oracle_sim *s, const double *eta)
+{
+    if (!eta) { free(s->etaField); s->etaField = NULL; return 0; }
+    if (!s->etaField) s->etaField = (double *)malloc(sizeof(double) * s->N);
+    if (!s->etaField) return -1;
+    memcpy(s->etaField, eta, sizeof(double) * s->N);
+    return 0;
+}
+
+void oracle_apply_boundary(struct oracle_sim *s) { apply_bc(s, s->U); }
+
+void oracle_apply_boundary_axis(struct oracle_sim *s, int axis, int kind)
+{
+    if (axis == 0) { if (kind == ORACLE_BC_PERIODIC) periodic_x(s, s->U); else symmetric_x(s, s->U); }
+    else           { if (kind == ORACLE_BC_PERIODIC) periodic_y(s, s->U); else symmetric_y(s, s->U); }
+}
+
+double oracle_calculate_dt(struct oracle_sim *s) { return calculate_dt(s); }
+
+void oracle_step(struct oracle_sim *s, int nsteps) { for (int n = 0; n < nsteps; ++n) one_step_rk2(s); }
+
+double oracle_get_dt(const struct oracle_sim *s) { return s->dt; }
+double oracle_get_time(const struct oracle_sim *s) { return s->time; }
+
+/* checkCalculationIsCrashed: P/resistiveMHD_2D.cu:294-309 */
+int oracle_check_crashed(const struct oracle_sim *s)
+{
+    for (int k = 0; k < s->N; ++k) if (isnan(s->U[k].e)) return 1;
+    return 0;
+}
+
+/* Fluxes of the last stage-1 call (the ones CT used), for white-box tests. */
+void oracle_get_stage1_fluxes(const struct oracle_sim *s, double *F, double *G)
+{
+    memcpy(F, s->oldFluxF, sizeof(Flux) * s->N);
+    memcpy(G, s->oldFluxG, sizeof(Flux) * s->N);
+}
+
+/* Stand-alone flux evaluation on a given state (shift, F, G, back), for the
+ * HLLD/resistive white-box tests.  Leaves U unchanged except for the face
+ * smoothing that shift+back applies, exactly as the reference does. */
+void oracle_compute_fluxes(struct oracle_sim *s, double *aos_state, double *F, double *G)
+{
+    Cons *U = (Cons *)aos_state;
+    shift_to_center(s, U);
+    get_flux_f(s, U, (Flux *)F);
+    get_flux_g(s, U, (Flux *)G);
+    back_to_center_half(s, U);
+}
+
+double oracle_cell_dt(const double *cell8, double gamma, double dx, double dy, double eps)
+{
+    Cons c;
+    memcpy(&c, cell8, sizeof c);
+    return cell_dt(&c, gamma, dx, dy, eps);
+}

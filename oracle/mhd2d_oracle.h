@@ -1,0 +1,35 @@
+/* mhd2d_oracle.h -- C API of the CPU oracle (test infrastructure only). */
+#ifndef MHD2D_ORACLE_H
+#define MHD2D_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORACLE_BC_PERIODIC = 0, ORACLE_BC_SYMMETRIC = 1 };
+
+struct oracle_sim;
+
+/* nx, ny include the 3 ghost layers per side (reference convention). */
+struct oracle_sim *oracle_create(int nx, int ny, double dx, double dy, double gamma, double cfl,
+                                 double eps, double eta, int bc_x, int bc_y);
+void oracle_destroy(struct oracle_sim *s);
+void oracle_set_state(struct oracle_sim *s, const double *aos);
+void oracle_get_state(const struct oracle_sim *s, double *aos);
+int oracle_set_eta_field(struct oracle_sim *s, const double *eta);
+void oracle_apply_boundary(struct oracle_sim *s);
+void oracle_apply_boundary_axis(struct oracle_sim *s, int axis, int kind);
+double oracle_calculate_dt(struct oracle_sim *s);
+void oracle_step(struct oracle_sim *s, int nsteps);
+double oracle_get_dt(const struct oracle_sim *s);
+double oracle_get_time(const struct oracle_sim *s);
+int oracle_check_crashed(const struct oracle_sim *s);
+void oracle_get_stage1_fluxes(const struct oracle_sim *s, double *F, double *G);
+void oracle_compute_fluxes(struct oracle_sim *s, double *aos_state, double *F, double *G);
+double oracle_minmod(double x, double y, double eps);
+void oracle_hlld_flux(const double *left8, const double *right8, double gamma, double eps, double *flux8);
+double oracle_cell_dt(const double *cell8, double gamma, double dx, double dy, double eps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

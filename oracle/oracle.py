@@ -1,0 +1,159 @@
+"""ctypes wrapper of liboracle.so, the C restatement in mhd2d_oracle.c.
+
+Test infrastructure only: the product library never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+BC_PERIODIC, BC_SYMMETRIC = 0, 1
+
+_lib = None
+
+
+def build():
+    """Compiles liboracle.so with the recipe in this directory's Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    D = ctypes.POINTER(ctypes.c_double)
+    P = ctypes.c_void_p
+    c_d, c_i = ctypes.c_double, ctypes.c_int
+    sig = {
+        "oracle_create": (P, [c_i, c_i, c_d, c_d, c_d, c_d, c_d, c_d, c_i, c_i]),
+        "oracle_destroy": (None, [P]),
+        "oracle_set_state": (None, [P, D]),
+        "oracle_get_state": (None, [P, D]),
+        "oracle_set_eta_field": (c_i, [P, D]),
+        "oracle_apply_boundary": (None, [P]),
+        "oracle_apply_boundary_axis": (None, [P, c_i, c_i]),
+        "oracle_calculate_dt": (c_d, [P]),
+        "oracle_step": (None, [P, c_i]),
+        "oracle_get_dt": (c_d, [P]),
+        "oracle_get_time": (c_d, [P]),
+        "oracle_check_crashed": (c_i, [P]),
+        "oracle_get_stage1_fluxes": (None, [P, D, D]),
+        "oracle_compute_fluxes": (None, [P, D, D, D]),
+        "oracle_minmod": (c_d, [c_d, c_d, c_d]),
+        "oracle_hlld_flux": (None, [D, D, c_d, c_d, D]),
+        "oracle_cell_dt": (c_d, [D, c_d, c_d, c_d, c_d]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class OracleSim:
+    """The reference's ResistiveMHD2D on the CPU (same array shapes as the
+    HIP binding: (nx, ny, 8) float64, ghosts included)."""
+
+    def __init__(self, nx, ny, dx, dy, gamma=5.0 / 3.0, cfl=0.7, eps=1e-20, eta=0.0,
+                 bc_x=BC_PERIODIC, bc_y=BC_PERIODIC):
+        self._lib = load()
+        self.nx, self.ny = nx, ny
+        self.gamma, self.eps, self.dx, self.dy = gamma, eps, dx, dy
+        self._s = self._lib.oracle_create(nx, ny, dx, dy, gamma, cfl, eps, eta, bc_x, bc_y)
+        if not self._s:
+            raise ValueError("oracle_create failed (nx, ny must be >= 12)")
+
+    @classmethod
+    def from_config(cls, cfg):
+        return cls(cfg.nx, cfg.ny, cfg.dx, cfg.dy, cfg.gamma, cfg.cfl, cfg.eps, cfg.eta, cfg.bc_x, cfg.bc_y)
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            self._lib.oracle_destroy(self._s)
+            self._s = None
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (self.nx, self.ny, 8)
+        self._lib.oracle_set_state(self._s, _p(U))
+
+    def get_state(self):
+        out = np.empty((self.nx, self.ny, 8))
+        self._lib.oracle_get_state(self._s, _p(out))
+        return out
+
+    def set_eta_field(self, eta):
+        if eta is None:
+            self._lib.oracle_set_eta_field(self._s, None)
+            return
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        assert eta.shape == (self.nx, self.ny)
+        if self._lib.oracle_set_eta_field(self._s, _p(eta)) != 0:
+            raise MemoryError("oracle_set_eta_field")
+
+    def apply_boundary(self):
+        self._lib.oracle_apply_boundary(self._s)
+
+    def apply_boundary_axis(self, axis, kind):
+        self._lib.oracle_apply_boundary_axis(self._s, axis, kind)
+
+    def calculate_dt(self):
+        return self._lib.oracle_calculate_dt(self._s)
+
+    def step(self, n=1):
+        self._lib.oracle_step(self._s, int(n))
+
+    @property
+    def dt(self):
+        return self._lib.oracle_get_dt(self._s)
+
+    @property
+    def time(self):
+        return self._lib.oracle_get_time(self._s)
+
+    def crashed(self):
+        return bool(self._lib.oracle_check_crashed(self._s))
+
+    def stage1_fluxes(self):
+        F = np.empty((self.nx, self.ny, 8))
+        G = np.empty((self.nx, self.ny, 8))
+        self._lib.oracle_get_stage1_fluxes(self._s, _p(F), _p(G))
+        return F, G
+
+    def compute_fluxes(self, U):
+        """Fluxes F, G of state U (shift to centre, HLLD + resistive, back)."""
+        U = np.array(U, dtype=np.float64, order="C", copy=True)
+        F = np.empty((self.nx, self.ny, 8))
+        G = np.empty((self.nx, self.ny, 8))
+        self._lib.oracle_compute_fluxes(self._s, _p(U), _p(F), _p(G))
+        return F, G
+
+
+def minmod(x, y, eps=1e-20):
+    return load().oracle_minmod(float(x), float(y), float(eps))
+
+
+def hlld_flux(left, right, gamma=5.0 / 3.0, eps=1e-20):
+    """One HLLD interface flux; left/right = (rho, u, v, w, bX, bY, bZ, p)."""
+    L = np.ascontiguousarray(left, dtype=np.float64)
+    R = np.ascontiguousarray(right, dtype=np.float64)
+    out = np.empty(8)
+    load().oracle_hlld_flux(_p(L), _p(R), gamma, eps, _p(out))
+    return out
+
+
+def cell_dt(cell, gamma, dx, dy, eps=1e-20):
+    c = np.ascontiguousarray(cell, dtype=np.float64)
+    return load().oracle_cell_dt(_p(c), gamma, dx, dy, eps)

@@ -1,0 +1,442 @@
+// mhd2d_capi.hip -- host side of the C ABI declared in include/mhd2d.h:
+// device buffers, the per-step launch sequence and host transfers.
+//
+// Per RK2 step (the reference's ResistiveMHD2D::oneStepRK2,
+// ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_periodicXY/resistiveMHD_2D.cu:133-195):
+//   stage 1 (U -> U*, CT) ; boundary x, y of U*
+//   stage 2 (U*, U -> U in place) ; boundary x, y of U (+ dt of the next step)
+// dt never leaves the device: stage 2 and the boundary kernels reduce the
+// per-cell CFL limit of the new state into a control block, and the next
+// stage 1 turns it into dt.
+#include <hip/hip_runtime.h>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd2d.h"
+#include "mhd2d_kernels.h"
+
+using mhd::Ctrl;
+using mhd::Grid;
+using mhd::Phys;
+
+struct mhd_handle {
+    mhd_config cfg{};
+    Grid g{};
+    Phys ph{};
+    double *U = nullptr;      // state U^n (8 planes)
+    double *Us = nullptr;     // stage-1 state U*
+    double *eta = nullptr;    // optional per-cell resistivity plane
+    double *side = nullptr;   // CT value of the bottom boundary-face By per column
+    double *aos = nullptr;    // AoS staging for host transfers (allocated on first use)
+    Ctrl *ctrl = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int cur = 0;              // control slot holding the per-cell dt minimum of U
+    bool dt_valid = false;    // ctrl->acc[cur] matches the current U
+    int seg_request = 0;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(mhd_handle *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(mhd_handle *h, hipError_t e, const char *what)
+{
+    return fail(h, MHD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHECK(h, call)                                   \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #call);\
+    } while (0)
+
+constexpr int kDefaultSeg = 64;
+
+void layout(mhd_handle *h, int seg)
+{
+    Grid &g = h->g;
+    g.nx = h->cfg.nx;
+    g.ny = h->cfg.ny;
+    g.pitch = (h->cfg.ny + 7) & ~7;
+    g.plane = ((long long)g.nx * g.pitch + 63) & ~63LL;
+    g.seg = seg > 0 ? seg : kDefaultSeg;
+    g.nstrips = (h->cfg.ny - 2 * mhd::kGhost + mhd::kStripRows - 1) / mhd::kStripRows;
+    g.nsegs = (h->cfg.nx - 2 * mhd::kGhost + g.seg - 1) / g.seg;
+}
+
+void set_phys(mhd_handle *h)
+{
+    h->ph.gamma = h->cfg.gamma;
+    h->ph.dx = h->cfg.dx;
+    h->ph.dy = h->cfg.dy;
+    h->ph.eps = h->cfg.eps;
+    h->ph.cfl = h->cfg.cfl;
+    h->ph.eta = h->cfg.eta;
+}
+
+int check_config(const mhd_config *c, std::string &why)
+{
+    if (!c) { why = "config is NULL"; return 0; }
+    if (c->nx < 4 * MHD_GHOST || c->ny < 4 * MHD_GHOST) { why = "nx and ny must be >= 12 (6 interior cells)"; return 0; }
+    if ((long long)c->nx * c->ny > (1LL << 31)) { why = "nx*ny must fit in 2^31 cells"; return 0; }
+    if (!(c->dx > 0.0) || !(c->dy > 0.0)) { why = "dx and dy must be > 0"; return 0; }
+    if (!(c->gamma > 1.0)) { why = "gamma must be > 1"; return 0; }
+    if (!(c->cfl > 0.0)) { why = "cfl must be > 0"; return 0; }
+    if (!(c->eta >= 0.0)) { why = "eta must be >= 0"; return 0; }
+    if (!(c->eps >= 0.0)) { why = "eps must be >= 0"; return 0; }
+    if ((c->bc_x != MHD_BC_PERIODIC && c->bc_x != MHD_BC_SYMMETRIC) ||
+        (c->bc_y != MHD_BC_PERIODIC && c->bc_y != MHD_BC_SYMMETRIC)) { why = "bc_x/bc_y must be MHD_BC_PERIODIC or MHD_BC_SYMMETRIC"; return 0; }
+    return 1;
+}
+
+int ensure_aos(mhd_handle *h)
+{
+    if (h->aos) return MHD_OK;
+    const size_t bytes = sizeof(double) * 8 * (size_t)h->cfg.nx * h->cfg.ny;
+    HIPCHECK(h, hipMalloc(&h->aos, bytes));
+    return MHD_OK;
+}
+
+int apply_bc_pair(mhd_handle *h, double *u, Ctrl *ctrl, int slot)
+{
+    HIPCHECK(h, mhd::launch_bc_x(h->cfg.bc_x, u, ctrl, slot, h->g, h->ph, h->stream));
+    HIPCHECK(h, mhd::launch_bc_y(h->cfg.bc_y, u, ctrl, slot, h->g, h->ph, h->stream));
+    return MHD_OK;
+}
+
+int refresh_dt(mhd_handle *h)
+{
+    if (h->dt_valid) return MHD_OK;
+    HIPCHECK(h, mhd::launch_dt_all(h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
+    h->dt_valid = true;
+    return MHD_OK;
+}
+
+int read_ctrl(mhd_handle *h, Ctrl *out)
+{
+    HIPCHECK(h, hipMemcpyAsync(out, h->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return MHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mhd_last_error(const mhd_handle *h)
+{
+    return h ? h->err.c_str() : g_last_error.c_str();
+}
+
+int mhd_create(const mhd_config *cfg, mhd_handle **out)
+{
+    if (!out) return fail(nullptr, MHD_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    std::string why;
+    if (!check_config(cfg, why)) return fail(nullptr, MHD_ERR_INVALID, why);
+    mhd_handle *h = new (std::nothrow) mhd_handle();
+    if (!h) return fail(nullptr, MHD_ERR_NOMEM, "host allocation failed");
+    h->cfg = *cfg;
+    layout(h, 0);
+    set_phys(h);
+    hipError_t e;
+    if ((e = hipGetDevice(&h->device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&h->U, sizeof(double) * 8 * (size_t)h->g.plane)) != hipSuccess ||
+        (e = hipMalloc(&h->Us, sizeof(double) * 8 * (size_t)h->g.plane)) != hipSuccess ||
+        (e = hipMalloc(&h->side, sizeof(double) * (size_t)h->cfg.nx)) != hipSuccess ||
+        (e = hipMalloc(&h->ctrl, sizeof(Ctrl))) != hipSuccess) {
+        const std::string msg = std::string("mhd_create: ") + hipGetErrorString(e);
+        mhd_destroy(h);
+        return fail(nullptr, e == hipErrorOutOfMemory ? MHD_ERR_NOMEM : MHD_ERR_HIP, msg);
+    }
+    Ctrl c0{};
+    c0.acc[0] = c0.acc[1] = mhd::kInfBits;
+    if ((e = hipMemsetAsync(h->U, 0, sizeof(double) * 8 * (size_t)h->g.plane, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->Us, 0, sizeof(double) * 8 * (size_t)h->g.plane, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->side, 0, sizeof(double) * (size_t)h->cfg.nx, h->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(h->ctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess) {
+        const std::string msg = std::string("mhd_create: ") + hipGetErrorString(e);
+        mhd_destroy(h);
+        return fail(nullptr, MHD_ERR_HIP, msg);
+    }
+    *out = h;
+    return MHD_OK;
+}
+
+int mhd_destroy(mhd_handle *h)
+{
+    if (!h) return MHD_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(h->U);
+    (void)hipFree(h->Us);
+    (void)hipFree(h->eta);
+    (void)hipFree(h->side);
+    (void)hipFree(h->aos);
+    (void)hipFree(h->ctrl);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return MHD_OK;
+}
+
+int mhd_get_config(const mhd_handle *h, mhd_config *cfg)
+{
+    if (!h || !cfg) return fail(nullptr, MHD_ERR_INVALID, "NULL argument");
+    *cfg = h->cfg;
+    return MHD_OK;
+}
+
+int mhd_set_state(mhd_handle *h, const double *host_aos)
+{
+    if (!h || !host_aos) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    int rc = ensure_aos(h);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * 8 * (size_t)h->cfg.nx * h->cfg.ny;
+    HIPCHECK(h, hipMemcpyAsync(h->aos, host_aos, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, mhd::launch_unpack(h->aos, h->U, h->g, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    h->dt_valid = false;
+    return MHD_OK;
+}
+
+int mhd_get_state(mhd_handle *h, double *host_aos)
+{
+    if (!h || !host_aos) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    int rc = ensure_aos(h);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * 8 * (size_t)h->cfg.nx * h->cfg.ny;
+    HIPCHECK(h, mhd::launch_pack(h->U, h->aos, h->g, h->stream));
+    HIPCHECK(h, hipMemcpyAsync(host_aos, h->aos, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return MHD_OK;
+}
+
+int mhd_set_eta_field(mhd_handle *h, const double *host_eta)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if (!host_eta) {
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        HIPCHECK(h, hipFree(h->eta));
+        h->eta = nullptr;
+        return MHD_OK;
+    }
+    if (!h->eta) HIPCHECK(h, hipMalloc(&h->eta, sizeof(double) * (size_t)h->g.plane));
+    int rc = ensure_aos(h);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * (size_t)h->cfg.nx * h->cfg.ny;
+    HIPCHECK(h, hipMemcpyAsync(h->aos, host_eta, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, mhd::launch_scatter_plane(h->aos, h->eta, h->g, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return MHD_OK;
+}
+
+int mhd_set_eta(mhd_handle *h, double eta)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if (!(eta >= 0.0)) return fail(h, MHD_ERR_INVALID, "eta must be >= 0");
+    h->cfg.eta = eta;
+    set_phys(h);
+    return MHD_OK;
+}
+
+int mhd_apply_boundary(mhd_handle *h, int axis, int kind)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if ((axis != 0 && axis != 1) || (kind != MHD_BC_PERIODIC && kind != MHD_BC_SYMMETRIC))
+        return fail(h, MHD_ERR_INVALID, "axis must be 0/1 and kind MHD_BC_PERIODIC/MHD_BC_SYMMETRIC");
+    if (axis == 0)
+        HIPCHECK(h, mhd::launch_bc_x(kind, h->U, nullptr, 0, h->g, h->ph, h->stream));
+    else
+        HIPCHECK(h, mhd::launch_bc_y(kind, h->U, nullptr, 0, h->g, h->ph, h->stream));
+    h->dt_valid = false;
+    return MHD_OK;
+}
+
+int mhd_set_boundary(mhd_handle *h, int bc_x, int bc_y)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if ((bc_x != MHD_BC_PERIODIC && bc_x != MHD_BC_SYMMETRIC) || (bc_y != MHD_BC_PERIODIC && bc_y != MHD_BC_SYMMETRIC))
+        return fail(h, MHD_ERR_INVALID, "bc must be MHD_BC_PERIODIC or MHD_BC_SYMMETRIC");
+    h->cfg.bc_x = bc_x;
+    h->cfg.bc_y = bc_y;
+    return MHD_OK;
+}
+
+int mhd_calculate_dt(mhd_handle *h, double *dt)
+{
+    if (!h || !dt) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    HIPCHECK(h, mhd::launch_dt_all(h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
+    h->dt_valid = true;
+    Ctrl c;
+    int rc = read_ctrl(h, &c);
+    if (rc) return rc;
+    const double m = __builtin_bit_cast(double, c.acc[h->cur]);
+    const double lim = std::fmin(h->cfg.dx * h->cfg.dx / h->cfg.eta, h->cfg.dy * h->cfg.dy / h->cfg.eta);
+    *dt = std::fmin(m, lim) * h->cfg.cfl;
+    return MHD_OK;
+}
+
+int mhd_step_rk2(mhd_handle *h, int nsteps)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if (nsteps < 0) return fail(h, MHD_ERR_INVALID, "nsteps must be >= 0");
+    int rc = refresh_dt(h);
+    if (rc) return rc;
+    for (int n = 0; n < nsteps; ++n) {
+        HIPCHECK(h, mhd::launch_stage(1, h->U, h->Us, h->U, h->eta, h->side, h->ctrl, h->g, h->ph, h->cur, h->stream));
+        if ((rc = apply_bc_pair(h, h->Us, nullptr, 0))) return rc;
+        HIPCHECK(h, mhd::launch_stage(2, h->Us, h->U, h->U, h->eta, h->side, h->ctrl, h->g, h->ph, h->cur, h->stream));
+        if ((rc = apply_bc_pair(h, h->U, h->ctrl, h->cur ^ 1))) return rc;
+        h->cur ^= 1;
+    }
+    return MHD_OK;
+}
+
+int mhd_one_step_rk2(mhd_handle *h) { return mhd_step_rk2(h, 1); }
+
+int mhd_get_dt(mhd_handle *h, double *dt)
+{
+    if (!h || !dt) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    Ctrl c;
+    int rc = read_ctrl(h, &c);
+    if (rc) return rc;
+    *dt = c.dt;
+    return MHD_OK;
+}
+
+int mhd_get_time(mhd_handle *h, double *t)
+{
+    if (!h || !t) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    Ctrl c;
+    int rc = read_ctrl(h, &c);
+    if (rc) return rc;
+    *t = c.time;
+    return MHD_OK;
+}
+
+int mhd_set_time(mhd_handle *h, double t)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    Ctrl c;
+    int rc = read_ctrl(h, &c);
+    if (rc) return rc;
+    c.time = t;
+    HIPCHECK(h, hipMemcpyAsync(h->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return MHD_OK;
+}
+
+int mhd_check_crashed(mhd_handle *h, int *crashed)
+{
+    if (!h || !crashed) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    int rc = refresh_dt(h);   // a state set from outside is scanned for NaN here
+    if (rc) return rc;
+    Ctrl c;
+    rc = read_ctrl(h, &c);
+    if (rc) return rc;
+    *crashed = c.crashed ? 1 : 0;
+    return MHD_OK;
+}
+
+int mhd_save(mhd_handle *h, const char *dir, const char *name, int step)
+{
+    if (!h || !dir || !name) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    std::vector<double> buf;
+    try {
+        buf.resize(8 * (size_t)h->cfg.nx * h->cfg.ny);
+    } catch (...) {
+        return fail(h, MHD_ERR_NOMEM, "host allocation failed");
+    }
+    int rc = mhd_get_state(h, buf.data());
+    if (rc) return rc;
+    const std::string path = std::string(dir) + "/" + name + "_" + std::to_string(step) + ".bin";
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return fail(h, MHD_ERR_IO, "cannot open " + path + ": " + std::strerror(errno));
+    const size_t n = std::fwrite(buf.data(), sizeof(double), buf.size(), f);
+    const int closed = std::fclose(f);
+    if (n != buf.size() || closed != 0) return fail(h, MHD_ERR_IO, "short write to " + path);
+    return MHD_OK;
+}
+
+int mhd_load(mhd_handle *h, const char *path)
+{
+    if (!h || !path) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    std::vector<double> buf;
+    try {
+        buf.resize(8 * (size_t)h->cfg.nx * h->cfg.ny);
+    } catch (...) {
+        return fail(h, MHD_ERR_NOMEM, "host allocation failed");
+    }
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(h, MHD_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+    const size_t n = std::fread(buf.data(), sizeof(double), buf.size(), f);
+    const bool extra = std::fgetc(f) != EOF;
+    std::fclose(f);
+    if (n != buf.size() || extra)
+        return fail(h, MHD_ERR_IO, std::string(path) + ": size does not match nx*ny*8 doubles");
+    return mhd_set_state(h, buf.data());
+}
+
+int mhd_synchronize(mhd_handle *h)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return MHD_OK;
+}
+
+int mhd_init_orszag_tang(mhd_handle *h)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    std::vector<double> buf(8 * (size_t)h->cfg.nx * h->cfg.ny);
+    int rc = mhd_fill_orszag_tang(&h->cfg, buf.data());
+    if (rc) return fail(h, rc, "mhd_fill_orszag_tang failed");
+    if ((rc = mhd_set_state(h, buf.data()))) return rc;
+    if ((rc = mhd_apply_boundary(h, 0, h->cfg.bc_x))) return rc;
+    return mhd_apply_boundary(h, 1, h->cfg.bc_y);
+}
+
+int mhd_init_harris(mhd_handle *h, unsigned long long seed, double noise_amp, double eta0, double eta1)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    std::vector<double> buf(8 * (size_t)h->cfg.nx * h->cfg.ny);
+    int rc = mhd_fill_harris(&h->cfg, seed, noise_amp, buf.data());
+    if (rc) return fail(h, rc, "mhd_fill_harris failed");
+    if ((rc = mhd_set_state(h, buf.data()))) return rc;
+    std::vector<double> eta((size_t)h->cfg.nx * h->cfg.ny);
+    if ((rc = mhd_fill_harris_eta(&h->cfg, eta0, eta1, eta.data()))) return fail(h, rc, "mhd_fill_harris_eta failed");
+    if ((rc = mhd_set_eta_field(h, eta.data()))) return rc;
+    if ((rc = mhd_apply_boundary(h, 0, h->cfg.bc_x))) return rc;
+    return mhd_apply_boundary(h, 1, h->cfg.bc_y);
+}
+
+int mhd_set_sweep_length(mhd_handle *h, int columns)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if (columns < 0 || columns > 1 << 20) return fail(h, MHD_ERR_INVALID, "columns out of range");
+    layout(h, columns);
+    return MHD_OK;
+}
+
+int mhd_get_sweep_length(const mhd_handle *h, int *columns)
+{
+    if (!h || !columns) return fail(nullptr, MHD_ERR_INVALID, "NULL argument");
+    *columns = h->g.seg;
+    return MHD_OK;
+}
+
+}  // extern "C"

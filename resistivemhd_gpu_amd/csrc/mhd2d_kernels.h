@@ -1,0 +1,29 @@
+// mhd2d_kernels.h -- launch interface of the gfx950 kernels (internal to the
+// library; the public ABI is include/mhd2d.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "mhd2d_device.h"
+
+namespace mhd {
+
+constexpr int kGhost = 3;        // ghost layers per side
+constexpr int kStripRows = 60;   // interior rows a wavefront updates (64 lanes - 4 halo rows)
+
+struct Grid {
+    int nx, ny;          // cells per axis including ghosts
+    int pitch;           // doubles between consecutive x rows of a plane
+    long long plane;     // doubles between planes
+    int seg;             // columns one wavefront sweeps
+    int nstrips, nsegs;  // work decomposition: strips along y, segments along x
+};
+
+hipError_t launch_stage(int stage, const double *uin, double *uout, const double *uold, const double *eta,
+                        double *side, Ctrl *ctrl, const Grid &g, const Phys &ph, int cur, hipStream_t s);
+hipError_t launch_bc_x(int kind, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s);
+hipError_t launch_bc_y(int kind, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s);
+hipError_t launch_dt_all(const double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s);
+hipError_t launch_unpack(const double *aos, double *u, const Grid &g, hipStream_t s);
+hipError_t launch_pack(const double *u, double *aos, const Grid &g, hipStream_t s);
+hipError_t launch_scatter_plane(const double *src, double *dst, const Grid &g, hipStream_t s);
+
+}  // namespace mhd

@@ -1,0 +1,307 @@
+"""ctypes binding of libmhd2d.so (the C ABI in include/mhd2d.h) and a Python
+mirror of the reference's single-GPU class ``ResistiveMHD2D``
+(ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_periodicXY/resistiveMHD_2D.hpp:9-52).
+
+The HIP library is required: loading raises if it is missing or fails to
+load; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmhd2d.so")
+
+MHD_BC_PERIODIC = 0
+MHD_BC_SYMMETRIC = 1
+GHOST = 3
+NVAR = 8
+
+# every entry point include/mhd2d.h declares
+EXPORTS = (
+    "mhd_default_config", "mhd_create", "mhd_destroy", "mhd_last_error", "mhd_get_config",
+    "mhd_set_state", "mhd_get_state", "mhd_set_eta_field", "mhd_set_eta", "mhd_apply_boundary",
+    "mhd_set_boundary", "mhd_calculate_dt", "mhd_one_step_rk2", "mhd_step_rk2", "mhd_get_dt",
+    "mhd_get_time", "mhd_set_time", "mhd_check_crashed", "mhd_save", "mhd_load", "mhd_synchronize",
+    "mhd_fill_orszag_tang", "mhd_fill_harris", "mhd_fill_harris_eta", "mhd_init_orszag_tang",
+    "mhd_init_harris", "mhd_set_sweep_length", "mhd_get_sweep_length",
+)
+
+
+class MhdConfig(ctypes.Structure):
+    """struct mhd_config of include/mhd2d.h."""
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+        ("xmin", ctypes.c_double), ("ymin", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("eps", ctypes.c_double), ("eta", ctypes.c_double),
+        ("bc_x", ctypes.c_int), ("bc_y", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class MHDError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libmhd2d.so once and declares argument types."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MHDError(f"{path} is missing: build it with `python -m resistivemhd_gpu_amd.build`")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    C = ctypes.POINTER(MhdConfig)
+    I = ctypes.c_int
+    sig = {
+        "mhd_default_config": (I, [C, I, I, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "mhd_create": (I, [C, ctypes.POINTER(P)]),
+        "mhd_destroy": (I, [P]),
+        "mhd_last_error": (ctypes.c_char_p, [P]),
+        "mhd_get_config": (I, [P, C]),
+        "mhd_set_state": (I, [P, D]),
+        "mhd_get_state": (I, [P, D]),
+        "mhd_set_eta_field": (I, [P, D]),
+        "mhd_set_eta": (I, [P, ctypes.c_double]),
+        "mhd_apply_boundary": (I, [P, I, I]),
+        "mhd_set_boundary": (I, [P, I, I]),
+        "mhd_calculate_dt": (I, [P, D]),
+        "mhd_one_step_rk2": (I, [P]),
+        "mhd_step_rk2": (I, [P, I]),
+        "mhd_get_dt": (I, [P, D]),
+        "mhd_get_time": (I, [P, D]),
+        "mhd_set_time": (I, [P, ctypes.c_double]),
+        "mhd_check_crashed": (I, [P, ctypes.POINTER(I)]),
+        "mhd_save": (I, [P, ctypes.c_char_p, ctypes.c_char_p, I]),
+        "mhd_load": (I, [P, ctypes.c_char_p]),
+        "mhd_synchronize": (I, [P]),
+        "mhd_fill_orszag_tang": (I, [C, D]),
+        "mhd_fill_harris": (I, [C, ctypes.c_ulonglong, ctypes.c_double, D]),
+        "mhd_fill_harris_eta": (I, [C, ctypes.c_double, ctypes.c_double, D]),
+        "mhd_init_orszag_tang": (I, [P]),
+        "mhd_init_harris": (I, [P, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "mhd_set_sweep_length": (I, [P, I]),
+        "mhd_get_sweep_length": (I, [P, ctypes.POINTER(I)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc, handle=None):
+    if rc != 0:
+        msg = load_library().mhd_last_error(handle)
+        raise MHDError(f"libmhd2d error {rc}: {msg.decode() if msg else ''}")
+
+
+def default_config(nx_int, ny_int, xmin, lx, ymin, ly, **overrides):
+    """Reference example defaults (gamma 5/3, CFL 0.7, EPS 1e-20) for an
+    interior of nx_int x ny_int cells; nx, ny of the result include ghosts."""
+    cfg = MhdConfig()
+    _check(load_library().mhd_default_config(ctypes.byref(cfg), nx_int, ny_int, xmin, lx, ymin, ly))
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise KeyError(k)
+        setattr(cfg, k, v)
+    return cfg
+
+
+def orszag_tang_config(n=256, eta=1e-3):
+    """Orszag-Tang vortex on [0, 2pi]^2, periodic x and y."""
+    return default_config(n, n, 0.0, 2.0 * np.pi, 0.0, 2.0 * np.pi, eta=eta)
+
+
+def harris_config(nx=4096, ny=2048, eta_max=6e-3):
+    """Harris sheet on [-100, 100] x [-50, 50], periodic x, symmetric y.
+    eta_max is the dt diffusion limit (the peak of the resistivity field)."""
+    return default_config(nx, ny, -100.0, 200.0, -50.0, 100.0, eta=eta_max,
+                          bc_x=MHD_BC_PERIODIC, bc_y=MHD_BC_SYMMETRIC)
+
+
+def fill_orszag_tang(cfg):
+    out = np.empty((cfg.nx, cfg.ny, NVAR), dtype=np.float64)
+    _check(load_library().mhd_fill_orszag_tang(ctypes.byref(cfg), _dptr(out)))
+    return out
+
+
+def fill_harris(cfg, seed=0, noise_amp=1e-3):
+    out = np.empty((cfg.nx, cfg.ny, NVAR), dtype=np.float64)
+    _check(load_library().mhd_fill_harris(ctypes.byref(cfg), seed, noise_amp, _dptr(out)))
+    return out
+
+
+def fill_harris_eta(cfg, eta0=1e-3, eta1=5e-3):
+    out = np.empty((cfg.nx, cfg.ny), dtype=np.float64)
+    _check(load_library().mhd_fill_harris_eta(ctypes.byref(cfg), eta0, eta1, _dptr(out)))
+    return out
+
+
+class ResistiveMHD2D:
+    """Mirror of the reference's ``class ResistiveMHD2D`` over the C ABI.
+
+    State arrays are numpy float64 of shape (nx, ny, 8) in the reference's
+    ConservationParameter order (rho, rhoU, rhoV, rhoW, bX, bY, bZ, e), which
+    is the reference's linear layout k = j + i*ny with 8 doubles per cell.
+    """
+
+    def __init__(self, cfg):
+        self._lib = load_library()
+        self.cfg = MhdConfig.from_buffer_copy(cfg)
+        h = ctypes.c_void_p()
+        _check(self._lib.mhd_create(ctypes.byref(self.cfg), ctypes.byref(h)))
+        self._h = h
+
+    # -- lifetime -------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mhd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _ok(self, rc):
+        _check(rc, self._h)
+
+    @property
+    def shape(self):
+        return (self.cfg.nx, self.cfg.ny, NVAR)
+
+    # -- reference API --------------------------------------------------
+    def initializeU(self, U, apply_boundary=True):
+        """Uploads a full state (ghosts included) and, as the examples'
+        initializeU does, applies the configured boundary."""
+        self.setU(U)
+        if apply_boundary:
+            self.applyBoundary()
+
+    def setU(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != self.shape:
+            raise ValueError(f"state shape {U.shape} != {self.shape}")
+        self._ok(self._lib.mhd_set_state(self._h, _dptr(U)))
+
+    def getU(self):
+        out = np.empty(self.shape, dtype=np.float64)
+        self._ok(self._lib.mhd_get_state(self._h, _dptr(out)))
+        return out
+
+    def oneStepRK2(self):
+        self._ok(self._lib.mhd_one_step_rk2(self._h))
+
+    def steps(self, n):
+        """n RK2 steps with no host synchronisation in between."""
+        self._ok(self._lib.mhd_step_rk2(self._h, int(n)))
+
+    def calculateDt(self):
+        dt = ctypes.c_double()
+        self._ok(self._lib.mhd_calculate_dt(self._h, ctypes.byref(dt)))
+        return dt.value
+
+    def checkCalculationIsCrashed(self):
+        c = ctypes.c_int()
+        self._ok(self._lib.mhd_check_crashed(self._h, ctypes.byref(c)))
+        return bool(c.value)
+
+    def save(self, directoryname, filenameWithoutStep, step):
+        self._ok(self._lib.mhd_save(self._h, os.fsencode(directoryname), os.fsencode(filenameWithoutStep),
+                                    int(step)))
+
+    def load(self, path):
+        self._ok(self._lib.mhd_load(self._h, os.fsencode(path)))
+
+    # Boundary::periodicBoundaryX2nd etc. (P/boundary.hpp, S/boundary.hpp)
+    def periodicBoundaryX2nd(self):
+        self._ok(self._lib.mhd_apply_boundary(self._h, 0, MHD_BC_PERIODIC))
+
+    def periodicBoundaryY2nd(self):
+        self._ok(self._lib.mhd_apply_boundary(self._h, 1, MHD_BC_PERIODIC))
+
+    def symmetricBoundaryX2nd(self):
+        self._ok(self._lib.mhd_apply_boundary(self._h, 0, MHD_BC_SYMMETRIC))
+
+    def symmetricBoundaryY2nd(self):
+        self._ok(self._lib.mhd_apply_boundary(self._h, 1, MHD_BC_SYMMETRIC))
+
+    def applyBoundary(self):
+        """The configured boundary, x first then y."""
+        self._ok(self._lib.mhd_apply_boundary(self._h, 0, self.cfg.bc_x))
+        self._ok(self._lib.mhd_apply_boundary(self._h, 1, self.cfg.bc_y))
+
+    def setBoundary(self, bc_x, bc_y):
+        self._ok(self._lib.mhd_set_boundary(self._h, bc_x, bc_y))
+        self.cfg.bc_x, self.cfg.bc_y = bc_x, bc_y
+
+    # -- resistivity, time ----------------------------------------------
+    def setEtaField(self, eta):
+        if eta is None:
+            self._ok(self._lib.mhd_set_eta_field(self._h, None))
+            return
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        if eta.shape != self.shape[:2]:
+            raise ValueError(f"eta shape {eta.shape} != {self.shape[:2]}")
+        self._ok(self._lib.mhd_set_eta_field(self._h, _dptr(eta)))
+
+    def setEta(self, eta):
+        self._ok(self._lib.mhd_set_eta(self._h, float(eta)))
+        self.cfg.eta = float(eta)
+
+    @property
+    def dt(self):
+        v = ctypes.c_double()
+        self._ok(self._lib.mhd_get_dt(self._h, ctypes.byref(v)))
+        return v.value
+
+    @property
+    def totalTime(self):
+        v = ctypes.c_double()
+        self._ok(self._lib.mhd_get_time(self._h, ctypes.byref(v)))
+        return v.value
+
+    @totalTime.setter
+    def totalTime(self, t):
+        self._ok(self._lib.mhd_set_time(self._h, float(t)))
+
+    def synchronize(self):
+        self._ok(self._lib.mhd_synchronize(self._h))
+
+    def set_sweep_length(self, columns):
+        self._ok(self._lib.mhd_set_sweep_length(self._h, int(columns)))
+
+    def sweep_length(self):
+        v = ctypes.c_int()
+        self._ok(self._lib.mhd_get_sweep_length(self._h, ctypes.byref(v)))
+        return v.value
+
+    # -- setups ---------------------------------------------------------
+    def initOrszagTang(self):
+        self._ok(self._lib.mhd_init_orszag_tang(self._h))
+
+    def initHarris(self, seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3):
+        self._ok(self._lib.mhd_init_harris(self._h, seed, noise_amp, eta0, eta1))
