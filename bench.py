@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Throughput of the plain FP64 resistive-MHD RK2 step on one MI355X.
+
+Workload: Harris current sheet, 4096 x 2048 interior cells on
+[-100, 100] x [-50, 50], periodic x, symmetric y, Bx = tanh y,
+rho = sech^2 y + 0.2, p = rho/2, per-cell resistivity
+1e-3 + 5e-3 exp(-(x^2 + y^2)/4), seeded white-noise vy of amplitude 1e-3.
+The state is resident in HBM before the timed region; K steps run with no
+host synchronisation inside it.
+
+Prints one JSON line: zone-cycles/s (interior cells x RK2 steps / s), the
+roofline of the dominant stage kernel (HIP events on the library's stream)
+and the CPU oracle timed on a bounded sample of the same setup.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "zone-cycles/s FP64 4096² (%HBM peak); batched fwd+adjoint runs/s vs plain"
+HBM_PEAK_GBS = 8000.0
+STATE_B = 64   # 8 doubles per cell
+ETA_B = 8      # per-cell resistivity plane
+
+
+def stage_bytes(zones, eta_field):
+    """Algorithmic HBM bytes of one launch of each stage kernel.
+    stage 1: read U^n, write U* ; stage 2: read U*, read U^n, write U^{n+1};
+    each stage reads the resistivity plane once."""
+    e = ETA_B if eta_field else 0
+    return zones * (2 * STATE_B + e), zones * (3 * STATE_B + e)
+
+
+def cpu_baseline(nx, ny, steps):
+    from oracle.oracle import OracleSim, num_threads
+    import resistivemhd_gpu_amd as mhd
+    cfg = mhd.harris_config(nx, ny)
+    o = OracleSim.from_config(cfg)
+    o.set_state(mhd.fill_harris(cfg, seed=0, noise_amp=1e-3))
+    o.set_eta_field(mhd.fill_harris_eta(cfg))
+    o.apply_boundary()
+    o.step(1)  # first touch of the intermediates
+    t0 = time.perf_counter()
+    o.step(steps)
+    t = time.perf_counter() - t0
+    return {
+        "value": nx * ny * steps / t,
+        "unit": "zone-cycles/s",
+        "cores": num_threads(),
+        "kind": "port",
+        "sample": f"CPU oracle (C restatement, OpenMP), Harris sheet {nx}x{ny} interior, {steps} RK2 steps, {t:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--sweep", type=int, default=0, help="columns per wavefront sweep (0 = library default)")
+    ap.add_argument("--numerics", choices=["exact", "fast"], default="exact",
+                    help="exact: bitwise reference arithmetic (library default); fast: rounding-level differences")
+    ap.add_argument("--cpu-nx", type=int, default=1024)
+    ap.add_argument("--cpu-ny", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build runs the step on a single MI355X "
+              "(multi-GPU sharding is a design, not a built path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    root = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, root)
+    import torch
+    import resistivemhd_gpu_amd as mhd
+
+    cfg = mhd.harris_config(args.nx, args.ny)
+    sim = mhd.ResistiveMHD2D(cfg)
+    sim.set_numerics(args.numerics)
+    if args.sweep:
+        sim.set_sweep_length(args.sweep)
+    sim.initHarris(seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3)
+    zones = args.nx * args.ny
+
+    sim.steps(args.warmup)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    sim.profile_enable(True)
+    t0 = time.perf_counter()
+    sim.steps(args.steps)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms1, ms2, nt = sim.profile_read()
+    sim.profile_enable(False)
+    crashed = sim.checkCalculationIsCrashed()
+
+    seconds = t1 - t0
+    value = zones * args.steps / seconds
+    b1, b2 = stage_bytes(zones, True)
+    avg1, avg2 = ms1 / nt / 1e3, ms2 / nt / 1e3
+    # dominant kernel = the stage with the larger summed time
+    if avg2 >= avg1:
+        kname, kbytes, kavg = "rk_stage_kernel<2>", b2, avg2
+    else:
+        kname, kbytes, kavg = "rk_stage_kernel<1>", b1, avg1
+    achieved = kbytes / kavg / 1e9
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": seconds / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"harris_sheet_{args.nx}x{args.ny}_periodic_x_symmetric_y_eta_field",
+            "nx": args.nx, "ny": args.ny, "ghost": 3, "sweep_columns": sim.sweep_length(),
+            "numerics": args.numerics,
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": kbytes,
+            "avg_launch_ms": kavg * 1e3,
+            "stage_ms": [ms1 / nt, ms2 / nt],
+        },
+        "step_hbm_frac": value * (b1 + b2) / zones / (HBM_PEAK_GBS * 1e9),
+        "crashed": crashed,
+    }
+    del sim
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_ny, args.cpu_steps)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -141,10 +141,33 @@ int mhd_fill_harris_eta(const mhd_config *cfg, double eta0, double eta1, double 
 int mhd_init_orszag_tang(mhd_handle *h);
 int mhd_init_harris(mhd_handle *h, unsigned long long seed, double noise_amp, double eta0, double eta1);
 
+/* Arithmetic of the step (no counterpart in the reference, whose single
+ * code path is what MHD_NUMERICS_EXACT reproduces):
+ *  - MHD_NUMERICS_EXACT (default): the reference's expressions operation for
+ *    operation, with IEEE division and square root and no contraction;
+ *    results are bitwise equal to the reference arithmetic on the same inputs.
+ *  - MHD_NUMERICS_FAST: shared reciprocals, squared wave speeds, FMA
+ *    contraction.  It differs from EXACT by rounding, but the reference's HLLD
+ *    takes the physical flux of each star state, which jumps where a
+ *    rotational wave speed crosses zero, so a rounding-level difference that
+ *    flips a region seeds a larger local difference: FAST is not held to the
+ *    1e-10 bar (tests check it at 1e-8 relative L2). */
+enum mhd_numerics { MHD_NUMERICS_FAST = 0, MHD_NUMERICS_EXACT = 1 };
+int mhd_set_numerics(mhd_handle *h, int mode);
+int mhd_get_numerics(const mhd_handle *h, int *mode);
+
 /* Launch-shape control (segment length along x of one wave's sweep; 0 picks
  * the built-in default).  Exposed for the autotuner and the tests. */
 int mhd_set_sweep_length(mhd_handle *h, int columns);
 int mhd_get_sweep_length(const mhd_handle *h, int *columns);
+
+/* Stage timing (no counterpart in the reference, which has no timers):
+ * when enabled, each step records HIP events on the handle's stream around
+ * the two stage kernels; mhd_profile_read synchronises and returns the summed
+ * kernel time in ms of stage 1 and stage 2 launches since enabling, and the
+ * number of steps timed. */
+int mhd_profile_enable(mhd_handle *h, int on);
+int mhd_profile_read(mhd_handle *h, double *ms_stage1, double *ms_stage2, int *steps);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,9 @@
 #include <string.h>
 
 #include "mhd2d_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef struct { double rho, rhoU, rhoV, rhoW, bX, bY, bZ, e; } Cons;   /* P/conservation_parameter_struct.hpp:5-27 */
 typedef struct { double rho, u, v, w, bX, bY, bZ, p; } Prim;            /* P/basic_parameter_struct.hpp:5-27 */
@@ -817,4 +820,13 @@ double oracle_cell_dt(const double *cell8, double gamma, double dx, double dy, d
     Cons c;
     memcpy(&c, cell8, sizeof c);
     return cell_dt(&c, gamma, dx, dy, eps);
+}
+
+int oracle_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

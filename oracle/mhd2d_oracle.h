@@ -28,6 +28,7 @@ void oracle_compute_fluxes(struct oracle_sim *s, double *aos_state, double *F, d
 double oracle_minmod(double x, double y, double eps);
 void oracle_hlld_flux(const double *left8, const double *right8, double gamma, double eps, double *flux8);
 double oracle_cell_dt(const double *cell8, double gamma, double dx, double dy, double eps);
+int oracle_num_threads(void);
 
 #ifdef __cplusplus
 }

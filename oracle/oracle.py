@@ -49,6 +49,7 @@ def load():
         "oracle_minmod": (c_d, [c_d, c_d, c_d]),
         "oracle_hlld_flux": (None, [D, D, c_d, c_d, D]),
         "oracle_cell_dt": (c_d, [D, c_d, c_d, c_d, c_d]),
+        "oracle_num_threads": (c_i, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -157,3 +158,8 @@ def hlld_flux(left, right, gamma=5.0 / 3.0, eps=1e-20):
 def cell_dt(cell, gamma, dx, dy, eps=1e-20):
     c = np.ascontiguousarray(cell, dtype=np.float64)
     return load().oracle_cell_dt(_p(c), gamma, dx, dy, eps)
+
+
+def num_threads():
+    """OpenMP threads the oracle's loops run on."""
+    return load().oracle_num_threads()

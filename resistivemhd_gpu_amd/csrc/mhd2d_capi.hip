@@ -36,9 +36,13 @@ struct mhd_handle {
     Ctrl *ctrl = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
+    int num_cu = 0;
+    int seg_request = 0;      // 0: sweep length chosen from the occupancy
     int cur = 0;              // control slot holding the per-cell dt minimum of U
     bool dt_valid = false;    // ctrl->acc[cur] matches the current U
-    int seg_request = 0;
+    bool exact = true;        // numerics: exact reference arithmetic (default) or fast
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;   // 4 events per timed step: stage-1 start/end, stage-2 start/end
     std::string err;
 };
 
@@ -64,7 +68,27 @@ int hip_fail(mhd_handle *h, hipError_t e, const char *what)
         if (e_ != hipSuccess) return hip_fail(h, e_, #call);\
     } while (0)
 
-constexpr int kDefaultSeg = 64;
+constexpr int kSweepWarmup = 4;   // columns a sweep loads before its first update
+
+// Sweep length along x.  Every (strip, segment) item is one wavefront; a
+// segment of L columns costs about L + kSweepWarmup column steps.  The
+// launch runs in ceil(items / slots) rounds, slots = resident stage
+// wavefronts on the chip, so L is chosen to minimise rounds * (L + warm-up).
+int choose_seg(int nx_int, int nstrips, long long slots)
+{
+    int best = nx_int;
+    double best_cost = 1e300;
+    for (int L = 8; L <= nx_int; ++L) {
+        const long long items = (long long)nstrips * ((nx_int + L - 1) / L);
+        const long long rounds = (items + slots - 1) / slots;
+        const double cost = (double)rounds * (L + kSweepWarmup);
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            best = L;
+        }
+    }
+    return best;
+}
 
 void layout(mhd_handle *h, int seg)
 {
@@ -73,9 +97,14 @@ void layout(mhd_handle *h, int seg)
     g.ny = h->cfg.ny;
     g.pitch = (h->cfg.ny + 7) & ~7;
     g.plane = ((long long)g.nx * g.pitch + 63) & ~63LL;
-    g.seg = seg > 0 ? seg : kDefaultSeg;
+    const int nx_int = h->cfg.nx - 2 * mhd::kGhost;
     g.nstrips = (h->cfg.ny - 2 * mhd::kGhost + mhd::kStripRows - 1) / mhd::kStripRows;
-    g.nsegs = (h->cfg.nx - 2 * mhd::kGhost + g.seg - 1) / g.seg;
+    if (seg <= 0) {
+        const long long slots = (long long)(h->num_cu > 0 ? h->num_cu : 256) * mhd::stage_waves_per_cu(h->exact);
+        seg = choose_seg(nx_int, g.nstrips, slots);
+    }
+    g.seg = seg < nx_int ? seg : nx_int;
+    g.nsegs = (nx_int + g.seg - 1) / g.seg;
 }
 
 void set_phys(mhd_handle *h)
@@ -113,15 +142,15 @@ int ensure_aos(mhd_handle *h)
 
 int apply_bc_pair(mhd_handle *h, double *u, Ctrl *ctrl, int slot)
 {
-    HIPCHECK(h, mhd::launch_bc_x(h->cfg.bc_x, u, ctrl, slot, h->g, h->ph, h->stream));
-    HIPCHECK(h, mhd::launch_bc_y(h->cfg.bc_y, u, ctrl, slot, h->g, h->ph, h->stream));
+    HIPCHECK(h, mhd::launch_bc_x(h->cfg.bc_x, h->exact, u, ctrl, slot, h->g, h->ph, h->stream));
+    HIPCHECK(h, mhd::launch_bc_y(h->cfg.bc_y, h->exact, u, ctrl, slot, h->g, h->ph, h->stream));
     return MHD_OK;
 }
 
 int refresh_dt(mhd_handle *h)
 {
     if (h->dt_valid) return MHD_OK;
-    HIPCHECK(h, mhd::launch_dt_all(h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
+    HIPCHECK(h, mhd::launch_dt_all(h->exact, h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
     h->dt_valid = true;
     return MHD_OK;
 }
@@ -151,11 +180,16 @@ int mhd_create(const mhd_config *cfg, mhd_handle **out)
     mhd_handle *h = new (std::nothrow) mhd_handle();
     if (!h) return fail(nullptr, MHD_ERR_NOMEM, "host allocation failed");
     h->cfg = *cfg;
-    layout(h, 0);
     set_phys(h);
     hipError_t e;
     if ((e = hipGetDevice(&h->device)) != hipSuccess ||
-        (e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&h->num_cu, hipDeviceAttributeMultiprocessorCount, h->device)) != hipSuccess) {
+        const std::string msg = std::string("mhd_create: ") + hipGetErrorString(e);
+        delete h;
+        return fail(nullptr, MHD_ERR_HIP, msg);
+    }
+    layout(h, 0);
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc(&h->U, sizeof(double) * 8 * (size_t)h->g.plane)) != hipSuccess ||
         (e = hipMalloc(&h->Us, sizeof(double) * 8 * (size_t)h->g.plane)) != hipSuccess ||
         (e = hipMalloc(&h->side, sizeof(double) * (size_t)h->cfg.nx)) != hipSuccess ||
@@ -183,6 +217,7 @@ int mhd_destroy(mhd_handle *h)
 {
     if (!h) return MHD_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     (void)hipFree(h->U);
     (void)hipFree(h->Us);
     (void)hipFree(h->eta);
@@ -260,9 +295,9 @@ int mhd_apply_boundary(mhd_handle *h, int axis, int kind)
     if ((axis != 0 && axis != 1) || (kind != MHD_BC_PERIODIC && kind != MHD_BC_SYMMETRIC))
         return fail(h, MHD_ERR_INVALID, "axis must be 0/1 and kind MHD_BC_PERIODIC/MHD_BC_SYMMETRIC");
     if (axis == 0)
-        HIPCHECK(h, mhd::launch_bc_x(kind, h->U, nullptr, 0, h->g, h->ph, h->stream));
+        HIPCHECK(h, mhd::launch_bc_x(kind, h->exact, h->U, nullptr, 0, h->g, h->ph, h->stream));
     else
-        HIPCHECK(h, mhd::launch_bc_y(kind, h->U, nullptr, 0, h->g, h->ph, h->stream));
+        HIPCHECK(h, mhd::launch_bc_y(kind, h->exact, h->U, nullptr, 0, h->g, h->ph, h->stream));
     h->dt_valid = false;
     return MHD_OK;
 }
@@ -280,7 +315,7 @@ int mhd_set_boundary(mhd_handle *h, int bc_x, int bc_y)
 int mhd_calculate_dt(mhd_handle *h, double *dt)
 {
     if (!h || !dt) return fail(h, MHD_ERR_INVALID, "NULL argument");
-    HIPCHECK(h, mhd::launch_dt_all(h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
+    HIPCHECK(h, mhd::launch_dt_all(h->exact, h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
     h->dt_valid = true;
     Ctrl c;
     int rc = read_ctrl(h, &c);
@@ -298,12 +333,51 @@ int mhd_step_rk2(mhd_handle *h, int nsteps)
     int rc = refresh_dt(h);
     if (rc) return rc;
     for (int n = 0; n < nsteps; ++n) {
-        HIPCHECK(h, mhd::launch_stage(1, h->U, h->Us, h->U, h->eta, h->side, h->ctrl, h->g, h->ph, h->cur, h->stream));
+        hipEvent_t ev[4] = {};
+        if (h->profiling) {
+            for (int q = 0; q < 4; ++q) {
+                HIPCHECK(h, hipEventCreate(&ev[q]));
+                h->ev.push_back(ev[q]);
+            }
+        }
+        if (ev[0]) HIPCHECK(h, hipEventRecord(ev[0], h->stream));
+        HIPCHECK(h, mhd::launch_stage(1, h->exact, h->U, h->Us, h->U, h->eta, h->side, h->ctrl, h->g, h->ph, h->cur, h->stream));
+        if (ev[1]) HIPCHECK(h, hipEventRecord(ev[1], h->stream));
         if ((rc = apply_bc_pair(h, h->Us, nullptr, 0))) return rc;
-        HIPCHECK(h, mhd::launch_stage(2, h->Us, h->U, h->U, h->eta, h->side, h->ctrl, h->g, h->ph, h->cur, h->stream));
+        if (ev[2]) HIPCHECK(h, hipEventRecord(ev[2], h->stream));
+        HIPCHECK(h, mhd::launch_stage(2, h->exact, h->Us, h->U, h->U, h->eta, h->side, h->ctrl, h->g, h->ph, h->cur, h->stream));
+        if (ev[3]) HIPCHECK(h, hipEventRecord(ev[3], h->stream));
         if ((rc = apply_bc_pair(h, h->U, h->ctrl, h->cur ^ 1))) return rc;
         h->cur ^= 1;
     }
+    return MHD_OK;
+}
+
+int mhd_profile_enable(mhd_handle *h, int on)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+    h->ev.clear();
+    h->profiling = on != 0;
+    return MHD_OK;
+}
+
+int mhd_profile_read(mhd_handle *h, double *ms_stage1, double *ms_stage2, int *steps)
+{
+    if (!h || !ms_stage1 || !ms_stage2 || !steps) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    double s1 = 0.0, s2 = 0.0;
+    for (size_t q = 0; q + 3 < h->ev.size(); q += 4) {
+        float a = 0.f, b = 0.f;
+        HIPCHECK(h, hipEventElapsedTime(&a, h->ev[q], h->ev[q + 1]));
+        HIPCHECK(h, hipEventElapsedTime(&b, h->ev[q + 2], h->ev[q + 3]));
+        s1 += a;
+        s2 += b;
+    }
+    *ms_stage1 = s1;
+    *ms_stage2 = s2;
+    *steps = (int)(h->ev.size() / 4);
     return MHD_OK;
 }
 
@@ -428,7 +502,27 @@ int mhd_set_sweep_length(mhd_handle *h, int columns)
 {
     if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
     if (columns < 0 || columns > 1 << 20) return fail(h, MHD_ERR_INVALID, "columns out of range");
+    h->seg_request = columns;
     layout(h, columns);
+    return MHD_OK;
+}
+
+int mhd_set_numerics(mhd_handle *h, int mode)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if (mode != MHD_NUMERICS_FAST && mode != MHD_NUMERICS_EXACT)
+        return fail(h, MHD_ERR_INVALID, "mode must be MHD_NUMERICS_FAST or MHD_NUMERICS_EXACT");
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    h->exact = mode == MHD_NUMERICS_EXACT;
+    h->dt_valid = false;   // the next dt is reduced with the new arithmetic
+    layout(h, h->seg_request);   // occupancy differs between the two kernels
+    return MHD_OK;
+}
+
+int mhd_get_numerics(const mhd_handle *h, int *mode)
+{
+    if (!h || !mode) return fail(nullptr, MHD_ERR_INVALID, "NULL argument");
+    *mode = h->exact ? MHD_NUMERICS_EXACT : MHD_NUMERICS_FAST;
     return MHD_OK;
 }
 

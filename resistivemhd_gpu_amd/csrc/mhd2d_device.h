@@ -73,7 +73,7 @@ struct Face7 {
 // HLLD flux at one interface with normal field bX (both sides share it):
 // parameters P/hlld.cu:256-433, candidates P/hlld.cu:469-549, selection as
 // the sum of six indicator-weighted candidates P/hlld.cu:35-107.
-__device__ __forceinline__ Flux8 hlld(const Face7 &Lq, const Face7 &Rq, double bX, const Phys &ph)
+__device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, double bX, const Phys &ph)
 {
     const double gamma = ph.gamma, eps = ph.eps;
     const double rhoL = Lq.rho, uL = Lq.u, vL = Lq.v, wL = Lq.w, bXL = bX, bYL = Lq.by, bZL = Lq.bz, pL = Lq.p;
@@ -185,8 +185,8 @@ __device__ __forceinline__ Flux8 hlld(const Face7 &Lq, const Face7 &Rq, double b
 
 // calculateDtFunctor, P/resistiveMHD_2D.cu:247-274: the stored face B is
 // used as if it were centred.
-__device__ __forceinline__ double cell_dt(double rho, double mx, double my, double mz,
-                                          double bX, double bY, double bZ, double e, const Phys &ph)
+__device__ __forceinline__ double cell_dt_exact(double rho, double mx, double my, double mz,
+                                                double bX, double bY, double bZ, double e, const Phys &ph)
 {
     const double u = mx / rho, v = my / rho, w = mz / rho;
     const double p = (ph.gamma - 1.0)
@@ -197,6 +197,164 @@ __device__ __forceinline__ double cell_dt(double rho, double mx, double my, doub
     const double maxSpeedX = fabs(u) + sqrt(cs * cs + ca * ca);
     const double maxSpeedY = fabs(v) + sqrt(cs * cs + ca * ca);
     return 1.0 / (maxSpeedX / ph.dx + maxSpeedY / ph.dy + ph.eps);
+}
+
+// ---------------------------------------------------------------------------
+// Fast numerics: the same formulas with shared reciprocals (hardware
+// reciprocal / reciprocal square root refined by Newton steps to within an
+// ulp or two), squared wave speeds reused instead of square roots of
+// squares, FMA contraction, and only the selected HLLD region's flux
+// evaluated.  Results differ from the exact path by rounding only.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double rcp_fast(double x)
+{
+#pragma clang fp contract(fast)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// 1/sqrt(x) for x > 0
+__device__ __forceinline__ double rsqrt_fast(double x)
+{
+#pragma clang fp contract(fast)
+    double y = __builtin_amdgcn_rsq(x);
+    double h = 0.5 * y;
+    double g = x * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    r = fma(-g, h, 0.5);
+    h = fma(h, r, h);
+    return 2.0 * h;
+}
+
+__device__ __forceinline__ double sqrt_fast(double x)
+{
+    return __builtin_sqrt(x);
+}
+
+__device__ __forceinline__ Flux8 hlld_fast(const Face7 &L, const Face7 &R, double bx, const Phys &ph,
+                                           double inv_gm1)
+{
+#pragma clang fp contract(fast)
+    const double gamma = ph.gamma, eps = ph.eps;
+    const double bx2 = bx * bx;
+
+    const double irhoL = rcp_fast(L.rho), irhoR = rcp_fast(R.rho);
+    const double B2L = bx2 + L.by * L.by + L.bz * L.bz;
+    const double B2R = bx2 + R.by * R.by + R.bz * R.bz;
+    const double eL = L.p * inv_gm1 + 0.5 * L.rho * (L.u * L.u + L.v * L.v + L.w * L.w) + 0.5 * B2L;
+    const double eR = R.p * inv_gm1 + 0.5 * R.rho * (R.u * R.u + R.v * R.v + R.w * R.w) + 0.5 * B2R;
+    const double pTL = L.p + 0.5 * B2L;
+    const double pTR = R.p + 0.5 * B2R;
+
+    const double cs2L = gamma * L.p * irhoL, ca2L = B2L * irhoL, va2L = bx2 * irhoL;
+    const double cs2R = gamma * R.p * irhoR, ca2R = B2R * irhoR, va2R = bx2 * irhoR;
+    const double aL = cs2L + ca2L, aR = cs2R + ca2R;
+    const double cfL = sqrt_fast(0.5 * (aL + sqrt_fast(aL * aL - 4.0 * cs2L * va2L)));
+    const double cfR = sqrt_fast(0.5 * (aR + sqrt_fast(aR * aR - 4.0 * cs2R * va2R)));
+
+    const double cfm = tmax(cfL, cfR);
+    const double SL = tmin(tmin(L.u, R.u) - cfm, 0.0);
+    const double SR = tmax(tmax(L.u, R.u) + cfm, 0.0);
+
+    const double dL = SL - L.u, dR = SR - R.u;
+    const double rdL = dL * L.rho, rdR = dR * R.rho;
+    const double iden = rcp_fast(rdR - rdL + eps);
+    const double SM = (rdR * R.u - rdL * L.u - pTR + pTL) * iden;
+    const double pT1 = (rdR * pTL - rdL * pTR + L.rho * R.rho * dR * dL * (R.u - L.u)) * iden;
+
+    // single-star states
+    const double iSLSM = rcp_fast(SL - SM + eps), iSRSM = rcp_fast(SR - SM + eps);
+    const double rho1L = rdL * iSLSM, rho1R = rdR * iSRSM;
+    const double iDL = rcp_fast(rdL * (SL - SM) - bx2 + eps);
+    const double iDR = rcp_fast(rdR * (SR - SM) - bx2 + eps);
+    const double tL = bx * (SM - L.u) * iDL, tR = bx * (SM - R.u) * iDR;
+    const double v1L = L.v - L.by * tL, w1L = L.w - L.bz * tL;
+    const double v1R = R.v - R.by * tR, w1R = R.w - R.bz * tR;
+    const double fL = (rdL * dL - bx2) * iDL, fR = (rdR * dR - bx2) * iDR;
+    const double by1L = L.by * fL, bz1L = L.bz * fL;
+    const double by1R = R.by * fR, bz1R = R.bz * fR;
+    const double vb1L = SM * bx + v1L * by1L + w1L * bz1L;
+    const double vb1R = SM * bx + v1R * by1R + w1R * bz1R;
+    const double e1L = (dL * eL - pTL * L.u + pT1 * SM + bx * ((L.u * bx + L.v * L.by + L.w * L.bz) - vb1L)) * iSLSM;
+    const double e1R = (dR * eR - pTR * R.u + pT1 * SM + bx * ((R.u * bx + R.v * R.by + R.w * R.bz) - vb1R)) * iSRSM;
+
+    const double isqL = rsqrt_fast(rho1L), isqR = rsqrt_fast(rho1R);
+    const double sqL = rho1L * isqL, sqR = rho1R * isqR;
+    const double abx = fabs(bx);
+    const double S1L = SM - abx * isqL;
+    const double S1R = SM + abx * isqR;
+
+    // double-star states
+    const double s = sgn(bx);
+    const double isum = rcp_fast(sqL + sqR + eps);
+    const double v2 = (sqL * v1L + sqR * v1R + (by1R - by1L) * s) * isum;
+    const double w2 = (sqL * w1L + sqR * w1R + (bz1R - bz1L) * s) * isum;
+    const double sqLR = sqL * sqR;
+    const double by2 = (sqL * by1R + sqR * by1L + sqLR * (v1R - v1L) * s) * isum;
+    const double bz2 = (sqL * bz1R + sqR * bz1L + sqLR * (w1R - w1L) * s) * isum;
+    const double vb2 = SM * bx + v2 * by2 + w2 * bz2;
+    const double e2L = e1L - sqL * (vb1L - vb2) * s;
+    const double e2R = e1R + sqR * (vb1R - vb2) * s;
+
+    // Region of the interface (P/hlld.cu:64-69).  With SL <= 0 <= SR at most
+    // one of the star regions holds; the outer-right region (SR == 0) may hold
+    // together with one of them, and its flux is then added, as the
+    // reference's indicator sum does.  A NaN speed poisons the flux, as a NaN
+    // candidate does in the reference's sum.
+    const bool left = SM > 0.0;
+    const bool inner = left ? !(S1L > 0.0) : (S1R > 0.0);
+    const bool main_on = left || (S1R > 0.0) || (SR > 0.0);
+    const double rho = left ? rho1L : rho1R;
+    const double v = inner ? v2 : (left ? v1L : v1R);
+    const double w = inner ? w2 : (left ? w1L : w1R);
+    const double by = inner ? by2 : (left ? by1L : by1R);
+    const double bz = inner ? bz2 : (left ? bz1L : bz1R);
+    const double e = inner ? (left ? e2L : e2R) : (left ? e1L : e1R);
+    const double m = main_on ? 1.0 : 0.0;
+    Flux8 f = one_flux(rho, SM, v, w, bx, by, bz, e, pT1);
+    f.f0 *= m; f.f1 *= m; f.f2 *= m; f.f3 *= m; f.f5 *= m; f.f6 *= m; f.f7 *= m;
+    if (SR <= 0.0) {
+        const Flux8 o = one_flux(R.rho, R.u, R.v, R.w, bx, R.by, R.bz, eR, pTR);
+        f.f0 += o.f0; f.f1 += o.f1; f.f2 += o.f2; f.f3 += o.f3; f.f5 += o.f5; f.f6 += o.f6; f.f7 += o.f7;
+    }
+    if (!(S1L == S1L) || !(S1R == S1R) || !(SM == SM)) {
+        const double nan = __builtin_nan("");
+        f.f0 = f.f1 = f.f2 = f.f3 = f.f5 = f.f6 = f.f7 = nan;
+    }
+    return f;
+}
+
+__device__ __forceinline__ double cell_dt_fast(double rho, double mx, double my, double mz,
+                                               double bX, double bY, double bZ, double e, const Phys &ph)
+{
+#pragma clang fp contract(fast)
+    const double ir = rcp_fast(rho);
+    const double u = mx * ir, v = my * ir, w = mz * ir;
+    const double B2 = bX * bX + bY * bY + bZ * bZ;
+    const double p = (ph.gamma - 1.0) * (e - 0.5 * rho * (u * u + v * v + w * w) - 0.5 * B2);
+    const double c = sqrt_fast((ph.gamma * p + B2) * ir);
+    return rcp_fast((fabs(u) + c) / ph.dx + (fabs(v) + c) / ph.dy + ph.eps);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ Flux8 hlld(const Face7 &L, const Face7 &R, double bx, const Phys &ph, double inv_gm1)
+{
+    if (EXACT) return hlld_exact(L, R, bx, ph);
+    return hlld_fast(L, R, bx, ph, inv_gm1);
+}
+
+template <bool EXACT>
+__device__ __forceinline__ double cell_dt(double rho, double mx, double my, double mz, double bX, double bY,
+                                          double bZ, double e, const Phys &ph)
+{
+    if (EXACT) return cell_dt_exact(rho, mx, my, mz, bX, bY, bZ, e, ph);
+    return cell_dt_fast(rho, mx, my, mz, bX, bY, bZ, e, ph);
 }
 
 // Wavefront (64-lane) reductions.

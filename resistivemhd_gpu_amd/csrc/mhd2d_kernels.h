@@ -17,11 +17,19 @@ struct Grid {
     int nstrips, nsegs;  // work decomposition: strips along y, segments along x
 };
 
-hipError_t launch_stage(int stage, const double *uin, double *uout, const double *uold, const double *eta,
-                        double *side, Ctrl *ctrl, const Grid &g, const Phys &ph, int cur, hipStream_t s);
-hipError_t launch_bc_x(int kind, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s);
-hipError_t launch_bc_y(int kind, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s);
-hipError_t launch_dt_all(const double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s);
+// exact = true: operation-for-operation reference arithmetic (built without
+// contraction it rounds as the reference does); false: fast numerics.
+hipError_t launch_stage(int stage, bool exact, const double *uin, double *uout, const double *uold,
+                        const double *eta, double *side, Ctrl *ctrl, const Grid &g, const Phys &ph, int cur,
+                        hipStream_t s);
+hipError_t launch_bc_x(int kind, bool exact, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
+                       hipStream_t s);
+hipError_t launch_bc_y(int kind, bool exact, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
+                       hipStream_t s);
+hipError_t launch_dt_all(bool exact, const double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
+                         hipStream_t s);
+// Resident 64-lane stage workgroups per CU (the lower of the two stages).
+int stage_waves_per_cu(bool exact);
 hipError_t launch_unpack(const double *aos, double *u, const Grid &g, hipStream_t s);
 hipError_t launch_pack(const double *u, double *aos, const Grid &g, hipStream_t s);
 hipError_t launch_scatter_plane(const double *src, double *dst, const Grid &g, hipStream_t s);

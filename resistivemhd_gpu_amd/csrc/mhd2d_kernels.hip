@@ -31,30 +31,42 @@ struct Cell {
     double rho, u, v, w, bxc, byc, bz, px, py;
 };
 
+template <bool EXACT>
 __device__ __forceinline__ Cell make_cell(double rho, double mx, double my, double mz, double bxc,
                                           double byc, double bz, double e, const Phys &ph)
 {
     Cell c;
     c.rho = rho;
-    c.u = mx / rho;
-    c.v = my / rho;
-    c.w = mz / rho;
     c.bxc = bxc;
     c.byc = byc;
     c.bz = bz;
-    // x sweep: GetBasicParamterFunctor on the centred state
-    c.px = (ph.gamma - 1.0)
-         * (e - 0.5 * (rho * (c.u * c.u + c.v * c.v + c.w * c.w))
-         - 0.5 * (bxc * bxc + byc * byc + bz * bz));
-    // y sweep: the same functor on the rotated state (P/hlld.cu:134-152),
-    // whose sums run in the rotated order
-    c.py = (ph.gamma - 1.0)
-         * (e - 0.5 * (rho * (c.v * c.v + c.w * c.w + c.u * c.u))
-         - 0.5 * (byc * byc + bz * bz + bxc * bxc));
+    if (EXACT) {
+        c.u = mx / rho;
+        c.v = my / rho;
+        c.w = mz / rho;
+        // x sweep: GetBasicParamterFunctor on the centred state
+        c.px = (ph.gamma - 1.0)
+             * (e - 0.5 * (rho * (c.u * c.u + c.v * c.v + c.w * c.w))
+             - 0.5 * (bxc * bxc + byc * byc + bz * bz));
+        // y sweep: the same functor on the rotated state (P/hlld.cu:134-152),
+        // whose sums run in the rotated order
+        c.py = (ph.gamma - 1.0)
+             * (e - 0.5 * (rho * (c.v * c.v + c.w * c.w + c.u * c.u))
+             - 0.5 * (byc * byc + bz * bz + bxc * bxc));
+    } else {
+#pragma clang fp contract(fast)
+        const double ir = rcp_fast(rho);
+        c.u = mx * ir;
+        c.v = my * ir;
+        c.w = mz * ir;
+        c.px = (ph.gamma - 1.0) * (e - 0.5 * rho * (c.u * c.u + c.v * c.v + c.w * c.w)
+                                     - 0.5 * (bxc * bxc + byc * byc + bz * bz));
+        c.py = c.px;
+    }
     return c;
 }
 
-template <int STAGE>
+template <int STAGE, bool EXACT>
 __global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                                                       const double *uold, const double *__restrict__ etaf,
                                                       double *__restrict__ side, Ctrl *__restrict__ ctrl,
@@ -95,6 +107,8 @@ __global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__
     const double dtdx = dt / ph.dx;
     const double dtdy = dt / ph.dy;
     const double two_dx = 2.0 * ph.dx, two_dy = 2.0 * ph.dy;
+    const double inv_2dx = 1.0 / two_dx, inv_2dy = 1.0 / two_dy;
+    const double inv_gm1 = 1.0 / (ph.gamma - 1.0);
 
     const double *Rh = PL(uin, 0), *Mx = PL(uin, 1), *My = PL(uin, 2), *Mz = PL(uin, 3);
     const double *Bx = PL(uin, 4), *By = PL(uin, 5), *Bz = PL(uin, 6), *En = PL(uin, 7);
@@ -116,7 +130,7 @@ __global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__
         const double bxc = 0.5 * (bx + bx_prev);
         bx_prev = bx;
         const double byc = 0.5 * (by + bym);
-        const Cell C = make_cell(rho, mx, my, mz, bxc, byc, bz, e, ph);
+        const Cell C = make_cell<EXACT>(rho, mx, my, mz, bxc, byc, bz, e, ph);
 
         if (c >= i0) {
             // x-slopes of cell c-1 (P/muscl.cu:6-29,67-90): right state of face
@@ -133,9 +147,16 @@ __global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__
             const double bxc_up = __shfl_down(B.bxc, 1, 64), bxc_dn = __shfl_up(B.bxc, 1, 64);
             const double bz_up = __shfl_down(B.bz, 1, 64), bz_dn = __shfl_up(B.bz, 1, 64);
             const double eta = etaf ? etaf[(size_t)(c - 1) * g.pitch + jl] : ph.eta;
-            const double jY = -(C.bz - A.bz) / two_dx;
-            const double jZ = (C.byc - A.byc) / two_dx - (bxc_up - bxc_dn) / two_dy;
-            const double jX = (bz_up - bz_dn) / two_dy;
+            double jX, jY, jZ;
+            if (EXACT) {
+                jY = -(C.bz - A.bz) / two_dx;
+                jZ = (C.byc - A.byc) / two_dx - (bxc_up - bxc_dn) / two_dy;
+                jX = (bz_up - bz_dn) / two_dy;
+            } else {
+                jY = -(C.bz - A.bz) * inv_2dx;
+                jZ = (C.byc - A.byc) * inv_2dx - (bxc_up - bxc_dn) * inv_2dy;
+                jX = (bz_up - bz_dn) * inv_2dy;
+            }
             const double eJX = eta * jX, eJY = eta * jY, eJZ = eta * jZ;
             const double eJYBZ = eJY * B.bz, eJZBY = eJZ * B.byc;
             const double eJZBX = eJZ * B.bxc, eJXBZ = eJX * B.bz;
@@ -150,7 +171,7 @@ __global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__
                 Rx.by = B.byc - 0.5 * sby;
                 Rx.bz = B.bz - 0.5 * sbz;
                 Rx.p = B.px - 0.5 * sp;
-                F = hlld(Lx, Rx, 0.5 * (A.bxc + B.bxc), ph);
+                F = hlld<EXACT>(Lx, Rx, 0.5 * (A.bxc + B.bxc), ph, inv_gm1);
                 F.f5 -= 0.5 * (rJZ + eJZ);
                 F.f6 += 0.5 * (rJY + eJY);
                 F.f7 += 0.5 * (rJYBZ + eJYBZ) - 0.5 * (rJZBY + eJZBY);
@@ -191,7 +212,7 @@ __global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__
                 Rg.bz = __shfl_down(q5 - 0.5 * t5, 1, 64);
                 Rg.p = __shfl_down(q6 - 0.5 * t6, 1, 64);
                 const double bn = 0.5 * (B.byc + __shfl_down(B.byc, 1, 64));
-                const Flux8 Gr = hlld(Lg, Rg, bn, ph);
+                const Flux8 Gr = hlld<EXACT>(Lg, Rg, bn, ph, inv_gm1);
                 // rotate back (P/hlld.cu:170-194)
                 G.f0 = Gr.f0; G.f1 = Gr.f3; G.f2 = Gr.f1; G.f3 = Gr.f2;
                 G.f4 = Gr.f6; G.f5 = Gr.f4; G.f6 = Gr.f5; G.f7 = Gr.f7;
@@ -256,7 +277,7 @@ __global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__
                             PL(uout, 4)[os] = n4; PL(uout, 5)[os] = n5; PL(uout, 6)[os] = n6; PL(uout, 7)[os] = n7;
                             // fused epilogue: per-cell dt of the new state for
                             // the next step, and the crash flag
-                            const double d = cell_dt(n0, n1, n2, n3, n4, n5, n6, n7, ph);
+                            const double d = cell_dt<EXACT>(n0, n1, n2, n3, n4, n5, n6, n7, ph);
                             if (d < dtloc) dtloc = d;
                             nanflag |= isnan(n7) ? 1 : 0;
                         } else {
@@ -284,7 +305,7 @@ __global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__
 // Periodic x (P/boundary.cu:4-71) or symmetric x (S/boundary.cu:4-104), one
 // thread per row.  The symmetric bX rebuild reads By of the interior columns
 // the mirror copied from (identical values), so one pass suffices.
-template <int KIND>
+template <int KIND, bool EXACT>
 __global__ __launch_bounds__(256) void bc_x_kernel(double *u, Ctrl *ctrl, int slot, Grid g, Phys ph)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -337,7 +358,7 @@ __global__ __launch_bounds__(256) void bc_x_kernel(double *u, Ctrl *ctrl, int sl
         if (ctrl && j >= kGhost && j < g.ny - kGhost) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double d = cell_dt(cell[q][0], cell[q][1], cell[q][2], cell[q][3], cell[q][4], cell[q][5],
+                const double d = cell_dt<EXACT>(cell[q][0], cell[q][1], cell[q][2], cell[q][3], cell[q][4], cell[q][5],
                                          cell[q][6], cell[q][7], ph);
                 if (d < dloc) dloc = d;
             }
@@ -348,7 +369,7 @@ __global__ __launch_bounds__(256) void bc_x_kernel(double *u, Ctrl *ctrl, int sl
 
 // Periodic y (P/boundary.cu:74-141) or symmetric y (S/boundary.cu:107-205),
 // one thread per column, after the x boundary.
-template <int KIND>
+template <int KIND, bool EXACT>
 __global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int slot, Grid g, Phys ph)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -402,7 +423,7 @@ __global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int sl
         if (ctrl) {
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double d = cell_dt(cell[q][0], cell[q][1], cell[q][2], cell[q][3], cell[q][4], cell[q][5],
+                const double d = cell_dt<EXACT>(cell[q][0], cell[q][1], cell[q][2], cell[q][3], cell[q][4], cell[q][5],
                                          cell[q][6], cell[q][7], ph);
                 if (d < dloc) dloc = d;
             }
@@ -413,21 +434,21 @@ __global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int sl
 
 // Per-cell dt over all nx*ny cells (P/resistiveMHD_2D.cu:277-286) and the
 // NaN scan of e, for a state set from outside the step.
+template <bool EXACT>
 __global__ __launch_bounds__(256) void dt_all_kernel(const double *__restrict__ u, Ctrl *ctrl, int slot, Grid g,
                                                      Phys ph)
 {
     double dloc = __longlong_as_double((long long)kInfBits);
     int nanflag = 0;
-    const long long total = (long long)g.nx * g.ny;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
-         k += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(k / g.ny), j = (int)(k % g.ny);
+    const int i = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < g.ny) {
         const size_t o = (size_t)i * g.pitch + j;
         const double e = PL(u, 7)[o];
-        const double d = cell_dt(PL(u, 0)[o], PL(u, 1)[o], PL(u, 2)[o], PL(u, 3)[o], PL(u, 4)[o], PL(u, 5)[o],
-                                 PL(u, 6)[o], e, ph);
-        if (d < dloc) dloc = d;
-        nanflag |= isnan(e) ? 1 : 0;   // checkCalculationIsCrashed, P/resistiveMHD_2D.cu:294-309
+        dloc = cell_dt<EXACT>(PL(u, 0)[o], PL(u, 1)[o], PL(u, 2)[o], PL(u, 3)[o], PL(u, 4)[o], PL(u, 5)[o],
+                              PL(u, 6)[o], e, ph);
+        if (!(dloc == dloc)) dloc = __longlong_as_double((long long)kInfBits);
+        nanflag = isnan(e) ? 1 : 0;   // checkCalculationIsCrashed, P/resistiveMHD_2D.cu:294-309
     }
     accumulate_dt(ctrl, slot, dloc);
     const unsigned long long any = __ballot(nanflag);
@@ -487,45 +508,73 @@ static int grid_for(long long n, int block)
     return (int)b;
 }
 
-hipError_t launch_stage(int stage, const double *uin, double *uout, const double *uold, const double *eta,
-                        double *side, Ctrl *ctrl, const Grid &g, const Phys &ph, int cur, hipStream_t s)
+hipError_t launch_stage(int stage, bool exact, const double *uin, double *uout, const double *uold,
+                        const double *eta, double *side, Ctrl *ctrl, const Grid &g, const Phys &ph, int cur,
+                        hipStream_t s)
 {
     const int blocks = g.nstrips * g.nsegs;
-    if (stage == 1)
-        hipLaunchKernelGGL(rk_stage_kernel<1>, dim3(blocks), dim3(64), 0, s, uin, uout, uold, eta, side, ctrl, g, ph,
-                           cur);
-    else
-        hipLaunchKernelGGL(rk_stage_kernel<2>, dim3(blocks), dim3(64), 0, s, uin, uout, uold, eta, side, ctrl, g, ph,
-                           cur);
+#define MHD_LAUNCH_STAGE(ST, EX)                                                                               \
+    hipLaunchKernelGGL((rk_stage_kernel<ST, EX>), dim3(blocks), dim3(64), 0, s, uin, uout, uold, eta, side, ctrl, \
+                       g, ph, cur)
+    if (stage == 1) {
+        if (exact) MHD_LAUNCH_STAGE(1, true); else MHD_LAUNCH_STAGE(1, false);
+    } else {
+        if (exact) MHD_LAUNCH_STAGE(2, true); else MHD_LAUNCH_STAGE(2, false);
+    }
+#undef MHD_LAUNCH_STAGE
     return hipGetLastError();
 }
 
-hipError_t launch_bc_x(int kind, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s)
+hipError_t launch_bc_x(int kind, bool exact, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
+                       hipStream_t s)
 {
     const int blocks = (g.ny + 255) / 256;
-    if (kind == 0)
-        hipLaunchKernelGGL(bc_x_kernel<0>, dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
-    else
-        hipLaunchKernelGGL(bc_x_kernel<1>, dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+    if (kind == 0) {
+        if (exact) hipLaunchKernelGGL((bc_x_kernel<0, true>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+        else hipLaunchKernelGGL((bc_x_kernel<0, false>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+    } else {
+        if (exact) hipLaunchKernelGGL((bc_x_kernel<1, true>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+        else hipLaunchKernelGGL((bc_x_kernel<1, false>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_bc_y(int kind, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s)
+hipError_t launch_bc_y(int kind, bool exact, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
+                       hipStream_t s)
 {
     const int blocks = (g.nx + 255) / 256;
-    if (kind == 0)
-        hipLaunchKernelGGL(bc_y_kernel<0>, dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
-    else
-        hipLaunchKernelGGL(bc_y_kernel<1>, dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+    if (kind == 0) {
+        if (exact) hipLaunchKernelGGL((bc_y_kernel<0, true>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+        else hipLaunchKernelGGL((bc_y_kernel<0, false>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+    } else {
+        if (exact) hipLaunchKernelGGL((bc_y_kernel<1, true>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+        else hipLaunchKernelGGL((bc_y_kernel<1, false>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_dt_all(const double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph, hipStream_t s)
+hipError_t launch_dt_all(bool exact, const double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
+                         hipStream_t s)
 {
     hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, ctrl, slot);
-    hipLaunchKernelGGL(dt_all_kernel, dim3(grid_for((long long)g.nx * g.ny, 256)), dim3(256), 0, s, u, ctrl, slot, g,
-                       ph);
+    const dim3 grid((g.ny + 255) / 256, g.nx);
+    if (exact) hipLaunchKernelGGL((dt_all_kernel<true>), grid, dim3(256), 0, s, u, ctrl, slot, g, ph);
+    else hipLaunchKernelGGL((dt_all_kernel<false>), grid, dim3(256), 0, s, u, ctrl, slot, g, ph);
     return hipGetLastError();
+}
+
+int stage_waves_per_cu(bool exact)
+{
+    int a = 0, b = 0;
+    if (exact) {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_kernel<1, true>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_kernel<2, true>, 64, 0);
+    } else {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_kernel<1, false>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_kernel<2, false>, 64, 0);
+    }
+    const int w = a < b ? a : b;
+    return w > 0 ? w : 1;
 }
 
 hipError_t launch_unpack(const double *aos, double *u, const Grid &g, hipStream_t s)

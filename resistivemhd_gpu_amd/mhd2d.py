@@ -25,8 +25,12 @@ EXPORTS = (
     "mhd_set_boundary", "mhd_calculate_dt", "mhd_one_step_rk2", "mhd_step_rk2", "mhd_get_dt",
     "mhd_get_time", "mhd_set_time", "mhd_check_crashed", "mhd_save", "mhd_load", "mhd_synchronize",
     "mhd_fill_orszag_tang", "mhd_fill_harris", "mhd_fill_harris_eta", "mhd_init_orszag_tang",
-    "mhd_init_harris", "mhd_set_sweep_length", "mhd_get_sweep_length",
+    "mhd_init_harris", "mhd_set_sweep_length", "mhd_get_sweep_length", "mhd_profile_enable",
+    "mhd_profile_read", "mhd_set_numerics", "mhd_get_numerics",
 )
+
+MHD_NUMERICS_FAST = 0
+MHD_NUMERICS_EXACT = 1
 
 
 class MhdConfig(ctypes.Structure):
@@ -92,6 +96,10 @@ def load_library(path=LIB_PATH):
         "mhd_init_harris": (I, [P, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "mhd_set_sweep_length": (I, [P, I]),
         "mhd_get_sweep_length": (I, [P, ctypes.POINTER(I)]),
+        "mhd_profile_enable": (I, [P, I]),
+        "mhd_profile_read": (I, [P, D, D, ctypes.POINTER(I)]),
+        "mhd_set_numerics": (I, [P, I]),
+        "mhd_get_numerics": (I, [P, ctypes.POINTER(I)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -291,6 +299,17 @@ class ResistiveMHD2D:
     def synchronize(self):
         self._ok(self._lib.mhd_synchronize(self._h))
 
+    def set_numerics(self, mode):
+        """MHD_NUMERICS_FAST (default) or MHD_NUMERICS_EXACT."""
+        if isinstance(mode, str):
+            mode = {"fast": MHD_NUMERICS_FAST, "exact": MHD_NUMERICS_EXACT}[mode]
+        self._ok(self._lib.mhd_set_numerics(self._h, int(mode)))
+
+    def numerics(self):
+        v = ctypes.c_int()
+        self._ok(self._lib.mhd_get_numerics(self._h, ctypes.byref(v)))
+        return "exact" if v.value == MHD_NUMERICS_EXACT else "fast"
+
     def set_sweep_length(self, columns):
         self._ok(self._lib.mhd_set_sweep_length(self._h, int(columns)))
 
@@ -298,6 +317,15 @@ class ResistiveMHD2D:
         v = ctypes.c_int()
         self._ok(self._lib.mhd_get_sweep_length(self._h, ctypes.byref(v)))
         return v.value
+
+    def profile_enable(self, on=True):
+        self._ok(self._lib.mhd_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        """(ms of stage-1 kernels, ms of stage-2 kernels, steps timed)."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._ok(self._lib.mhd_profile_read(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
 
     # -- setups ---------------------------------------------------------
     def initOrszagTang(self):

@@ -25,8 +25,9 @@ def oracle_run(cfg, U0, steps, eta_field=None):
     return o
 
 
-def gpu_run(cfg, U0, steps, eta_field=None, sweep=None):
+def gpu_run(cfg, U0, steps, eta_field=None, sweep=None, numerics="fast"):
     sim = mhd.ResistiveMHD2D(cfg)
+    sim.set_numerics(numerics)
     sim.setU(U0)
     if eta_field is not None:
         sim.setEtaField(eta_field)

@@ -1,7 +1,12 @@
 """GPU parity: the HIP step against the CPU oracle on the same inputs.
 
-Tolerance: relative L2 error per conserved variable <= 1e-10 after the
-stated number of steps (the FP64 bar), and the dt sequence to 1e-12.
+Two arithmetic modes are checked:
+  exact -- the reference's expressions operation for operation: the state
+           after N steps must be bitwise identical to the oracle's;
+  fast  -- shared reciprocals / FMA: the reference's HLLD flux jumps where a
+           rotational wave speed crosses zero, so rounding-level differences
+           can flip a region; checked at relative L2 <= 1e-8 per variable and
+           dt, time to 1e-10.  Exact (the default) carries the 1e-10 bar.
 """
 import numpy as np
 import pytest
@@ -11,44 +16,68 @@ from tests.helpers import NAMES, boundary_applied, gpu_run, oracle_run, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-10
+TOL_FAST = 1e-8
 
 
-def _report(tag, err):
-    print(tag, " ".join(f"{n}={e:.2e}" for n, e in zip(NAMES, err)), flush=True)
-
-
-def _compare(cfg, U0, steps, eta=None, sweep=None, tag=""):
-    print(f"case {tag}: nx={cfg.nx} ny={cfg.ny} steps={steps} sweep={sweep}", flush=True)
+def _compare(cfg, U0, steps, numerics, eta=None, sweep=None, tag=""):
+    print(f"case {tag}: nx={cfg.nx} ny={cfg.ny} bc=({cfg.bc_x},{cfg.bc_y}) steps={steps} "
+          f"numerics={numerics} sweep={sweep}", flush=True)
     o = oracle_run(cfg, U0, steps, eta)
-    g = gpu_run(cfg, U0, steps, eta, sweep)
+    g = gpu_run(cfg, U0, steps, eta, sweep, numerics)
     Uo, Ug = o.get_state(), g.getU()
     err = rel_l2(Ug, Uo)
-    _report(tag, err)
+    print(tag, numerics, " ".join(f"{n}={e:.2e}" for n, e in zip(NAMES, err)), flush=True)
     assert np.all(np.isfinite(Ug))
-    assert np.all(err <= TOL), err
-    assert abs(g.dt - o.dt) <= 1e-12 * abs(o.dt)
-    assert abs(g.totalTime - o.time) <= 1e-12 * abs(o.time)
+    assert not g.checkCalculationIsCrashed()
+    if numerics == "exact":
+        assert np.array_equal(Ug, Uo), err
+        assert g.dt == o.dt and g.totalTime == o.time
+    else:
+        assert np.all(err <= TOL_FAST), err
+        assert abs(g.dt - o.dt) <= 1e-10 * abs(o.dt)
+        assert abs(g.totalTime - o.time) <= 1e-10 * abs(o.time)
     return Ug, Uo
 
 
-def test_orszag_tang_64_periodic():
+@pytest.mark.parametrize("numerics", ["exact", "fast"])
+def test_orszag_tang_64_periodic(numerics):
     cfg = mhd.orszag_tang_config(64, eta=1e-3)
     U0 = boundary_applied(cfg, mhd.fill_orszag_tang(cfg))
-    _compare(cfg, U0, 100, tag="ot64")
+    _compare(cfg, U0, 100, numerics, tag="ot64")
 
 
-def test_orszag_tang_ragged_symmetric_xy():
+@pytest.mark.parametrize("numerics", ["exact", "fast"])
+def test_orszag_tang_ragged_symmetric_xy(numerics):
     cfg = mhd.orszag_tang_config(70, eta=1e-3)
     cfg.ny = 70 + 6 + 13  # ragged strip count along y
     cfg.bc_x = mhd.MHD_BC_SYMMETRIC
     cfg.bc_y = mhd.MHD_BC_SYMMETRIC
     U0 = boundary_applied(cfg, mhd.fill_orszag_tang(cfg))
-    _compare(cfg, U0, 40, sweep=16, tag="ot-sym")
+    _compare(cfg, U0, 40, numerics, sweep=16, tag="ot-sym")
 
 
-def test_harris_small_eta_field():
+@pytest.mark.parametrize("numerics", ["exact", "fast"])
+def test_harris_small_eta_field(numerics):
     cfg = mhd.harris_config(128, 96)
     U0 = boundary_applied(cfg, mhd.fill_harris(cfg, seed=3, noise_amp=1e-2))
     eta = mhd.fill_harris_eta(cfg)
-    _compare(cfg, U0, 60, eta=eta, tag="harris")
+    _compare(cfg, U0, 60, numerics, eta=eta, tag="harris")
+
+
+@pytest.mark.parametrize("numerics", ["exact", "fast"])
+def test_orszag_tang_256_100_steps(numerics):
+    """The plain-method case at its full size: Orszag-Tang 256^2, eta 1e-3,
+    CFL 0.7, checked after 100 steps."""
+    cfg = mhd.orszag_tang_config(256, eta=1e-3)
+    U0 = boundary_applied(cfg, mhd.fill_orszag_tang(cfg))
+    _compare(cfg, U0, 100, numerics, tag="ot256")
+
+
+@pytest.mark.parametrize("sweep", [0, 8, 33, 1000])
+def test_sweep_lengths_agree(sweep):
+    """The result does not depend on how columns are split between
+    wavefronts (segment warm-up recomputes the same values)."""
+    cfg = mhd.harris_config(150, 70)
+    U0 = boundary_applied(cfg, mhd.fill_harris(cfg, seed=5, noise_amp=1e-2))
+    eta = mhd.fill_harris_eta(cfg)
+    _compare(cfg, U0, 20, "exact", eta=eta, sweep=sweep, tag=f"sweep{sweep}")
