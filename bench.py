@@ -32,6 +32,20 @@ def stage_bytes(zones, eta_field):
     return zones * (2 * STATE_B + e), zones * (3 * STATE_B + e)
 
 
+def measured_traffic(root, numerics, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
+    1 GiB device copy), or None if there is none."""
+    path = os.path.join(root, "profiles", f"rk_stage_kernel_harris4096x2048_{numerics}_hbm_counters.md")
+    if not os.path.exists(path):
+        return None
+    want = f"traffic_bytes_per_launch {kernel} ="
+    for line in open(path):
+        if line.startswith(want):
+            return float(line.split("=")[1])
+    return None
+
+
 def cpu_baseline(nx, ny, steps):
     from oracle.oracle import OracleSim, num_threads
     import resistivemhd_gpu_amd as mhd
@@ -67,6 +81,8 @@ def main():
     ap.add_argument("--cpu-ny", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (reference point for HBM byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -106,10 +122,14 @@ def main():
     b1, b2 = stage_bytes(zones, True)
     avg1, avg2 = ms1 / nt / 1e3, ms2 / nt / 1e3
     # dominant kernel = the stage with the larger summed time
+    exact = "true" if args.numerics == "exact" else "false"
     if avg2 >= avg1:
-        kname, kbytes, kavg = "rk_stage_kernel<2>", b2, avg2
+        kname, kbytes, kavg = f"rk_stage_kernel<2, {exact}>", b2, avg2
     else:
-        kname, kbytes, kavg = "rk_stage_kernel<1>", b1, avg1
+        kname, kbytes, kavg = f"rk_stage_kernel<1, {exact}>", b1, avg1
+    traffic = None
+    if (args.nx, args.ny) == (4096, 2048):
+        traffic = measured_traffic(root, args.numerics, kname)
     achieved = kbytes / kavg / 1e9
     line = {
         "metric": METRIC,
@@ -137,7 +157,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": kbytes,
             "avg_launch_ms": kavg * 1e3,
             "stage_ms": [ms1 / nt, ms2 / nt],
@@ -146,6 +166,15 @@ def main():
         "crashed": crashed,
     }
     del sim
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line["calibration_copy_bytes"] = n * 8
+        del src, dst
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_ny, args.cpu_steps)
     print(json.dumps(line), flush=True)

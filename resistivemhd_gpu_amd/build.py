@@ -32,18 +32,20 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
+def build(verbose=False, out=LIB, defines=()):
+    """Compiles the library to `out`; `defines` are extra -D settings
+    (e.g. MHD_WAVES_PER_EU_EXACT=1) for launch-shape experiments."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
+    tmp = out + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-D" + d for d in defines] + ["-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libmhd2d.so")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

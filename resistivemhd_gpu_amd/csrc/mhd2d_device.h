@@ -46,6 +46,50 @@ __device__ __forceinline__ double minmod(double x, double y, double eps)
     return s * tmax(tmin(fabs(x), s * y), eps);
 }
 
+// IEEE f64 division a/b as the compiler expands it (v_div_scale of the
+// denominator, reciprocal refined by two Newton steps, v_div_scale of the
+// numerator, one correction, v_div_fmas, v_div_fixup), with the reciprocal of
+// the scaled denominator computed once per denominator and reused.  A lane
+// whose scaled denominator differs from the shared one (div_scale rescales
+// only for extreme exponent gaps) recomputes it, so every quotient is
+// bitwise the compiler's a/b.
+struct SharedDen {
+    double sb;   // v_div_scale(b, b, a0)
+    double r;    // refined reciprocal of sb
+};
+
+__device__ __forceinline__ double div_recip_core(double sb)
+{
+    double r = __builtin_amdgcn_rcp(sb);
+    double e = fma(-sb, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-sb, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ SharedDen div_prepare(double a0, double b)
+{
+    bool unused;
+    SharedDen d;
+    d.sb = __builtin_amdgcn_div_scale(a0, b, false, &unused);
+    d.r = div_recip_core(d.sb);
+    return d;
+}
+
+__device__ __forceinline__ double div_by(double a, double b, const SharedDen &d)
+{
+    bool unused, flag;
+    const double sb = __builtin_amdgcn_div_scale(a, b, false, &unused);
+    double r = d.r;
+    const bool other = __double_as_longlong(sb) != __double_as_longlong(d.sb);
+    if (__builtin_expect(__any(other), 0) && other) r = div_recip_core(sb);
+    const double sa = __builtin_amdgcn_div_scale(a, b, true, &flag);
+    double q = sa * r;
+    const double e = fma(-sb, q, sa);
+    q = __builtin_amdgcn_div_fmas(e, r, q, flag);
+    return __builtin_amdgcn_div_fixup(q, b, a);
+}
+
 // P/hlld.cu:26-32
 __device__ __forceinline__ double sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
 
@@ -79,24 +123,30 @@ __device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, do
     const double rhoL = Lq.rho, uL = Lq.u, vL = Lq.v, wL = Lq.w, bXL = bX, bYL = Lq.by, bZL = Lq.bz, pL = Lq.p;
     const double rhoR = Rq.rho, uR = Rq.u, vR = Rq.v, wR = Rq.w, bXR = bX, bYR = Rq.by, bZR = Rq.bz, pR = Rq.p;
 
-    const double eL = pL / (gamma - 1.0)
+    // Divisions sharing a denominator reuse its reciprocal (div_by is the
+    // compiler's IEEE division bit for bit).
+    const double gm1 = gamma - 1.0;
+    const SharedDen dg = div_prepare(pL, gm1);
+    const double eL = div_by(pL, gm1, dg)
                     + 0.5 * rhoL * (uL * uL + vL * vL + wL * wL)
                     + 0.5 * (bXL * bXL + bYL * bYL + bZL * bZL);
     const double pTL = pL + 0.5 * (bXL * bXL + bYL * bYL + bZL * bZL);
-    const double eR = pR / (gamma - 1.0)
+    const double eR = div_by(pR, gm1, dg)
                     + 0.5 * rhoR * (uR * uR + vR * vR + wR * wR)
                     + 0.5 * (bXR * bXR + bYR * bYR + bZR * bZR);
     const double pTR = pR + 0.5 * (bXR * bXR + bYR * bYR + bZR * bZR);
 
-    const double csL = sqrt(gamma * pL / rhoL);
-    const double caL = sqrt((bXL * bXL + bYL * bYL + bZL * bZL) / rhoL);
-    const double vaL = sqrt(bXL * bXL / rhoL);
+    const SharedDen drL = div_prepare(gamma * pL, rhoL);
+    const double csL = sqrt(div_by(gamma * pL, rhoL, drL));
+    const double caL = sqrt(div_by(bXL * bXL + bYL * bYL + bZL * bZL, rhoL, drL));
+    const double vaL = sqrt(div_by(bXL * bXL, rhoL, drL));
     const double cfL = sqrt(0.5 * (csL * csL + caL * caL
                      + sqrt((csL * csL + caL * caL) * (csL * csL + caL * caL)
                      - 4.0 * csL * csL * vaL * vaL)));
-    const double csR = sqrt(gamma * pR / rhoR);
-    const double caR = sqrt((bXR * bXR + bYR * bYR + bZR * bZR) / rhoR);
-    const double vaR = sqrt(bXR * bXR / rhoR);
+    const SharedDen drR = div_prepare(gamma * pR, rhoR);
+    const double csR = sqrt(div_by(gamma * pR, rhoR, drR));
+    const double caR = sqrt(div_by(bXR * bXR + bYR * bYR + bZR * bZR, rhoR, drR));
+    const double vaR = sqrt(div_by(bXR * bXR, rhoR, drR));
     const double cfR = sqrt(0.5 * (csR * csR + caR * caR
                      + sqrt((csR * csR + caR * caR) * (csR * csR + caR * caR)
                      - 4.0 * csR * csR * vaR * vaR)));
@@ -106,43 +156,51 @@ __device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, do
     SL = tmin(SL, 0.0);
     SR = tmax(SR, 0.0);
 
-    const double SM = ((SR - uR) * rhoR * uR - (SL - uL) * rhoL * uL - pTR + pTL)
-                    / ((SR - uR) * rhoR - (SL - uL) * rhoL + eps);
-    const double pT1 = ((SR - uR) * rhoR * pTL - (SL - uL) * rhoL * pTR
-                     + rhoL * rhoR * (SR - uR) * (SL - uL) * (uR - uL))
-                     / ((SR - uR) * rhoR - (SL - uL) * rhoL + eps);
+    const double D0 = (SR - uR) * rhoR - (SL - uL) * rhoL + eps;
+    const double nSM = (SR - uR) * rhoR * uR - (SL - uL) * rhoL * uL - pTR + pTL;
+    const SharedDen d0 = div_prepare(nSM, D0);
+    const double SM = div_by(nSM, D0, d0);
+    const double pT1 = div_by((SR - uR) * rhoR * pTL - (SL - uL) * rhoL * pTR
+                              + rhoL * rhoR * (SR - uR) * (SL - uL) * (uR - uL), D0, d0);
 
-    const double rho1L = rhoL * (SL - uL) / (SL - SM + eps);
+    const double DSL = SL - SM + eps;
+    const double DL = rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps;
+    const double nrho1L = rhoL * (SL - uL);
+    const SharedDen dSL = div_prepare(nrho1L, DSL);
+    const double n_v1L = bXL * bYL * (SM - uL);
+    const SharedDen dL = div_prepare(n_v1L, DL);
+    const double rho1L = div_by(nrho1L, DSL, dSL);
     const double u1L = SM;
-    const double v1L = vL - bXL * bYL * (SM - uL) / (rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps);
-    const double w1L = wL - bXL * bZL * (SM - uL) / (rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps);
+    const double v1L = vL - div_by(n_v1L, DL, dL);
+    const double w1L = wL - div_by(bXL * bZL * (SM - uL), DL, dL);
     const double bX1L = bXL;
-    const double bY1L = bYL * (rhoL * (SL - uL) * (SL - uL) - bXL * bXL)
-                      / (rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps);
-    const double bZ1L = bZL * (rhoL * (SL - uL) * (SL - uL) - bXL * bXL)
-                      / (rhoL * (SL - uL) * (SL - SM) - bXL * bXL + eps);
-    const double e1L = ((SL - uL) * eL - pTL * uL + pT1 * SM
-                     + bXL * ((uL * bXL + vL * bYL + wL * bZL) - (u1L * bX1L + v1L * bY1L + w1L * bZ1L)))
-                     / (SL - SM + eps);
+    const double bY1L = div_by(bYL * (rhoL * (SL - uL) * (SL - uL) - bXL * bXL), DL, dL);
+    const double bZ1L = div_by(bZL * (rhoL * (SL - uL) * (SL - uL) - bXL * bXL), DL, dL);
+    const double e1L = div_by((SL - uL) * eL - pTL * uL + pT1 * SM
+                              + bXL * ((uL * bXL + vL * bYL + wL * bZL) - (u1L * bX1L + v1L * bY1L + w1L * bZ1L)),
+                              DSL, dSL);
 
-    const double rho1R = rhoR * (SR - uR) / (SR - SM + eps);
+    const double DSR = SR - SM + eps;
+    const double DR = rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps;
+    const double nrho1R = rhoR * (SR - uR);
+    const SharedDen dSR = div_prepare(nrho1R, DSR);
+    const double n_v1R = bXR * bYR * (SM - uR);
+    const SharedDen dR = div_prepare(n_v1R, DR);
+    const double rho1R = div_by(nrho1R, DSR, dSR);
     const double u1R = SM;
-    const double v1R = vR - bXR * bYR * (SM - uR) / (rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps);
-    const double w1R = wR - bXR * bZR * (SM - uR) / (rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps);
+    const double v1R = vR - div_by(n_v1R, DR, dR);
+    const double w1R = wR - div_by(bXR * bZR * (SM - uR), DR, dR);
     const double bX1R = bXR;
-    const double bY1R = bYR * (rhoR * (SR - uR) * (SR - uR) - bXR * bXR)
-                      / (rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps);
-    const double bZ1R = bZR * (rhoR * (SR - uR) * (SR - uR) - bXR * bXR)
-                      / (rhoR * (SR - uR) * (SR - SM) - bXR * bXR + eps);
-    const double e1R = ((SR - uR) * eR - pTR * uR + pT1 * SM
-                     + bXR * ((uR * bXR + vR * bYR + wR * bZR) - (u1R * bX1R + v1R * bY1R + w1R * bZ1R)))
-                     / (SR - SM + eps);
+    const double bY1R = div_by(bYR * (rhoR * (SR - uR) * (SR - uR) - bXR * bXR), DR, dR);
+    const double bZ1R = div_by(bZR * (rhoR * (SR - uR) * (SR - uR) - bXR * bXR), DR, dR);
+    const double e1R = div_by((SR - uR) * eR - pTR * uR + pT1 * SM
+                              + bXR * ((uR * bXR + vR * bYR + wR * bZR) - (u1R * bX1R + v1R * bY1R + w1R * bZ1R)),
+                              DSR, dSR);
 
     const double S1L = SM - sqrt(bX1L * bX1L / rho1L);
     const double S1R = SM + sqrt(bX1R * bX1R / rho1R);
 
-    // Region indicators (P/hlld.cu:64-69).  After the clamp SL <= 0 <= SR,
-    // so the outer-left region is never selected.
+    // Region indicators (P/hlld.cu:64-69).
     const bool in_oL = (SL > 0.0);
     const bool in_mL = (SL <= 0.0) && (0.0 < S1L);
     const bool in_iL = (S1L <= 0.0) && (0.0 < SM);
@@ -153,12 +211,13 @@ __device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, do
     const double sq1L = sqrt(rho1L), sq1R = sqrt(rho1R);
     const double sbx = sgn(bX1L);
     const double u2 = SM;
-    const double v2 = (sq1L * v1L + sq1R * v1R + (bY1R - bY1L) * sbx) / (sq1L + sq1R + eps);
-    const double w2 = (sq1L * w1L + sq1R * w1R + (bZ1R - bZ1L) * sbx) / (sq1L + sq1R + eps);
-    const double bY2 = (sq1L * bY1R + sq1R * bY1L + sqrt(rho1L * rho1R) * (v1R - v1L) * sbx)
-                     / (sq1L + sq1R + eps);
-    const double bZ2 = (sq1L * bZ1R + sq1R * bZ1L + sqrt(rho1L * rho1R) * (w1R - w1L) * sbx)
-                     / (sq1L + sq1R + eps);
+    const double D2 = sq1L + sq1R + eps;
+    const double n_v2 = sq1L * v1L + sq1R * v1R + (bY1R - bY1L) * sbx;
+    const SharedDen d2 = div_prepare(n_v2, D2);
+    const double v2 = div_by(n_v2, D2, d2);
+    const double w2 = div_by(sq1L * w1L + sq1R * w1R + (bZ1R - bZ1L) * sbx, D2, d2);
+    const double bY2 = div_by(sq1L * bY1R + sq1R * bY1L + sqrt(rho1L * rho1R) * (v1R - v1L) * sbx, D2, d2);
+    const double bZ2 = div_by(sq1L * bZ1R + sq1R * bZ1L + sqrt(rho1L * rho1R) * (w1R - w1L) * sbx, D2, d2);
     const double e2L = e1L - sq1L
                      * ((u1L * bX1L + v1L * bY1L + w1L * bZ1L) - (u2 * bXL + v2 * bY2 + w2 * bZ2))
                      * sgn(bXL);
@@ -166,20 +225,63 @@ __device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, do
                      * ((u1R * bX1R + v1R * bY1R + w1R * bZ1R) - (u2 * bXR + v2 * bY2 + w2 * bZ2))
                      * sgn(bXR);
 
-    const Flux8 oL = one_flux(rhoL, uL, vL, wL, bXL, bYL, bZL, eL, pTL);
-    const Flux8 mL = one_flux(rho1L, u1L, v1L, w1L, bXL, bY1L, bZ1L, e1L, pT1);
-    const Flux8 iL = one_flux(rho1L, u2, v2, w2, bXL, bY2, bZ2, e2L, pT1);
-    const Flux8 oR = one_flux(rhoR, uR, vR, wR, bXR, bYR, bZR, eR, pTR);
-    const Flux8 mR = one_flux(rho1R, u1R, v1R, w1R, bXR, bY1R, bZ1R, e1R, pT1);
-    const Flux8 iR = one_flux(rho1R, u2, v2, w2, bXR, bY2, bZ2, e2R, pT1);
+    // The reference sums all six candidates times 0/1 weights.  When every
+    // candidate is finite that sum equals the sum of the active candidates
+    // alone.  After the clamp SL <= 0 <= SR and with S1L <= SM <= S1R, at most
+    // one star region is active and the outer-right region (SR == 0) may be
+    // active beside it, so one candidate is evaluated (plus the right state's
+    // flux when SR == 0).  Lanes where that reasoning does not hold (a
+    // non-finite star state, or an overlap the ordering excludes) take the
+    // literal six-term sum below, so the result is the reference's in all cases.
+    const int nmain = (int)in_mL + (int)in_iL + (int)in_iR + (int)in_mR;
+    const double chk = rho1L + v1L + w1L + bY1L + bZ1L + e1L + rho1R + v1R + w1R + bY1R + bZ1R + e1R
+                     + v2 + w2 + bY2 + bZ2 + e2L + e2R + SM + pT1 + eL + eR + pTL + pTR;
+    const bool literal = in_oL || nmain > 1 || !(chk - chk == 0.0);
 
-    const double woL = (double)in_oL, wmL = (double)in_mL, wiL = (double)in_iL;
-    const double woR = (double)in_oR, wmR = (double)in_mR, wiR = (double)in_iR;
-#define MHD_SEL(c) (oL.c * woL + mL.c * wmL + iL.c * wiL + oR.c * woR + mR.c * wmR + iR.c * wiR)
     Flux8 f;
-    f.f0 = MHD_SEL(f0); f.f1 = MHD_SEL(f1); f.f2 = MHD_SEL(f2); f.f3 = MHD_SEL(f3);
-    f.f4 = MHD_SEL(f4); f.f5 = MHD_SEL(f5); f.f6 = MHD_SEL(f6); f.f7 = MHD_SEL(f7);
-#undef MHD_SEL
+    if (__builtin_expect(__any(literal), 0) && literal) {
+        // P/hlld.cu:35-107: ((((oL*w + mL*w) + iL*w) + oR*w) + mR*w) + iR*w,
+        // candidates evaluated one at a time (P/hlld.cu:537-542).
+        const double woL = (double)in_oL, wmL = (double)in_mL, wiL = (double)in_iL;
+        const double woR = (double)in_oR, wmR = (double)in_mR, wiR = (double)in_iR;
+        Flux8 c = one_flux(rhoL, uL, vL, wL, bXL, bYL, bZL, eL, pTL);
+        f.f0 = c.f0 * woL; f.f1 = c.f1 * woL; f.f2 = c.f2 * woL; f.f3 = c.f3 * woL;
+        f.f4 = c.f4 * woL; f.f5 = c.f5 * woL; f.f6 = c.f6 * woL; f.f7 = c.f7 * woL;
+#define MHD_ACC(C, W)                                                                       \
+        f.f0 = f.f0 + C.f0 * W; f.f1 = f.f1 + C.f1 * W; f.f2 = f.f2 + C.f2 * W;             \
+        f.f3 = f.f3 + C.f3 * W; f.f4 = f.f4 + C.f4 * W; f.f5 = f.f5 + C.f5 * W;             \
+        f.f6 = f.f6 + C.f6 * W; f.f7 = f.f7 + C.f7 * W;
+        c = one_flux(rho1L, u1L, v1L, w1L, bXL, bY1L, bZ1L, e1L, pT1);
+        MHD_ACC(c, wmL)
+        c = one_flux(rho1L, u2, v2, w2, bXL, bY2, bZ2, e2L, pT1);
+        MHD_ACC(c, wiL)
+        c = one_flux(rhoR, uR, vR, wR, bXR, bYR, bZR, eR, pTR);
+        MHD_ACC(c, woR)
+        c = one_flux(rho1R, u1R, v1R, w1R, bXR, bY1R, bZ1R, e1R, pT1);
+        MHD_ACC(c, wmR)
+        c = one_flux(rho1R, u2, v2, w2, bXR, bY2, bZ2, e2R, pT1);
+        MHD_ACC(c, wiR)
+#undef MHD_ACC
+    } else {
+        const bool left = in_mL || in_iL;
+        const bool inner = in_iL || in_iR;
+        const double rho = left ? rho1L : rho1R;
+        const double v = inner ? v2 : (left ? v1L : v1R);
+        const double w = inner ? w2 : (left ? w1L : w1R);
+        const double by = inner ? bY2 : (left ? bY1L : bY1R);
+        const double bz = inner ? bZ2 : (left ? bZ1L : bZ1R);
+        const double e = inner ? (left ? e2L : e2R) : (left ? e1L : e1R);
+        if (nmain == 1) {
+            f = one_flux(rho, SM, v, w, bX, by, bz, e, pT1);
+        } else {
+            f.f0 = f.f1 = f.f2 = f.f3 = f.f4 = f.f5 = f.f6 = f.f7 = 0.0;
+        }
+        if (__builtin_expect(__any(in_oR), 0) && in_oR) {
+            const Flux8 o = one_flux(rhoR, uR, vR, wR, bXR, bYR, bZR, eR, pTR);
+            f.f0 = f.f0 + o.f0; f.f1 = f.f1 + o.f1; f.f2 = f.f2 + o.f2; f.f3 = f.f3 + o.f3;
+            f.f4 = f.f4 + o.f4; f.f5 = f.f5 + o.f5; f.f6 = f.f6 + o.f6; f.f7 = f.f7 + o.f7;
+        }
+    }
     return f;
 }
 
@@ -188,12 +290,13 @@ __device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, do
 __device__ __forceinline__ double cell_dt_exact(double rho, double mx, double my, double mz,
                                                 double bX, double bY, double bZ, double e, const Phys &ph)
 {
-    const double u = mx / rho, v = my / rho, w = mz / rho;
+    const SharedDen dr = div_prepare(mx, rho);
+    const double u = div_by(mx, rho, dr), v = div_by(my, rho, dr), w = div_by(mz, rho, dr);
     const double p = (ph.gamma - 1.0)
                    * (e - 0.5 * rho * (u * u + v * v + w * w)
                    - 0.5 * (bX * bX + bY * bY + bZ * bZ));
-    const double cs = sqrt(ph.gamma * p / rho);
-    const double ca = sqrt((bX * bX + bY * bY + bZ * bZ) / rho);
+    const double cs = sqrt(div_by(ph.gamma * p, rho, dr));
+    const double ca = sqrt(div_by(bX * bX + bY * bY + bZ * bZ, rho, dr));
     const double maxSpeedX = fabs(u) + sqrt(cs * cs + ca * ca);
     const double maxSpeedY = fabs(v) + sqrt(cs * cs + ca * ca);
     return 1.0 / (maxSpeedX / ph.dx + maxSpeedY / ph.dy + ph.eps);

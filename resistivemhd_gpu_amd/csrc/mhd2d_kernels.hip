@@ -41,9 +41,10 @@ __device__ __forceinline__ Cell make_cell(double rho, double mx, double my, doub
     c.byc = byc;
     c.bz = bz;
     if (EXACT) {
-        c.u = mx / rho;
-        c.v = my / rho;
-        c.w = mz / rho;
+        const SharedDen dr = div_prepare(mx, rho);
+        c.u = div_by(mx, rho, dr);
+        c.v = div_by(my, rho, dr);
+        c.w = div_by(mz, rho, dr);
         // x sweep: GetBasicParamterFunctor on the centred state
         c.px = (ph.gamma - 1.0)
              * (e - 0.5 * (rho * (c.u * c.u + c.v * c.v + c.w * c.w))
@@ -66,8 +67,19 @@ __device__ __forceinline__ Cell make_cell(double rho, double mx, double my, doub
     return c;
 }
 
+// Resident wavefronts per SIMD the register allocation targets (the exact
+// arithmetic needs more live values than the fast one).
+#ifndef MHD_WAVES_PER_EU_EXACT
+#define MHD_WAVES_PER_EU_EXACT 2
+#endif
+#ifndef MHD_WAVES_PER_EU_FAST
+#define MHD_WAVES_PER_EU_FAST 2
+#endif
+
 template <int STAGE, bool EXACT>
-__global__ __launch_bounds__(64) void rk_stage_kernel(const double *__restrict__ uin, double *uout,
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
+void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                                                       const double *uold, const double *__restrict__ etaf,
                                                       double *__restrict__ side, Ctrl *__restrict__ ctrl,
                                                       Grid g, Phys ph, int cur)
@@ -440,15 +452,17 @@ __global__ __launch_bounds__(256) void dt_all_kernel(const double *__restrict__ 
 {
     double dloc = __longlong_as_double((long long)kInfBits);
     int nanflag = 0;
-    const int i = blockIdx.y;
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < g.ny) {
-        const size_t o = (size_t)i * g.pitch + j;
-        const double e = PL(u, 7)[o];
-        dloc = cell_dt<EXACT>(PL(u, 0)[o], PL(u, 1)[o], PL(u, 2)[o], PL(u, 3)[o], PL(u, 4)[o], PL(u, 5)[o],
-                              PL(u, 6)[o], e, ph);
-        if (!(dloc == dloc)) dloc = __longlong_as_double((long long)kInfBits);
-        nanflag = isnan(e) ? 1 : 0;   // checkCalculationIsCrashed, P/resistiveMHD_2D.cu:294-309
+    // grid-stride over rows (blockIdx.x) and columns (threads): a few
+    // thousand wavefronts each fold many cells before one atomic
+    for (int i = blockIdx.x; i < g.nx; i += gridDim.x) {
+        for (int j = threadIdx.x; j < g.ny; j += blockDim.x) {
+            const size_t o = (size_t)i * g.pitch + j;
+            const double e = PL(u, 7)[o];
+            const double d = cell_dt<EXACT>(PL(u, 0)[o], PL(u, 1)[o], PL(u, 2)[o], PL(u, 3)[o], PL(u, 4)[o],
+                                            PL(u, 5)[o], PL(u, 6)[o], e, ph);
+            if (d < dloc) dloc = d;
+            nanflag |= isnan(e) ? 1 : 0;   // checkCalculationIsCrashed, P/resistiveMHD_2D.cu:294-309
+        }
     }
     accumulate_dt(ctrl, slot, dloc);
     const unsigned long long any = __ballot(nanflag);
@@ -557,7 +571,7 @@ hipError_t launch_dt_all(bool exact, const double *u, Ctrl *ctrl, int slot, cons
                          hipStream_t s)
 {
     hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, ctrl, slot);
-    const dim3 grid((g.ny + 255) / 256, g.nx);
+    const dim3 grid(g.nx < 1024 ? g.nx : 1024);
     if (exact) hipLaunchKernelGGL((dt_all_kernel<true>), grid, dim3(256), 0, s, u, ctrl, slot, g, ph);
     else hipLaunchKernelGGL((dt_all_kernel<false>), grid, dim3(256), 0, s, u, ctrl, slot, g, ph);
     return hipGetLastError();

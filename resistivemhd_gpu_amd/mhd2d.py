@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmhd2d.so")
+# MHD2D_LIBRARY points at another build of the same library (launch-shape
+# experiments); by default the in-tree build is used.
+LIB_PATH = os.environ.get("MHD2D_LIBRARY") or os.path.join(HERE, "libmhd2d.so")
 
 MHD_BC_PERIODIC = 0
 MHD_BC_SYMMETRIC = 1
