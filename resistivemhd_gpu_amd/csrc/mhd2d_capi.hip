@@ -105,6 +105,11 @@ void layout(mhd_handle *h, int seg)
     }
     g.seg = seg < nx_int ? seg : nx_int;
     g.nsegs = (nx_int + g.seg - 1) / g.seg;
+    // MI355X: 256 CUs in 8 XCDs of 32
+    g.xcds = h->num_cu >= 64 ? h->num_cu / 32 : 1;
+    if (g.xcds < 1) g.xcds = 1;
+    const int items = g.nstrips * g.nsegs;
+    g.items_per_xcd = (items + g.xcds - 1) / g.xcds;
 }
 
 void set_phys(mhd_handle *h)

@@ -15,6 +15,8 @@ struct Grid {
     long long plane;     // doubles between planes
     int seg;             // columns one wavefront sweeps
     int nstrips, nsegs;  // work decomposition: strips along y, segments along x
+    int xcds;            // L2 domains workgroups are dealt to round-robin
+    int items_per_xcd;   // ceil(nstrips * nsegs / xcds)
 };
 
 // exact = true: operation-for-operation reference arithmetic (built without

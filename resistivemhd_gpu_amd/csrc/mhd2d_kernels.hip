@@ -85,7 +85,12 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                                                       Grid g, Phys ph, int cur)
 {
     const int lane = threadIdx.x;
-    const int item = blockIdx.x;
+    // Workgroups are dealt to the XCDs round-robin (block b runs on XCD
+    // b % xcds).  Each XCD gets a contiguous range of items (strips fastest),
+    // so neighbouring strips and segments, which read the same halo lines,
+    // share that XCD's L2.
+    const int item = (blockIdx.x % g.xcds) * g.items_per_xcd + blockIdx.x / g.xcds;
+    if (item >= g.nstrips * g.nsegs) return;
     const int sy = item % g.nstrips;
     const int sx = item / g.nstrips;
     const int j0 = kGhost + sy * kStripRows;
@@ -134,11 +139,14 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
     double dtloc = __longlong_as_double((long long)kInfBits);
     int nanflag = 0;
 
+    double eta_prev = 0.0;   // resistivity of cell c-1
+
     for (int c = i0 - 2; c <= iend + 1; ++c) {
         const size_t o = (size_t)c * g.pitch + jl;
         const double rho = Rh[o], mx = Mx[o], my = My[o], mz = Mz[o];
         const double bx = Bx[o], by = By[o], bz = Bz[o], e = En[o];
         const double bym = By[o - 1];
+        const double eta_c = etaf ? etaf[o] : ph.eta;
         const double bxc = 0.5 * (bx + bx_prev);
         bx_prev = bx;
         const double byc = 0.5 * (by + bym);
@@ -158,7 +166,7 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
             // resistive terms of cell c-1 (EX/bhattacharjee/main_bhattacharjee.cu:142-241)
             const double bxc_up = __shfl_down(B.bxc, 1, 64), bxc_dn = __shfl_up(B.bxc, 1, 64);
             const double bz_up = __shfl_down(B.bz, 1, 64), bz_dn = __shfl_up(B.bz, 1, 64);
-            const double eta = etaf ? etaf[(size_t)(c - 1) * g.pitch + jl] : ph.eta;
+            const double eta = eta_prev;
             double jX, jY, jZ;
             if (EXACT) {
                 jY = -(C.bz - A.bz) / two_dx;
@@ -255,14 +263,16 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                     if (STAGE == 1) {
                         // oneStepFirstFunctor (P/resistiveMHD_2D.cu:53-86) and
                         // CT_kernel (P/ct.cu:44-59)
-                        n0 = Rh[os] - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m);
-                        n1 = Mx[os] - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m);
-                        n2 = My[os] - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m);
-                        n3 = Mz[os] - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m);
-                        n4 = Bx[os] - dtdy * (E - Em);
-                        n5 = By[os] + dtdx * (E - Eprev);
-                        n6 = Bz[os] - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m);
-                        n7 = En[os] - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m);
+                        const double U0 = Rh[os], U1 = Mx[os], U2 = My[os], U3 = Mz[os];
+                        const double U4 = Bx[os], U5 = By[os], U6 = Bz[os], U7 = En[os];
+                        n0 = U0 - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m);
+                        n1 = U1 - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m);
+                        n2 = U2 - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m);
+                        n3 = U3 - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m);
+                        n4 = U4 - dtdy * (E - Em);
+                        n5 = U5 + dtdx * (E - Eprev);
+                        n6 = U6 - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m);
+                        n7 = U7 - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m);
                         if (upd_row) {
                             PL(uout, 0)[os] = n0; PL(uout, 1)[os] = n1; PL(uout, 2)[os] = n2; PL(uout, 3)[os] = n3;
                             PL(uout, 4)[os] = n4; PL(uout, 5)[os] = n5; PL(uout, 6)[os] = n6; PL(uout, 7)[os] = n7;
@@ -277,14 +287,18 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                         if (upd_row) {
                             // oneStepSecondFunctor (P/resistiveMHD_2D.cu:88-130);
                             // B is the stage-1 CT value (same EMF, same old B).
-                            n0 = 0.5 * (PL(uold, 0)[os] + Rh[os] - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m));
-                            n1 = 0.5 * (PL(uold, 1)[os] + Mx[os] - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m));
-                            n2 = 0.5 * (PL(uold, 2)[os] + My[os] - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m));
-                            n3 = 0.5 * (PL(uold, 3)[os] + Mz[os] - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m));
-                            n4 = Bx[os];
-                            n5 = By[os];
-                            n6 = 0.5 * (PL(uold, 6)[os] + Bz[os] - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m));
-                            n7 = 0.5 * (PL(uold, 7)[os] + En[os] - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m));
+                            const double S0 = Rh[os], S1 = Mx[os], S2 = My[os], S3 = Mz[os];
+                            const double S4 = Bx[os], S5 = By[os], S6 = Bz[os], S7 = En[os];
+                            const double O0 = PL(uold, 0)[os], O1 = PL(uold, 1)[os], O2 = PL(uold, 2)[os];
+                            const double O3 = PL(uold, 3)[os], O6 = PL(uold, 6)[os], O7 = PL(uold, 7)[os];
+                            n0 = 0.5 * (O0 + S0 - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m));
+                            n1 = 0.5 * (O1 + S1 - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m));
+                            n2 = 0.5 * (O2 + S2 - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m));
+                            n3 = 0.5 * (O3 + S3 - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m));
+                            n4 = S4;
+                            n5 = S5;
+                            n6 = 0.5 * (O6 + S6 - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m));
+                            n7 = 0.5 * (O7 + S7 - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m));
                             PL(uout, 0)[os] = n0; PL(uout, 1)[os] = n1; PL(uout, 2)[os] = n2; PL(uout, 3)[os] = n3;
                             PL(uout, 4)[os] = n4; PL(uout, 5)[os] = n5; PL(uout, 6)[os] = n6; PL(uout, 7)[os] = n7;
                             // fused epilogue: per-cell dt of the new state for
@@ -304,6 +318,7 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
         }
         A = B;
         B = C;
+        eta_prev = eta_c;
     }
 
     if (STAGE == 2) {
@@ -526,7 +541,7 @@ hipError_t launch_stage(int stage, bool exact, const double *uin, double *uout, 
                         const double *eta, double *side, Ctrl *ctrl, const Grid &g, const Phys &ph, int cur,
                         hipStream_t s)
 {
-    const int blocks = g.nstrips * g.nsegs;
+    const int blocks = g.xcds * g.items_per_xcd;
 #define MHD_LAUNCH_STAGE(ST, EX)                                                                               \
     hipLaunchKernelGGL((rk_stage_kernel<ST, EX>), dim3(blocks), dim3(64), 0, s, uin, uout, uold, eta, side, ctrl, \
                        g, ph, cur)

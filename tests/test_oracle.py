@@ -1,0 +1,272 @@
+"""Pins of the CPU oracle (oracle/mhd2d_oracle.c) against what the
+mathematics fixes, independent of the oracle's own expressions.
+
+The reference ships no tests, golden vectors or fixtures, so each oracle
+function is checked against closed forms, invariants and special cases:
+  minmod            -- truth table of the limiter including the EPS floor
+  HLLD              -- consistency F(U,U) = ideal-MHD flux (textbook formula
+                       written here), exact resolution of stationary contacts,
+                       mirror symmetry under x -> -x
+  resistive flux    -- linear B: constant current, closed-form eta*J flux
+  dt                -- closed form for a uniform state
+  RK2 + CT + BC     -- conservation of mass, momentum and energy under
+                       periodic boundaries, div B held to round-off, a uniform
+                       state stays bitwise uniform, boundary copy/mirror
+                       identities, div-free rebuilt ghost faces, the stencil
+                       extent of one step.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from oracle.oracle import OracleSim, BC_PERIODIC, BC_SYMMETRIC
+
+GAMMA = 5.0 / 3.0
+EPS = 1e-20
+
+
+# ----------------------------------------------------------------- minmod
+@pytest.mark.parametrize("x,y,expect", [
+    (1.0, 2.0, 1.0), (2.0, 1.0, 1.0), (-1.0, -3.0, -1.0), (-3.0, -1.0, -1.0),
+    (1.0, -1.0, EPS), (-1.0, 1.0, -EPS), (3.0, 0.0, EPS), (-3.0, 0.0, -EPS),
+    (0.0, 5.0, 0.0), (0.0, -5.0, 0.0), (1e-30, 1.0, EPS), (2.5, 2.5, 2.5),
+])
+def test_minmod_truth_table(x, y, expect):
+    # limiter: sign(x) * max(min(|x|, sign(x) y), EPS); equal signs give the
+    # smaller magnitude, opposite signs give sign(x)*EPS, x == 0 gives 0
+    assert orc.minmod(x, y, EPS) == expect
+
+
+# ------------------------------------------------------------------- HLLD
+def ideal_mhd_flux(rho, u, v, w, bx, by, bz, p, gamma=GAMMA):
+    """Textbook x-flux of ideal MHD in conservative variables."""
+    b2 = bx * bx + by * by + bz * bz
+    pt = p + 0.5 * b2
+    e = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w) + 0.5 * b2
+    return np.array([
+        rho * u,
+        rho * u * u + pt - bx * bx,
+        rho * u * v - bx * by,
+        rho * u * w - bx * bz,
+        0.0,
+        u * by - v * bx,
+        u * bz - w * bx,
+        (e + pt) * u - bx * (u * bx + v * by + w * bz),
+    ])
+
+
+def _states(seed, n):
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        rho = rng.uniform(0.1, 3.0)
+        u, v, w = rng.uniform(-2, 2, 3)
+        bx, by, bz = rng.uniform(-2, 2, 3)
+        p = rng.uniform(0.05, 3.0)
+        yield np.array([rho, u, v, w, bx, by, bz, p])
+
+
+def test_hlld_consistency():
+    for q in _states(1, 200):
+        f = orc.hlld_flux(q, q)
+        ref = ideal_mhd_flux(*q)
+        np.testing.assert_allclose(f, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("bx", [0.0, 0.7, -1.3])
+def test_hlld_resolves_stationary_contact(bx):
+    # density jump with equal velocity (at rest), pressure and field: the
+    # exact solution is a stationary contact; HLLD resolves it exactly
+    rng = np.random.default_rng(7)
+    for _ in range(50):
+        by, bz = rng.uniform(-1, 1, 2)
+        p = rng.uniform(0.2, 2.0)
+        L = np.array([rng.uniform(0.5, 2.0), 0.0, 0.0, 0.0, bx, by, bz, p])
+        R = L.copy()
+        R[0] = rng.uniform(0.05, 0.4)
+        f = orc.hlld_flux(L, R)
+        ref = ideal_mhd_flux(*L)  # zero velocity: the flux is the same on both sides
+        np.testing.assert_allclose(f, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_hlld_mirror_symmetry():
+    # x -> -x maps (u, Bx) -> (-u, -Bx) and the flux to
+    # (-f0, f1, -f2, -f3, 0, -f5, -f6, -f7) with left and right swapped
+    sgn = np.array([-1, 1, -1, -1, 1, -1, -1, -1.0])
+    for L, R in zip(_states(11, 300), _states(12, 300)):
+        R = R.copy()
+        R[4] = L[4]                      # one normal field per interface
+        m = lambda q: q * np.array([1, -1, 1, 1, -1, 1, 1, 1.0])
+        f = orc.hlld_flux(L, R)
+        g = orc.hlld_flux(m(R), m(L))
+        np.testing.assert_allclose(g, sgn * f, rtol=1e-9, atol=1e-11 * (1 + np.abs(f).max()))
+
+
+# --------------------------------------------------------------------- dt
+def test_dt_uniform_closed_form():
+    for q in _states(3, 50):
+        rho, u, v, w, bx, by, bz, p = q
+        e = p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v + w * w) + 0.5 * (bx * bx + by * by + bz * bz)
+        cell = np.array([rho, rho * u, rho * v, rho * w, bx, by, bz, e])
+        dx, dy = 0.1, 0.05
+        c = np.sqrt(GAMMA * p / rho + (bx * bx + by * by + bz * bz) / rho)
+        expect = 1.0 / ((abs(u) + c) / dx + (abs(v) + c) / dy)
+        assert orc.cell_dt(cell, GAMMA, dx, dy) == pytest.approx(expect, rel=1e-13)
+
+
+def _uniform_state(nx, ny, rho=1.2, u=0.3, v=-0.2, w=0.1, bx=0.5, by=-0.4, bz=0.2, p=0.8):
+    e = p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v + w * w) + 0.5 * (bx * bx + by * by + bz * bz)
+    U = np.empty((nx, ny, 8))
+    U[...] = [rho, rho * u, rho * v, rho * w, bx, by, bz, e]
+    return U
+
+
+def test_calculate_dt_with_diffusion_limit():
+    nx, ny, dx, dy = 20, 16, 0.1, 0.05
+    U = _uniform_state(nx, ny)
+    for eta, cfl in [(0.0, 0.7), (1e-3, 0.7), (0.5, 0.4)]:
+        o = OracleSim(nx, ny, dx, dy, GAMMA, cfl, EPS, eta)
+        o.set_state(U)
+        c = orc.cell_dt(U[0, 0], GAMMA, dx, dy)
+        lim = min(dx * dx / eta, dy * dy / eta) if eta > 0 else np.inf
+        assert o.calculate_dt() == pytest.approx(cfl * min(c, lim), rel=1e-15)
+
+
+# --------------------------------------------------------- resistive flux
+def test_resistive_flux_linear_field():
+    # rho, p uniform, v = 0, Bx = 0, By = a*x on the y-faces: J_z = a is
+    # uniform, the HLLD part vanishes (MUSCL reconstructs a linear By
+    # exactly so left and right states agree and u = v = 0), so
+    # F.f5 = -eta*a, F.f6 = 0, F.f7 = -eta*a*(By_i + By_{i+1})/2 on every
+    # interior x-face, and G.f4 = +eta*a on every interior y-face.
+    nx, ny, dx, dy, eta, a = 20, 18, 0.1, 0.1, 3e-3, 0.8
+    U = _uniform_state(nx, ny, u=0.0, v=0.0, w=0.0, bx=0.0, by=0.0, bz=0.0, p=1.0)
+    x = (np.arange(nx) - 2.5) * dx
+    U[:, :, 5] = a * x[:, None]
+    U[:, :, 7] += 0.5 * U[:, :, 5] ** 2
+    o = OracleSim(nx, ny, dx, dy, GAMMA, 0.7, EPS, eta)
+    F, G = o.compute_fluxes(U)
+    i, j = slice(3, nx - 4), slice(3, ny - 4)
+    np.testing.assert_allclose(F[i, j, 5], -eta * a, rtol=1e-12)
+    np.testing.assert_allclose(F[i, j, 6], 0.0, atol=1e-15)
+    byc = U[:, :, 5]
+    expect7 = -eta * a * 0.5 * (byc[3:nx - 4] + byc[4:nx - 3])
+    np.testing.assert_allclose(F[i, j, 7], expect7[:, j] if expect7.ndim == 2 else expect7, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(G[i, j, 4], eta * a, rtol=1e-12)
+
+
+# ----------------------------------------------- whole step: invariants
+def _ot_state(n, seed=None):
+    import resistivemhd_gpu_amd as mhd
+    cfg = mhd.orszag_tang_config(n, eta=1e-3)
+    U = mhd.fill_orszag_tang(cfg)
+    o = OracleSim.from_config(cfg)
+    o.set_state(U)
+    o.apply_boundary()
+    return cfg, o
+
+
+def _divb(U, dx, dy):
+    bx, by = U[:, :, 4], U[:, :, 5]
+    return (bx[3:-3, 3:-3] - bx[2:-4, 3:-3]) / dx + (by[3:-3, 3:-3] - by[3:-3, 2:-4]) / dy
+
+
+def test_conservation_periodic():
+    cfg, o = _ot_state(48)
+    U0 = o.get_state()
+    o.step(25)
+    U1 = o.get_state()
+    for v in (0, 1, 2, 3, 7):
+        s0, s1 = U0[3:-3, 3:-3, v].sum(), U1[3:-3, 3:-3, v].sum()
+        scale = np.abs(U0[3:-3, 3:-3, v]).sum()
+        assert abs(s1 - s0) <= 1e-12 * scale, (v, s0, s1)
+
+
+def test_div_b_preserved():
+    cfg, o = _ot_state(40)
+    # a state whose div B is not zero stays at its own value: CT changes
+    # face B by a discrete curl, whose divergence vanishes identically
+    U = o.get_state()
+    rng = np.random.default_rng(5)
+    U[:, :, 4] += 1e-3 * rng.standard_normal(U.shape[:2])
+    o.set_state(U)
+    o.apply_boundary()
+    U0 = o.get_state()
+    d0 = _divb(U0, cfg.dx, cfg.dy)
+    o.step(10)
+    d1 = _divb(o.get_state(), cfg.dx, cfg.dy)
+    assert np.abs(d1 - d0).max() <= 1e-10 * np.abs(d0).max()
+
+
+def test_uniform_state_stays_uniform():
+    nx, ny = 24, 20
+    U = _uniform_state(nx, ny)
+    o = OracleSim(nx, ny, 0.1, 0.1, GAMMA, 0.7, EPS, 1e-3)
+    o.set_state(U)
+    o.apply_boundary()
+    o.step(5)
+    V = o.get_state()
+    assert np.array_equal(V[3:-3, 3:-3], U[3:-3, 3:-3])
+
+
+def test_stencil_extent_one_step():
+    # a perturbation of one interior cell reaches at most 6 cells away in
+    # one RK2 step (3 per stage: MUSCL + face flux + CT/resistive corner).
+    # eta is large enough that dt is the diffusion limit, which does not
+    # depend on the state (otherwise the perturbed cell could move the
+    # global CFL minimum and change every cell).
+    cfg, o = _ot_state(40)
+    cfg.eta = 2.0
+    U0 = o.get_state()
+    o = OracleSim.from_config(cfg)
+    o.set_state(U0)
+    o.step(1)
+    A = o.get_state()
+    U1 = U0.copy()
+    ic, jc = 22, 21
+    U1[ic, jc, 0] *= 1.0 + 1e-6
+    o2 = OracleSim.from_config(cfg)
+    o2.set_state(U1)
+    o2.step(1)
+    B = o2.get_state()
+    changed = np.argwhere(np.any(A[3:-3, 3:-3] != B[3:-3, 3:-3], axis=2)) + 3
+    assert len(changed) > 0
+    assert np.abs(changed[:, 0] - ic).max() <= 6
+    assert np.abs(changed[:, 1] - jc).max() <= 6
+
+
+# -------------------------------------------------------------- boundary
+def test_periodic_boundary_copies():
+    nx, ny = 18, 16
+    rng = np.random.default_rng(2)
+    U = rng.standard_normal((nx, ny, 8))
+    o = OracleSim(nx, ny, 0.1, 0.1)
+    o.set_state(U)
+    o.apply_boundary()
+    V = o.get_state()
+    it = V[3:nx - 3, 3:ny - 3]
+    # every ghost cell equals the interior cell one period away (x first,
+    # then y over full columns, which fills the corners)
+    for i in range(nx):
+        for j in range(ny):
+            ii = (i - 3) % (nx - 6)
+            jj = (j - 3) % (ny - 6)
+            assert np.array_equal(V[i, j], it[ii, jj])
+
+
+def test_symmetric_boundary_mirror_and_divfree_ghosts():
+    nx, ny, dx, dy = 20, 18, 0.1, 0.07
+    rng = np.random.default_rng(4)
+    U = rng.standard_normal((nx, ny, 8))
+    o = OracleSim(nx, ny, dx, dy, bc_x=BC_SYMMETRIC, bc_y=BC_SYMMETRIC)
+    o.set_state(U)
+    o.apply_boundary_axis(0, BC_SYMMETRIC)
+    V = o.get_state()
+    # mirror of everything but Bx (normal momentum included, no sign flip)
+    for v in (0, 1, 2, 3, 5, 6, 7):
+        for g, s in ((0, 5), (1, 4), (2, 3), (nx - 1, nx - 6), (nx - 2, nx - 5), (nx - 3, nx - 4)):
+            assert np.array_equal(V[g, :, v], V[s, :, v])
+    # rebuilt ghost Bx makes the ghost cells divergence-free
+    bx, by = V[:, :, 4], V[:, :, 5]
+    for i in (1, 2, 3, nx - 3, nx - 2, nx - 1):
+        div = (bx[i, 1:] - bx[i - 1, 1:]) / dx + (by[i, 1:] - by[i, :-1]) / dy
+        assert np.abs(div).max() <= 1e-12 * (1 + np.abs(by).max() / dy)
