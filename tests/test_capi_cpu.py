@@ -92,3 +92,40 @@ def test_fill_rejects_bad_config():
     cfg.nx = 5
     with pytest.raises(mhd.MHDError):
         mhd.fill_orszag_tang(cfg)
+
+
+CPP_MAIN = r"""
+#include <cstdio>
+#include <vector>
+#include "mhd2d.hpp"
+// the reference example's time loop (EX/bhattacharjee/main_bhattacharjee.cu:264-296)
+int main() {
+    mhd_config cfg;
+    mhd_default_config(&cfg, 64, 64, 0.0, 1.0, 0.0, 1.0);
+    cfg.eta = 2e-4;
+    mhd2d::ResistiveMHD2D sim(cfg);
+    std::vector<double> U(sim.size());
+    mhd_fill_orszag_tang(&cfg, U.data());
+    sim.initializeU(U);
+    double totalTime = 0.0;
+    for (int step = 0; step < 3; ++step) {
+        if (step % 100 == 0) sim.save(".", "ot", step);
+        sim.oneStepRK2();
+        if (sim.checkCalculationIsCrashed()) return 1;
+        totalTime += sim.lastDt();
+    }
+    std::printf("%g\n", totalTime);
+    return 0;
+}
+"""
+
+
+def test_cpp_wrapper_compiles_and_links(tmp_path, mhdlib):
+    import subprocess
+    src = tmp_path / "main.cpp"
+    src.write_text(CPP_MAIN)
+    libdir = os.path.join(ROOT, "resistivemhd_gpu_amd")
+    res = subprocess.run(["g++", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), str(src),
+                          "-L", libdir, "-lmhd2d", "-Wl,-rpath," + libdir, "-o", str(tmp_path / "main")],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
