@@ -61,6 +61,12 @@ struct oracle_sim {
     /* per-cell resistivity sampled at cell (i, j): plays the role of the
      * example's getEta(i*dx, j*dy) (EX/plasmoid/main_plasmoid.cu:159-172) */
     double *etaField;
+    /* scheme: ORACLE_SCHEME_SINGLE (the single-GPU library) or
+     * ORACLE_SCHEME_MULTI_SHARE (the MPI library's step on one rank) */
+    int scheme;
+    /* multi-GPU library CT arrays, M/ct.cu:4-23 */
+    double *oNF5, *oNG4, *oF5, *oG4, *oNF0, *oNG0;
+    double *nNF5, *nNG4, *nF5, *nG4, *nNF0, *nNG0;
 };
 
 /* ---------------------------------------------------------------- minmod */
@@ -722,6 +728,277 @@ static void one_step_rk2(struct oracle_sim *s)
     s->steps += 1;
 }
 
+/* ================================================================ */
+/* Multi-GPU library step on one rank (one GPU's share of the       */
+/* decomposition).  M = ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU */
+/* Same HLLD, MUSCL and primitives as above (expression for          */
+/* expression; the M/ kernels only differ in their element ranges,   */
+/* which matter for ghost cells the halo overwrites).                */
+/* ================================================================ */
+
+static inline int msign(double x) { return (x > 0.0) - (x < 0.0); }   /* M/sign.cu */
+
+/* M/calculate_half_Q.cu:10-48 + M/muscl.cu (x: shift ny, y: shift 1),
+ * with the M/ element ranges. */
+static void m_hlld_sweep(struct oracle_sim *s, const Cons *U, int shift, Flux *out)
+{
+    int nx = s->nx, ny = s->ny;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx - 1; ++i)
+        for (int j = 0; j < ny - 1; ++j) {
+            int k = j + i * ny;
+            s->dQCenter[k] = basic_param(&U[k], &U[k + shift], s->gamma);
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j) {
+            int k = j + i * ny;
+            s->dQLeft[k] = left_param(&s->dQCenter[k - shift], &s->dQCenter[k], &s->dQCenter[k + shift], s->eps);
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx - 2; ++i)
+        for (int j = 0; j < ny - 2; ++j) {
+            int k = j + i * ny;
+            s->dQRight[k] = right_param(&s->dQCenter[k], &s->dQCenter[k + shift], &s->dQCenter[k + 2 * shift], s->eps);
+        }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < s->N; ++k)
+        out[k] = hlld_flux(&s->dQLeft[k], &s->dQRight[k], s->gamma, s->eps);
+}
+
+/* M/flux_solver.cu:73-104 (eta uniform: getEta at face positions) */
+static void m_resistive_x(struct oracle_sim *s, const Cons *U, Flux *flux)
+{
+    int nx = s->nx, ny = s->ny;
+    double dx = s->dx, dy = s->dy;
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx - 2; ++i)
+        for (int j = 1; j < ny - 1; ++j) {
+            int index = j + i * ny;
+            double jY, jZ, eta, etaJY, etaJZ, etaJYBZ, etaJZBY;
+            jY = -(U[index + ny].bZ - U[index].bZ) / dx;
+            jZ = 0.5 * (
+                 (U[index + ny].bY - U[index - ny].bY) / (2.0 * dx) - (U[index + 1].bX - U[index - 1].bX) / (2.0 * dy)
+               + (U[index + 2 * ny].bY - U[index].bY) / (2.0 * dx) - (U[index + ny + 1].bX - U[index + ny - 1].bX) / (2.0 * dy)
+            );
+            eta = s->eta;
+            etaJY = eta * jY;
+            etaJZ = eta * jZ;
+            etaJYBZ = etaJY * 0.5 * (U[index].bZ + U[index + ny].bZ);
+            etaJZBY = etaJZ * 0.5 * (U[index].bY + U[index + ny].bY);
+            flux[index].f5 -= etaJZ;
+            flux[index].f6 += etaJY;
+            flux[index].f7 += etaJYBZ - etaJZBY;
+        }
+}
+
+/* M/flux_solver.cu:158-189 */
+static void m_resistive_y(struct oracle_sim *s, const Cons *U, Flux *flux)
+{
+    int nx = s->nx, ny = s->ny;
+    double dx = s->dx, dy = s->dy;
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 2; ++j) {
+            int index = j + i * ny;
+            double jX, jZ, eta, etaJX, etaJZ, etaJZBX, etaJXBZ;
+            jX = (U[index + 1].bZ - U[index].bZ) / dy;
+            jZ = 0.5 * (
+                 (U[index + ny].bY - U[index - ny].bY) / (2.0 * dx) - (U[index + 1].bX - U[index - 1].bX) / (2.0 * dy)
+               + (U[index + ny + 1].bY - U[index - ny + 1].bY) / (2.0 * dx) - (U[index + 2].bX - U[index].bX) / (2.0 * dy)
+            );
+            eta = s->eta;
+            etaJX = eta * jX;
+            etaJZ = eta * jZ;
+            etaJXBZ = etaJX * 0.5 * (U[index].bZ + U[index + 1].bZ);
+            etaJZBX = etaJZ * 0.5 * (U[index].bX + U[index + 1].bX);
+            flux[index].f4 += etaJZ;
+            flux[index].f6 -= etaJX;
+            flux[index].f7 += etaJZBX - etaJXBZ;
+        }
+}
+
+/* FluxSolver::getFluxF/G of M/flux_solver.cu:11-34 with M/hlld.cu:105-245 */
+static void m_get_flux_f(struct oracle_sim *s, const Cons *U, Flux *out)
+{
+    m_hlld_sweep(s, U, s->ny, s->hlldFlux);
+    memcpy(out, s->hlldFlux, sizeof(Flux) * s->N);
+    m_resistive_x(s, U, out);
+}
+
+static void m_get_flux_g(struct oracle_sim *s, const Cons *U, Flux *out)
+{
+    int N = s->N;
+    Cons *t = s->tmpUForFluxG;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < N; ++k) {                         /* M/hlld.cu:131-152 */
+        t[k].rho = U[k].rho;
+        t[k].rhoU = U[k].rhoV; t[k].rhoV = U[k].rhoW; t[k].rhoW = U[k].rhoU;
+        t[k].bX = U[k].bY; t[k].bY = U[k].bZ; t[k].bZ = U[k].bX;
+        t[k].e = U[k].e;
+    }
+    m_hlld_sweep(s, t, 1, s->hlldFlux);
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < N; ++k) {                         /* M/hlld.cu:175-200 */
+        Flux f = s->hlldFlux[k], g;
+        g.f0 = f.f0; g.f1 = f.f3; g.f2 = f.f1; g.f3 = f.f2;
+        g.f4 = f.f6; g.f5 = f.f4; g.f6 = f.f5; g.f7 = f.f7;
+        out[k] = g;
+    }
+    m_resistive_y(s, U, out);
+}
+
+/* CT::setOldFlux2D / setNowFlux2D: M/ct.cu:26-65 */
+static void m_set_flux(struct oracle_sim *s, const Flux *F, const Flux *G, const Cons *U,
+                       double *NF5, double *NG4, double *F5, double *G4, double *NF0, double *NG0)
+{
+    int nx = s->nx, ny = s->ny;
+    double dx = s->dx, dy = s->dy;
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j) {
+            int index = j + i * ny;
+            double rho = U[index].rho;
+            double u = U[index].rhoU / rho;
+            double v = U[index].rhoV / rho;
+            double bX = 0.5 * (U[index].bX + U[index - ny].bX);
+            double bY = 0.5 * (U[index].bY + U[index - 1].bY);
+            double jZ = 0.25 * (
+                (U[index + ny].bY - U[index].bY) / dx - (U[index + 1].bX - U[index].bX) / dy
+              + (U[index - 1 + ny].bY - U[index - 1].bY) / dx - (U[index].bX - U[index - 1].bX) / dy
+              + (U[index - 1].bY - U[index - 1 - ny].bY) / dx - (U[index - ny].bX - U[index - ny - 1].bX) / dy
+              + (U[index].bY - U[index - ny].bY) / dx - (U[index + 1 - ny].bX - U[index - ny].bX) / dy
+            );
+            double eta = s->eta;
+            NF5[index] = F[index].f5;
+            NG4[index] = G[index].f4;
+            F5[index] = u * bY - v * bX - eta * jZ;
+            G4[index] = -(u * bY - v * bX - eta * jZ);
+            NF0[index] = F[index].f0;
+            NG0[index] = G[index].f0;
+        }
+}
+
+/* CT-contact corner EMF of one flux set: M/ct.cu:147-169 */
+static inline double m_emf(int index, int ny, const double *NF5, const double *NG4, const double *F5,
+                           const double *G4, const double *NF0, const double *NG0)
+{
+    double avg = 0.25 * (
+        - NF5[index] - NF5[index + 1]
+        + NG4[index] + NG4[index + ny]
+    );
+    double eS = (1.0 + msign(NF0[index])) * (NG4[index] - G4[index])
+              + (1.0 - msign(NF0[index])) * (NG4[index + ny] - G4[index + ny]);
+    double eN = (1.0 + msign(NF0[index + 1])) * (G4[index + 1] - NG4[index])
+              + (1.0 - msign(NF0[index + 1])) * (G4[index + 1 + ny] - NG4[index + ny]);
+    double eW = -(1.0 + msign(NG0[index])) * (NF5[index] - F5[index])
+               -(1.0 - msign(NG0[index])) * (NF5[index + 1] - F5[index + 1]);
+    double eE = -(1.0 + msign(NG0[index + ny])) * (F5[index + ny] - NF5[index])
+               -(1.0 - msign(NG0[index + ny])) * (F5[index + 1 + ny] - NF5[index + 1]);
+    return avg + 0.125 * (eS - eN + eW - eE);
+}
+
+/* CT::divBClean: M/ct.cu:120-218 */
+static void m_div_b_clean(struct oracle_sim *s, Cons *U)
+{
+    int nx = s->nx, ny = s->ny;
+    double *eZ = s->eZVector;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx - 1; ++i)
+        for (int j = 0; j < ny - 1; ++j) {
+            int index = j + i * ny;
+            double eOld = m_emf(index, ny, s->oNF5, s->oNG4, s->oF5, s->oG4, s->oNF0, s->oNG0);
+            double eNow = m_emf(index, ny, s->nNF5, s->nNG4, s->nF5, s->nG4, s->nNF0, s->nNG0);
+            eZ[index] = 0.5 * (eOld + eNow);
+        }
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j) {
+            int k = j + i * ny;
+            U[k].bX = s->bXOld[k] - s->dt / s->dy * (eZ[k] - eZ[k - 1]);
+            U[k].bY = s->bYOld[k] + s->dt / s->dx * (eZ[k] - eZ[k - ny]);
+        }
+}
+
+/* sendrecv_U on one rank (M/mpi.cu:104-188): the rank is its own left,
+ * right, up and down neighbour.  x: interior rows of the 3 edge columns;
+ * then y: the 3 edge rows of every column. */
+static void m_halo(struct oracle_sim *s, Cons *U)
+{
+    int nx = s->nx, ny = s->ny, b = 3, lnx = nx - 6, lny = ny - 6;
+    for (int i = 0; i < b; ++i)
+        for (int j = 0; j < lny; ++j) {
+            Cons r = U[j + b + (lnx + i) * ny];        /* sent right, received from the left */
+            Cons l = U[j + b + (b + i) * ny];          /* sent left, received from the right */
+            U[j + b + i * ny] = r;
+            U[j + b + (lnx + b + i) * ny] = l;
+        }
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < b; ++j) {
+            Cons d = U[j + lny + i * ny];
+            Cons u = U[j + b + i * ny];
+            U[j + i * ny] = d;
+            U[j + lny + b + i * ny] = u;
+        }
+}
+
+/* calculateDt: M/resistiveMHD2D.cu:489-556 -- interior cells only, no
+ * diffusion limit. */
+static double m_calculate_dt(struct oracle_sim *s)
+{
+    int nx = s->nx, ny = s->ny;
+    double m = cell_dt(&s->U[3 + 3 * ny], s->gamma, s->dx, s->dy, s->eps);
+    for (int i = 3; i < nx - 3; ++i)
+        for (int j = 3; j < ny - 3; ++j) {
+            double d = cell_dt(&s->U[j + i * ny], s->gamma, s->dx, s->dy, s->eps);
+            if (d < m) m = d;
+        }
+    s->dt = m * s->cfl;
+    return s->dt;
+}
+
+/* oneStepFirst_kernel / oneStepSecond_kernel ranges: 0 < i, 0 < j */
+static void m_one_step_rk2(struct oracle_sim *s)
+{
+    int N = s->N, nx = s->nx, ny = s->ny;
+    Cons *U = s->U, *UBar = s->UBar;
+    for (int k = 0; k < N; ++k) { s->bXOld[k] = U[k].bX; s->bYOld[k] = U[k].bY; }
+    memcpy(UBar, U, sizeof(Cons) * N);
+    m_calculate_dt(s);
+
+    shift_to_center(s, U);
+    m_get_flux_f(s, U, s->fluxF);
+    m_get_flux_g(s, U, s->fluxG);
+    back_to_center_half(s, U);
+    m_set_flux(s, s->fluxF, s->fluxG, U, s->oNF5, s->oNG4, s->oF5, s->oG4, s->oNF0, s->oNG0);
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j) {
+            int k = j + i * ny;
+            UBar[k] = stage_first(&U[k], &s->fluxF[k], &s->fluxF[k - ny], &s->fluxG[k], &s->fluxG[k - 1],
+                                  s->dt, s->dx, s->dy);
+        }
+    m_halo(s, UBar);
+
+    shift_to_center(s, UBar);
+    m_get_flux_f(s, UBar, s->fluxF);
+    m_get_flux_g(s, UBar, s->fluxG);
+    back_to_center_half(s, UBar);
+#pragma omp parallel for schedule(static)
+    for (int i = 1; i < nx; ++i)
+        for (int j = 1; j < ny; ++j) {
+            int k = j + i * ny;
+            U[k] = stage_second(&U[k], &UBar[k], &s->fluxF[k], &s->fluxF[k - ny], &s->fluxG[k], &s->fluxG[k - 1],
+                                s->dt, s->dx, s->dy);
+        }
+    m_halo(s, U);
+    m_set_flux(s, s->fluxF, s->fluxG, U, s->nNF5, s->nNG4, s->nF5, s->nG4, s->nNF0, s->nNG0);
+    m_div_b_clean(s, U);
+    m_halo(s, U);
+    s->time += s->dt;
+    s->steps += 1;
+}
+
 /* -------------------------------------------------------------- C API */
 struct oracle_sim *oracle_create(int nx, int ny, double dx, double dy, double gamma, double cfl,
                                  double eps, double eta, int bc_x, int bc_y)
@@ -743,6 +1020,15 @@ struct oracle_sim *oracle_create(int nx, int ny, double dx, double dy, double ga
     s->tmpUForFluxG = calloc(N, sizeof(Cons));
     s->oldFluxF = calloc(N, sizeof(Flux)); s->oldFluxG = calloc(N, sizeof(Flux));
     s->eZVector = calloc(N, sizeof(double));
+    s->oNF5 = calloc(N, sizeof(double)); s->oNG4 = calloc(N, sizeof(double)); s->oF5 = calloc(N, sizeof(double));
+    s->oG4 = calloc(N, sizeof(double)); s->oNF0 = calloc(N, sizeof(double)); s->oNG0 = calloc(N, sizeof(double));
+    s->nNF5 = calloc(N, sizeof(double)); s->nNG4 = calloc(N, sizeof(double)); s->nF5 = calloc(N, sizeof(double));
+    s->nG4 = calloc(N, sizeof(double)); s->nNF0 = calloc(N, sizeof(double)); s->nNG0 = calloc(N, sizeof(double));
+    if (!s->oNF5 || !s->oNG4 || !s->oF5 || !s->oG4 || !s->oNF0 || !s->oNG0 ||
+        !s->nNF5 || !s->nNG4 || !s->nF5 || !s->nG4 || !s->nNF0 || !s->nNG0) {
+        oracle_destroy(s);
+        return NULL;
+    }
     if (!s->U || !s->UBar || !s->fluxF || !s->fluxG || !s->dtVector || !s->bXOld || !s->bYOld ||
         !s->tmpVector || !s->solverFlux || !s->hlldFlux || !s->dQCenter || !s->dQLeft ||
         !s->dQRight || !s->tmpUForFluxG || !s->oldFluxF || !s->oldFluxG || !s->eZVector) {
@@ -759,6 +1045,8 @@ void oracle_destroy(struct oracle_sim *s)
     free(s->bYOld); free(s->tmpVector); free(s->solverFlux); free(s->hlldFlux); free(s->dQCenter);
     free(s->dQLeft); free(s->dQRight); free(s->tmpUForFluxG); free(s->oldFluxF); free(s->oldFluxG);
     free(s->eZVector); free(s->etaField);
+    free(s->oNF5); free(s->oNG4); free(s->oF5); free(s->oG4); free(s->oNF0); free(s->oNG0);
+    free(s->nNF5); free(s->nNG4); free(s->nF5); free(s->nG4); free(s->nNF0); free(s->nNG0);
     free(s);
 }
 
@@ -782,9 +1070,34 @@ void oracle_apply_boundary_axis(struct oracle_sim *s, int axis, int kind)
     else           { if (kind == ORACLE_BC_PERIODIC) periodic_y(s, s->U); else symmetric_y(s, s->U); }
 }
 
-double oracle_calculate_dt(struct oracle_sim *s) { return calculate_dt(s); }
+double oracle_calculate_dt(struct oracle_sim *s)
+{
+    return s->scheme == ORACLE_SCHEME_MULTI_SHARE ? m_calculate_dt(s) : calculate_dt(s);
+}
 
-void oracle_step(struct oracle_sim *s, int nsteps) { for (int n = 0; n < nsteps; ++n) one_step_rk2(s); }
+void oracle_apply_halo(struct oracle_sim *s) { m_halo(s, s->U); }
+
+void oracle_step(struct oracle_sim *s, int nsteps)
+{
+    for (int n = 0; n < nsteps; ++n) {
+        if (s->scheme == ORACLE_SCHEME_MULTI_SHARE) m_one_step_rk2(s);
+        else one_step_rk2(s);
+    }
+}
+
+int oracle_set_scheme(struct oracle_sim *s, int scheme)
+{
+    if (scheme == ORACLE_SCHEME_MULTI_SHARE && (s->etaField || s->bc_x != ORACLE_BC_PERIODIC || s->bc_y != ORACLE_BC_PERIODIC))
+        return -1;
+    s->scheme = scheme;
+    return 0;
+}
+
+/* checkCalculationIsCrashed of M/resistiveMHD2D.cu:567-579 also flags dt < 0 */
+int oracle_check_crashed_multi(const struct oracle_sim *s)
+{
+    return oracle_check_crashed(s) || s->dt < 0;
+}
 
 double oracle_get_dt(const struct oracle_sim *s) { return s->dt; }
 double oracle_get_time(const struct oracle_sim *s) { return s->time; }

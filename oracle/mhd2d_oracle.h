@@ -6,6 +6,7 @@ extern "C" {
 #endif
 
 enum { ORACLE_BC_PERIODIC = 0, ORACLE_BC_SYMMETRIC = 1 };
+enum { ORACLE_SCHEME_SINGLE = 0, ORACLE_SCHEME_MULTI_SHARE = 1 };
 
 struct oracle_sim;
 
@@ -29,6 +30,9 @@ double oracle_minmod(double x, double y, double eps);
 void oracle_hlld_flux(const double *left8, const double *right8, double gamma, double eps, double *flux8);
 double oracle_cell_dt(const double *cell8, double gamma, double dx, double dy, double eps);
 int oracle_num_threads(void);
+int oracle_set_scheme(struct oracle_sim *s, int scheme);
+int oracle_check_crashed_multi(const struct oracle_sim *s);
+void oracle_apply_halo(struct oracle_sim *s);
 
 #ifdef __cplusplus
 }

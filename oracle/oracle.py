@@ -11,6 +11,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 BC_PERIODIC, BC_SYMMETRIC = 0, 1
+SCHEME_SINGLE, SCHEME_MULTI_SHARE = 0, 1
 
 _lib = None
 
@@ -50,6 +51,9 @@ def load():
         "oracle_hlld_flux": (None, [D, D, c_d, c_d, D]),
         "oracle_cell_dt": (c_d, [D, c_d, c_d, c_d, c_d]),
         "oracle_num_threads": (c_i, []),
+        "oracle_set_scheme": (c_i, [P, c_i]),
+        "oracle_check_crashed_multi": (c_i, [P]),
+        "oracle_apply_halo": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -125,7 +129,20 @@ class OracleSim:
         return self._lib.oracle_get_time(self._s)
 
     def crashed(self):
+        if getattr(self, "scheme", SCHEME_SINGLE) == SCHEME_MULTI_SHARE:
+            return bool(self._lib.oracle_check_crashed_multi(self._s))
         return bool(self._lib.oracle_check_crashed(self._s))
+
+    def set_scheme(self, scheme):
+        """SCHEME_SINGLE (single-GPU library) or SCHEME_MULTI_SHARE (the MPI
+        library's step on one rank: periodic halo, uniform eta)."""
+        if self._lib.oracle_set_scheme(self._s, scheme) != 0:
+            raise ValueError("the multi-GPU scheme needs periodic x and y and a uniform eta")
+        self.scheme = scheme
+
+    def apply_halo(self):
+        """The MPI library's sendrecv_U on one rank."""
+        self._lib.oracle_apply_halo(self._s)
 
     def stage1_fluxes(self):
         F = np.empty((self.nx, self.ny, 8))

@@ -270,3 +270,68 @@ def test_symmetric_boundary_mirror_and_divfree_ghosts():
     for i in (1, 2, 3, nx - 3, nx - 2, nx - 1):
         div = (bx[i, 1:] - bx[i - 1, 1:]) / dx + (by[i, 1:] - by[i, :-1]) / dy
         assert np.abs(div).max() <= 1e-12 * (1 + np.abs(by).max() / dy)
+
+
+# ------------------------------------- multi-GPU library step on one rank
+from oracle.oracle import SCHEME_MULTI_SHARE
+
+
+def _ot_multi(n, steps):
+    cfg, o = _ot_state(n)
+    o.set_scheme(SCHEME_MULTI_SHARE)
+    return cfg, o
+
+
+def test_multi_halo_equals_periodic_fill():
+    # sendrecv_U to itself (M/mpi.cu:104-188) is the periodic fill
+    nx, ny = 20, 17
+    rng = np.random.default_rng(8)
+    U = rng.standard_normal((nx, ny, 8))
+    a = OracleSim(nx, ny, 0.1, 0.1)
+    a.set_state(U)
+    a.apply_boundary()
+    b = OracleSim(nx, ny, 0.1, 0.1)
+    b.set_state(U)
+    b.apply_halo()
+    assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_multi_conservation_and_divb():
+    cfg, o = _ot_multi(40, 0)
+    U0 = o.get_state()
+    d0 = _divb(U0, cfg.dx, cfg.dy)
+    o.step(20)
+    U1 = o.get_state()
+    for v in (0, 1, 2, 3, 7):
+        scale = np.abs(U0[3:-3, 3:-3, v]).sum()
+        assert abs(U1[3:-3, 3:-3, v].sum() - U0[3:-3, 3:-3, v].sum()) <= 1e-12 * scale
+    d1 = _divb(U1, cfg.dx, cfg.dy)
+    assert np.abs(d1 - d0).max() <= 1e-11
+    assert not o.crashed()
+
+
+def test_multi_dt_interior_only_no_diffusion_limit():
+    nx, ny, dx, dy = 20, 16, 0.1, 0.05
+    U = _uniform_state(nx, ny)
+    U[0, 0] = _uniform_state(1, 1, rho=0.01)[0, 0]    # a fast ghost cell is ignored
+    o = OracleSim(nx, ny, dx, dy, GAMMA, 0.7, EPS, 10.0)
+    o.set_scheme(SCHEME_MULTI_SHARE)
+    o.set_state(U)
+    assert o.calculate_dt() == pytest.approx(0.7 * orc.cell_dt(U[5, 5], GAMMA, dx, dy), rel=1e-15)
+
+
+def test_multi_uniform_state_stays_uniform():
+    nx, ny = 24, 20
+    U = _uniform_state(nx, ny)
+    o = OracleSim(nx, ny, 0.1, 0.1, GAMMA, 0.7, EPS, 1e-3)
+    o.set_scheme(SCHEME_MULTI_SHARE)
+    o.set_state(U)
+    o.apply_halo()
+    o.step(4)
+    assert np.array_equal(o.get_state()[3:-3, 3:-3], U[3:-3, 3:-3])
+
+
+def test_multi_scheme_rejects_symmetric_and_eta_field():
+    o = OracleSim(20, 20, 0.1, 0.1, bc_y=BC_SYMMETRIC)
+    with pytest.raises(ValueError):
+        o.set_scheme(SCHEME_MULTI_SHARE)
