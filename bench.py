@@ -1,10 +1,13 @@
 #!/usr/bin/env python3
 """Throughput of the plain FP64 resistive-MHD RK2 step on one MI355X.
 
-Workload: Harris current sheet, 4096 x 2048 interior cells on
-[-100, 100] x [-50, 50], periodic x, symmetric y, Bx = tanh y,
-rho = sech^2 y + 0.2, p = rho/2, per-cell resistivity
+Workload (default, --workload harris): Harris current sheet, 4096 x 2048
+interior cells on [-100, 100] x [-50, 50], periodic x, symmetric y,
+Bx = tanh y, rho = sech^2 y + 0.2, p = rho/2, per-cell resistivity
 1e-3 + 5e-3 exp(-(x^2 + y^2)/4), seeded white-noise vy of amplitude 1e-3.
+--workload turbulence: one GPU's subdomain of the MPI library's step,
+8192 x 8192 decaying turbulence (rho = p = 1, eta 1e-4, random-phase modes
+|k| <= 8 with rms v = rms B = 0.5), periodic halo filled on the device.
 The state is resident in HBM before the timed region; K steps run with no
 host synchronisation inside it.
 
@@ -24,12 +27,26 @@ STATE_B = 64   # 8 doubles per cell
 ETA_B = 8      # per-cell resistivity plane
 
 
-def stage_bytes(zones, eta_field):
+AUX_B = 32    # face fluxes F0, F5, G0, G4 the multi-GPU scheme's stages store
+
+
+def stage_bytes(zones, eta_field, multi=False):
     """Algorithmic HBM bytes of one launch of each stage kernel.
     stage 1: read U^n, write U* ; stage 2: read U*, read U^n, write U^{n+1};
-    each stage reads the resistivity plane once."""
+    each stage reads the resistivity plane once.  The multi-GPU scheme's
+    stages also write the four face fluxes of the CT-contact EMF."""
     e = ETA_B if eta_field else 0
-    return zones * (2 * STATE_B + e), zones * (3 * STATE_B + e)
+    a = AUX_B if multi else 0
+    return zones * (2 * STATE_B + e + a), zones * (3 * STATE_B + e + a)
+
+
+def multi_extra_bytes(zones):
+    """Algorithmic HBM bytes per step of the multi-GPU scheme's EMF and CT
+    kernels: two cell-EMF passes (read rho, rho u, rho v, Bx, By; write one
+    plane), the corner EMF (read 2 x 4 flux planes and 2 EMF planes, write
+    one) and the final CT (read U^n's Bx, By, E and the 6 other planes of U2,
+    write Bx, By)."""
+    return zones * (2 * (40 + 8) + (64 + 16 + 8) + (16 + 8 + 48 + 16))
 
 
 def measured_traffic(root, numerics, kernel):
@@ -46,14 +63,23 @@ def measured_traffic(root, numerics, kernel):
     return None
 
 
-def cpu_baseline(nx, ny, steps):
-    from oracle.oracle import OracleSim, num_threads
+def cpu_baseline(nx, ny, steps, workload="harris"):
+    from oracle.oracle import SCHEME_MULTI_SHARE, OracleSim, num_threads
     import resistivemhd_gpu_amd as mhd
-    cfg = mhd.harris_config(nx, ny)
-    o = OracleSim.from_config(cfg)
-    o.set_state(mhd.fill_harris(cfg, seed=0, noise_amp=1e-3))
-    o.set_eta_field(mhd.fill_harris_eta(cfg))
-    o.apply_boundary()
+    if workload == "turbulence":
+        cfg = mhd.default_config(nx, ny, 0.0, 2.0 * 3.141592653589793, 0.0, 2.0 * 3.141592653589793, eta=1e-4)
+        o = OracleSim.from_config(cfg)
+        o.set_scheme(SCHEME_MULTI_SHARE)
+        o.set_state(mhd.fill_turbulence(cfg, seed=0))
+        o.apply_halo()
+        what = "MPI library's step on one periodic subdomain, turbulence"
+    else:
+        cfg = mhd.harris_config(nx, ny)
+        o = OracleSim.from_config(cfg)
+        o.set_state(mhd.fill_harris(cfg, seed=0, noise_amp=1e-3))
+        o.set_eta_field(mhd.fill_harris_eta(cfg))
+        o.apply_boundary()
+        what = "Harris sheet"
     o.step(1)  # first touch of the intermediates
     t0 = time.perf_counter()
     o.step(steps)
@@ -63,17 +89,18 @@ def cpu_baseline(nx, ny, steps):
         "unit": "zone-cycles/s",
         "cores": num_threads(),
         "kind": "port",
-        "sample": f"CPU oracle (C restatement, OpenMP), Harris sheet {nx}x{ny} interior, {steps} RK2 steps, {t:.1f} s",
+        "sample": f"CPU oracle (C restatement, OpenMP), {what} {nx}x{ny} interior, {steps} RK2 steps, {t:.1f} s",
     }
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["harris", "turbulence"], default="harris")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=4096)
-    ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--nx", type=int, default=0, help="interior cells along x (0: 4096 harris, 8192 turbulence)")
+    ap.add_argument("--ny", type=int, default=0, help="interior cells along y (0: 2048 harris, 8192 turbulence)")
     ap.add_argument("--sweep", type=int, default=0, help="columns per wavefront sweep (0 = library default)")
     ap.add_argument("--numerics", choices=["exact", "fast"], default="exact",
                     help="exact: bitwise reference arithmetic (library default); fast: rounding-level differences")
@@ -96,12 +123,27 @@ def main():
     import torch
     import resistivemhd_gpu_amd as mhd
 
-    cfg = mhd.harris_config(args.nx, args.ny)
+    multi = args.workload == "turbulence"
+    if multi:
+        args.nx = args.nx or 8192
+        args.ny = args.ny or 8192
+        cfg = mhd.default_config(args.nx, args.ny, 0.0, 2.0 * 3.141592653589793, 0.0, 2.0 * 3.141592653589793,
+                                 eta=1e-4)
+    else:
+        args.nx = args.nx or 4096
+        args.ny = args.ny or 2048
+        cfg = mhd.harris_config(args.nx, args.ny)
     sim = mhd.ResistiveMHD2D(cfg)
     sim.set_numerics(args.numerics)
     if args.sweep:
         sim.set_sweep_length(args.sweep)
-    sim.initHarris(seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3)
+    if multi:
+        sim.set_scheme("multi")
+        sim.initTurbulence(seed=0, kmax=8, rms_v=0.5, rms_b=0.5)
+        workload = f"turbulence_{args.nx}x{args.ny}_one_subdomain_of_multi_gpu_step_periodic_halo"
+    else:
+        sim.initHarris(seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3)
+        workload = f"harris_sheet_{args.nx}x{args.ny}_periodic_x_symmetric_y_eta_field"
     zones = args.nx * args.ny
 
     sim.steps(args.warmup)
@@ -119,16 +161,17 @@ def main():
 
     seconds = t1 - t0
     value = zones * args.steps / seconds
-    b1, b2 = stage_bytes(zones, True)
+    b1, b2 = stage_bytes(zones, not multi, multi)
     avg1, avg2 = ms1 / nt / 1e3, ms2 / nt / 1e3
     # dominant kernel = the stage with the larger summed time
     exact = "true" if args.numerics == "exact" else "false"
+    scheme = 1 if multi else 0
     if avg2 >= avg1:
-        kname, kbytes, kavg = f"rk_stage_kernel<2, {exact}>", b2, avg2
+        kname, kbytes, kavg = f"rk_stage_kernel<2, {exact}, {scheme}>", b2, avg2
     else:
-        kname, kbytes, kavg = f"rk_stage_kernel<1, {exact}>", b1, avg1
+        kname, kbytes, kavg = f"rk_stage_kernel<1, {exact}, {scheme}>", b1, avg1
     traffic = None
-    if (args.nx, args.ny) == (4096, 2048):
+    if not multi and (args.nx, args.ny) == (4096, 2048):
         traffic = measured_traffic(root, args.numerics, kname)
     achieved = kbytes / kavg / 1e9
     line = {
@@ -145,7 +188,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"harris_sheet_{args.nx}x{args.ny}_periodic_x_symmetric_y_eta_field",
+            "workload": workload,
             "nx": args.nx, "ny": args.ny, "ghost": 3, "sweep_columns": sim.sweep_length(),
             "numerics": args.numerics,
             "parallelism": "single-gpu",
@@ -162,7 +205,8 @@ def main():
             "avg_launch_ms": kavg * 1e3,
             "stage_ms": [ms1 / nt, ms2 / nt],
         },
-        "step_hbm_frac": value * (b1 + b2) / zones / (HBM_PEAK_GBS * 1e9),
+        "step_hbm_frac": value * (b1 + b2 + (multi_extra_bytes(zones) if multi else 0)) / zones
+        / (HBM_PEAK_GBS * 1e9),
         "crashed": crashed,
     }
     del sim
@@ -176,7 +220,10 @@ def main():
         line["calibration_copy_bytes"] = n * 8
         del src, dst
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_ny, args.cpu_steps)
+        if multi:
+            line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_nx, args.cpu_steps, "turbulence")
+        else:
+            line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_ny, args.cpu_steps)
     print(json.dumps(line), flush=True)
 
 

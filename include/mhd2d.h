@@ -136,10 +136,45 @@ int mhd_fill_orszag_tang(const mhd_config *cfg, double *host_aos);
 int mhd_fill_harris(const mhd_config *cfg, unsigned long long seed, double noise_amp, double *host_aos);
 int mhd_fill_harris_eta(const mhd_config *cfg, double eta0, double eta1, double *host_eta);
 
+/* Decaying-turbulence state for the MPI library's subdomain case (no
+ * reference example sets it up; it stands in for the user-written
+ * initializeU of ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU/
+ * resistiveMHD2D.hpp): rho = 1, p = 1, w = Bz = 0; (vx, vy) and the corner
+ * vector potential Az are sums of cosines over the wave vectors
+ * n = (nx, ny) != 0 with |n| <= kmax, k = 2*pi*n / L (L = interior width),
+ * amplitudes 1/|n| (v) and 1/|n|^2 (Az), phases uniform from a counter hash
+ * of (seed, mode, field).  B is the discrete curl of Az
+ * (Bx = dAz/dy, By = -dAz/dx across each face), so the face divergence is
+ * zero to round-off.  The amplitudes are scaled so that the interior rms of
+ * |v| and of |B| (face values) are rms_v and rms_b, from the discrete
+ * Parseval sums (exact for kmax < N/2). */
+int mhd_fill_turbulence(const mhd_config *cfg, unsigned long long seed, int kmax, double rms_v, double rms_b,
+                        double *host_aos);
+
 /* Device-side helpers built on the setups: fill, upload, apply the
  * configured boundary (as initializeU does). */
 int mhd_init_orszag_tang(mhd_handle *h);
 int mhd_init_harris(mhd_handle *h, unsigned long long seed, double noise_amp, double eta0, double eta1);
+int mhd_init_turbulence(mhd_handle *h, unsigned long long seed, int kmax, double rms_v, double rms_b);
+
+/* Step scheme.
+ *  - MHD_SCHEME_SINGLE (default): the single-GPU library's oneStepRK2
+ *    (P/resistiveMHD_2D.cu:133-195, boundaries per cfg.bc_x / bc_y).
+ *  - MHD_SCHEME_MULTI_SHARE: the MPI library's ResistiveMHD2D::oneStepRK2
+ *    (ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU/resistiveMHD2D.cu:147-221)
+ *    on one GPU's subdomain that is its own neighbour on all four sides, so
+ *    sendrecv_U (ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU/mpi.cu:104-188)
+ *    becomes an on-device periodic halo fill.  Differences from SINGLE: the
+ *    face-centred resistive flux (flux_solver.cu:73-189 there), B advanced by
+ *    the flux divergence from the face-smoothed field in both stages, the
+ *    CT-contact corner EMF averaged over the two stages applied once to the
+ *    step's initial face B (ct.cu:26-218 there), and dt over interior cells
+ *    without the diffusion limit (resistiveMHD2D.cu:489-556 there); a step
+ *    with dt < 0 counts as crashed.  Needs periodic x and y and a uniform
+ *    eta (no eta field); MHD_ERR_STATE otherwise. */
+enum mhd_scheme { MHD_SCHEME_SINGLE = 0, MHD_SCHEME_MULTI_SHARE = 1 };
+int mhd_set_scheme(mhd_handle *h, int scheme);
+int mhd_get_scheme(const mhd_handle *h, int *scheme);
 
 /* Arithmetic of the step (no counterpart in the reference, whose single
  * code path is what MHD_NUMERICS_EXACT reproduces):

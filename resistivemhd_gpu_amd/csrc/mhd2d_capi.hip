@@ -8,6 +8,16 @@
 // dt never leaves the device: stage 2 and the boundary kernels reduce the
 // per-cell CFL limit of the new state into a control block, and the next
 // stage 1 turns it into dt.
+//
+// MHD_SCHEME_MULTI_SHARE runs the MPI library's step
+// (ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU/resistiveMHD2D.cu:147-221)
+// on one subdomain that is its own neighbour on every side:
+//   stage 1 (U -> U*, face fluxes F0 F5 G0 G4 -> aux[0:4]) ; Ec of smoothed U
+//   halo of U* ; stage 2 (U*, U -> U2, fluxes -> aux[4:8]) ; halo of U2 ;
+//   Ec of U2 ; corner EMF ; final CT of U2 from U's face B (+ next dt) ;
+//   halo of U2 ; U2 becomes U.
+// The self-exchange halo (M/mpi.cu:104-188 with every neighbour = self) is
+// the periodic fill in x then y.
 #include <hip/hip_runtime.h>
 #include <cerrno>
 #include <cmath>
@@ -15,6 +25,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/mhd2d.h"
@@ -30,6 +41,9 @@ struct mhd_handle {
     Phys ph{};
     double *U = nullptr;      // state U^n (8 planes)
     double *Us = nullptr;     // stage-1 state U*
+    double *U2 = nullptr;     // MULTI_SHARE: stage-2 state (8 planes)
+    double *aux = nullptr;    // MULTI_SHARE: face fluxes F0 F5 G0 G4 of stage 1, then of stage 2
+    double *ecE = nullptr;    // MULTI_SHARE: cell EMF of U^n, of U2, corner EMF E
     double *eta = nullptr;    // optional per-cell resistivity plane
     double *side = nullptr;   // CT value of the bottom boundary-face By per column
     double *aos = nullptr;    // AoS staging for host transfers (allocated on first use)
@@ -41,6 +55,7 @@ struct mhd_handle {
     int cur = 0;              // control slot holding the per-cell dt minimum of U
     bool dt_valid = false;    // ctrl->acc[cur] matches the current U
     bool exact = true;        // numerics: exact reference arithmetic (default) or fast
+    int scheme = MHD_SCHEME_SINGLE;
     bool profiling = false;
     std::vector<hipEvent_t> ev;   // 4 events per timed step: stage-1 start/end, stage-2 start/end
     std::string err;
@@ -152,10 +167,12 @@ int apply_bc_pair(mhd_handle *h, double *u, Ctrl *ctrl, int slot)
     return MHD_OK;
 }
 
+bool multi(const mhd_handle *h) { return h->scheme == MHD_SCHEME_MULTI_SHARE; }
+
 int refresh_dt(mhd_handle *h)
 {
     if (h->dt_valid) return MHD_OK;
-    HIPCHECK(h, mhd::launch_dt_all(h->exact, h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
+    HIPCHECK(h, mhd::launch_dt_all(h->exact, multi(h), h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
     h->dt_valid = true;
     return MHD_OK;
 }
@@ -225,6 +242,9 @@ int mhd_destroy(mhd_handle *h)
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     (void)hipFree(h->U);
     (void)hipFree(h->Us);
+    (void)hipFree(h->U2);
+    (void)hipFree(h->aux);
+    (void)hipFree(h->ecE);
     (void)hipFree(h->eta);
     (void)hipFree(h->side);
     (void)hipFree(h->aos);
@@ -269,6 +289,8 @@ int mhd_get_state(mhd_handle *h, double *host_aos)
 int mhd_set_eta_field(mhd_handle *h, const double *host_eta)
 {
     if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if (host_eta && multi(h))
+        return fail(h, MHD_ERR_STATE, "MHD_SCHEME_MULTI_SHARE takes a uniform eta (the MPI library's getEta)");
     if (!host_eta) {
         HIPCHECK(h, hipStreamSynchronize(h->stream));
         HIPCHECK(h, hipFree(h->eta));
@@ -312,6 +334,8 @@ int mhd_set_boundary(mhd_handle *h, int bc_x, int bc_y)
     if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
     if ((bc_x != MHD_BC_PERIODIC && bc_x != MHD_BC_SYMMETRIC) || (bc_y != MHD_BC_PERIODIC && bc_y != MHD_BC_SYMMETRIC))
         return fail(h, MHD_ERR_INVALID, "bc must be MHD_BC_PERIODIC or MHD_BC_SYMMETRIC");
+    if (multi(h) && (bc_x != MHD_BC_PERIODIC || bc_y != MHD_BC_PERIODIC))
+        return fail(h, MHD_ERR_STATE, "MHD_SCHEME_MULTI_SHARE fills its halo periodically");
     h->cfg.bc_x = bc_x;
     h->cfg.bc_y = bc_y;
     return MHD_OK;
@@ -320,12 +344,16 @@ int mhd_set_boundary(mhd_handle *h, int bc_x, int bc_y)
 int mhd_calculate_dt(mhd_handle *h, double *dt)
 {
     if (!h || !dt) return fail(h, MHD_ERR_INVALID, "NULL argument");
-    HIPCHECK(h, mhd::launch_dt_all(h->exact, h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
+    HIPCHECK(h, mhd::launch_dt_all(h->exact, multi(h), h->U, h->ctrl, h->cur, h->g, h->ph, h->stream));
     h->dt_valid = true;
     Ctrl c;
     int rc = read_ctrl(h, &c);
     if (rc) return rc;
-    const double m = __builtin_bit_cast(double, c.acc[h->cur]);
+    const double m = c.first_nan[h->cur] ? std::nan("") : __builtin_bit_cast(double, c.acc[h->cur]);
+    if (multi(h)) {
+        *dt = m * h->cfg.cfl;   // M/resistiveMHD2D.cu:541: interior cells, no diffusion limit
+        return MHD_OK;
+    }
     const double lim = std::fmin(h->cfg.dx * h->cfg.dx / h->cfg.eta, h->cfg.dy * h->cfg.dy / h->cfg.eta);
     *dt = std::fmin(m, lim) * h->cfg.cfl;
     return MHD_OK;
@@ -337,6 +365,9 @@ int mhd_step_rk2(mhd_handle *h, int nsteps)
     if (nsteps < 0) return fail(h, MHD_ERR_INVALID, "nsteps must be >= 0");
     int rc = refresh_dt(h);
     if (rc) return rc;
+    const bool m = multi(h);
+    const mhd::Grid &g = h->g;
+    const size_t plane = (size_t)g.plane;
     for (int n = 0; n < nsteps; ++n) {
         hipEvent_t ev[4] = {};
         if (h->profiling) {
@@ -345,16 +376,71 @@ int mhd_step_rk2(mhd_handle *h, int nsteps)
                 h->ev.push_back(ev[q]);
             }
         }
+        double *aux1 = m ? h->aux : nullptr, *aux2 = m ? h->aux + 4 * plane : nullptr;
+        double *out2 = m ? h->U2 : h->U;
         if (ev[0]) HIPCHECK(h, hipEventRecord(ev[0], h->stream));
-        HIPCHECK(h, mhd::launch_stage(1, h->exact, h->U, h->Us, h->U, h->eta, h->side, h->ctrl, h->g, h->ph, h->cur, h->stream));
+        HIPCHECK(h, mhd::launch_stage(1, h->exact, h->scheme, h->U, h->Us, h->U, h->eta, h->side, aux1, h->ctrl, g,
+                                      h->ph, h->cur, h->stream));
         if (ev[1]) HIPCHECK(h, hipEventRecord(ev[1], h->stream));
+        if (m) HIPCHECK(h, mhd::launch_m_ec(true, h->U, h->ecE, g, h->ph, h->stream));
         if ((rc = apply_bc_pair(h, h->Us, nullptr, 0))) return rc;
         if (ev[2]) HIPCHECK(h, hipEventRecord(ev[2], h->stream));
-        HIPCHECK(h, mhd::launch_stage(2, h->exact, h->Us, h->U, h->U, h->eta, h->side, h->ctrl, h->g, h->ph, h->cur, h->stream));
+        HIPCHECK(h, mhd::launch_stage(2, h->exact, h->scheme, h->Us, out2, h->U, h->eta, h->side, aux2, h->ctrl, g,
+                                      h->ph, h->cur, h->stream));
         if (ev[3]) HIPCHECK(h, hipEventRecord(ev[3], h->stream));
-        if ((rc = apply_bc_pair(h, h->U, h->ctrl, h->cur ^ 1))) return rc;
+        if (m) {
+            if ((rc = apply_bc_pair(h, h->U2, nullptr, 0))) return rc;
+            HIPCHECK(h, mhd::launch_m_ec(false, h->U2, h->ecE + plane, g, h->ph, h->stream));
+            HIPCHECK(h, mhd::launch_m_emf(aux1, aux2, h->ecE, h->ecE + plane, h->ecE + 2 * plane, g, h->stream));
+            HIPCHECK(h, mhd::launch_m_ct(h->exact, h->U, h->ecE + 2 * plane, h->U2, h->ctrl, h->cur ^ 1, g, h->ph,
+                                         h->stream));
+            if ((rc = apply_bc_pair(h, h->U2, nullptr, 0))) return rc;
+            std::swap(h->U, h->U2);
+        } else {
+            if ((rc = apply_bc_pair(h, h->U, h->ctrl, h->cur ^ 1))) return rc;
+        }
         h->cur ^= 1;
     }
+    return MHD_OK;
+}
+
+int mhd_set_scheme(mhd_handle *h, int scheme)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if (scheme != MHD_SCHEME_SINGLE && scheme != MHD_SCHEME_MULTI_SHARE)
+        return fail(h, MHD_ERR_INVALID, "scheme must be MHD_SCHEME_SINGLE or MHD_SCHEME_MULTI_SHARE");
+    if (scheme == MHD_SCHEME_MULTI_SHARE) {
+        if (h->cfg.bc_x != MHD_BC_PERIODIC || h->cfg.bc_y != MHD_BC_PERIODIC)
+            return fail(h, MHD_ERR_STATE, "MHD_SCHEME_MULTI_SHARE needs periodic x and y (its halo is the self-exchange)");
+        if (h->eta) return fail(h, MHD_ERR_STATE, "MHD_SCHEME_MULTI_SHARE takes a uniform eta; clear the eta field");
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        const size_t plane = sizeof(double) * (size_t)h->g.plane;
+        if (!h->U2) {
+            hipError_t e;
+            if ((e = hipMalloc(&h->U2, 8 * plane)) != hipSuccess || (e = hipMalloc(&h->aux, 8 * plane)) != hipSuccess ||
+                (e = hipMalloc(&h->ecE, 3 * plane)) != hipSuccess) {
+                (void)hipFree(h->U2);
+                (void)hipFree(h->aux);
+                (void)hipFree(h->ecE);
+                h->U2 = h->aux = h->ecE = nullptr;
+                return fail(h, e == hipErrorOutOfMemory ? MHD_ERR_NOMEM : MHD_ERR_HIP,
+                            std::string("mhd_set_scheme: ") + hipGetErrorString(e));
+            }
+            HIPCHECK(h, hipMemsetAsync(h->U2, 0, 8 * plane, h->stream));
+            HIPCHECK(h, hipMemsetAsync(h->aux, 0, 8 * plane, h->stream));
+            HIPCHECK(h, hipMemsetAsync(h->ecE, 0, 3 * plane, h->stream));
+            HIPCHECK(h, hipStreamSynchronize(h->stream));
+        }
+    }
+    h->scheme = scheme;
+    h->dt_valid = false;   // the two schemes reduce dt over different cells
+    return MHD_OK;
+}
+
+int mhd_get_scheme(const mhd_handle *h, int *scheme)
+{
+    if (!h || !scheme) return fail(nullptr, MHD_ERR_INVALID, "NULL argument");
+    *scheme = h->scheme;
     return MHD_OK;
 }
 
@@ -429,6 +515,7 @@ int mhd_check_crashed(mhd_handle *h, int *crashed)
     rc = read_ctrl(h, &c);
     if (rc) return rc;
     *crashed = c.crashed ? 1 : 0;
+    if (multi(h) && c.dt < 0.0) *crashed = 1;   // M/resistiveMHD2D.cu:576
     return MHD_OK;
 }
 
@@ -499,6 +586,22 @@ int mhd_init_harris(mhd_handle *h, unsigned long long seed, double noise_amp, do
     std::vector<double> eta((size_t)h->cfg.nx * h->cfg.ny);
     if ((rc = mhd_fill_harris_eta(&h->cfg, eta0, eta1, eta.data()))) return fail(h, rc, "mhd_fill_harris_eta failed");
     if ((rc = mhd_set_eta_field(h, eta.data()))) return rc;
+    if ((rc = mhd_apply_boundary(h, 0, h->cfg.bc_x))) return rc;
+    return mhd_apply_boundary(h, 1, h->cfg.bc_y);
+}
+
+int mhd_init_turbulence(mhd_handle *h, unsigned long long seed, int kmax, double rms_v, double rms_b)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    std::vector<double> buf;
+    try {
+        buf.resize(8 * (size_t)h->cfg.nx * h->cfg.ny);
+    } catch (...) {
+        return fail(h, MHD_ERR_NOMEM, "host allocation failed");
+    }
+    int rc = mhd_fill_turbulence(&h->cfg, seed, kmax, rms_v, rms_b, buf.data());
+    if (rc) return fail(h, rc, "mhd_fill_turbulence: invalid arguments (kmax must be >= 1 and < half the interior size)");
+    if ((rc = mhd_set_state(h, buf.data()))) return rc;
     if ((rc = mhd_apply_boundary(h, 0, h->cfg.bc_x))) return rc;
     return mhd_apply_boundary(h, 1, h->cfg.bc_y);
 }

@@ -25,6 +25,9 @@ struct Ctrl {
     double time;                 // accumulated time
     long long steps;             // completed steps
     int crashed;                 // NaN found in e after the last step
+    // per slot: the dt of the cell a sequential min_element visits first is
+    // NaN, which then sticks (a NaN anywhere else is passed over)
+    int first_nan[2];
     int pad;
 };
 

@@ -21,15 +21,25 @@ struct Grid {
 
 // exact = true: operation-for-operation reference arithmetic (built without
 // contraction it rounds as the reference does); false: fast numerics.
-hipError_t launch_stage(int stage, bool exact, const double *uin, double *uout, const double *uold,
-                        const double *eta, double *side, Ctrl *ctrl, const Grid &g, const Phys &ph, int cur,
-                        hipStream_t s);
+// scheme 0: the single-GPU library's stage; 1: the MPI library's stage on
+// one subdomain, which also stores the face fluxes F0, F5, G0, G4 in aux.
+hipError_t launch_stage(int stage, bool exact, int scheme, const double *uin, double *uout, const double *uold,
+                        const double *eta, double *side, double *aux, Ctrl *ctrl, const Grid &g, const Phys &ph,
+                        int cur, hipStream_t s);
 hipError_t launch_bc_x(int kind, bool exact, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
                        hipStream_t s);
 hipError_t launch_bc_y(int kind, bool exact, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
                        hipStream_t s);
-hipError_t launch_dt_all(bool exact, const double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
-                         hipStream_t s);
+// interior: dt over interior cells only (the MPI library's calculateDt)
+hipError_t launch_dt_all(bool exact, bool interior, const double *u, Ctrl *ctrl, int slot, const Grid &g,
+                         const Phys &ph, hipStream_t s);
+// MPI library's CT-contact EMF: cell-centred EMF of a state (smoothed or as
+// stored), corner EMF from two flux sets, final CT with the next dt.
+hipError_t launch_m_ec(bool smooth, const double *u, double *ec, const Grid &g, const Phys &ph, hipStream_t s);
+hipError_t launch_m_emf(const double *aux_old, const double *aux_now, const double *ec_old, const double *ec_now,
+                        double *E, const Grid &g, hipStream_t s);
+hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *u, Ctrl *ctrl, int slot,
+                       const Grid &g, const Phys &ph, hipStream_t s);
 // Resident 64-lane stage workgroups per CU (the lower of the two stages).
 int stage_waves_per_cu(bool exact);
 hipError_t launch_unpack(const double *aos, double *u, const Grid &g, hipStream_t s);

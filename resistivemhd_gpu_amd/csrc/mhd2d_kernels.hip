@@ -76,14 +76,23 @@ __device__ __forceinline__ Cell make_cell(double rho, double mx, double my, doub
 #define MHD_WAVES_PER_EU_FAST 2
 #endif
 
-template <int STAGE, bool EXACT>
+// SCHEME 0: the single-GPU library's step (P/resistiveMHD_2D.cu:133-195).
+// SCHEME 1: the MPI library's step on one subdomain
+// (ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU/resistiveMHD2D.cu:147-221,
+// "M/" below): face-centred resistive flux (M/flux_solver.cu:73-189), B
+// advanced by flux divergence from the face-smoothed field in both stages,
+// no CT in the stages; the face fluxes the CT-contact EMF needs are stored
+// in `aux` (planes F0, F5 at x-faces and G0, G4 at y-faces, owned by the
+// cell on their low side) and the stage-2 state goes to uout != uold.
+template <int STAGE, bool EXACT, int SCHEME>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
 void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                                                       const double *uold, const double *__restrict__ etaf,
-                                                      double *__restrict__ side, Ctrl *__restrict__ ctrl,
-                                                      Grid g, Phys ph, int cur)
+                                                      double *__restrict__ side, double *__restrict__ aux,
+                                                      Ctrl *__restrict__ ctrl, Grid g, Phys ph, int cur)
 {
+    constexpr bool MULTI = SCHEME == 1;
     const int lane = threadIdx.x;
     // Workgroups are dealt to the XCDs round-robin (block b runs on XCD
     // b % xcds).  Each XCD gets a contiguous range of items (strips fastest),
@@ -106,12 +115,17 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
     // the per-cell minimum reduced at the end of the previous step.
     double dt;
     if (STAGE == 1) {
-        const double m = __longlong_as_double((long long)ctrl->acc[cur]);
-        const double lim = fmin(ph.dx * ph.dx / ph.eta, ph.dy * ph.dy / ph.eta);
-        dt = fmin(m, lim) * ph.cfl;
+        const double m = ctrl->first_nan[cur] ? __builtin_nan("") : __longlong_as_double((long long)ctrl->acc[cur]);
+        if (MULTI) {
+            dt = m * ph.cfl;   // M/resistiveMHD2D.cu:541: no diffusion limit
+        } else {
+            const double lim = fmin(ph.dx * ph.dx / ph.eta, ph.dy * ph.dy / ph.eta);
+            dt = fmin(m, lim) * ph.cfl;
+        }
         if (item == 0 && lane == 0) {
             ctrl->dt = dt;
             ctrl->acc[cur ^ 1] = kInfBits;
+            ctrl->first_nan[cur ^ 1] = 0;
             ctrl->crashed = 0;
         }
     } else {
@@ -136,6 +150,7 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
     Flux8 Fprev{}, Gprev{};                          // F(c-3+1/2), G(c-2)
     double Eprev = 0.0;                              // E at corner (c-3+1/2, j+1/2)
     double rJY = 0.0, rJZ = 0.0, rJYBZ = 0.0, rJZBY = 0.0;   // resistive terms of cell c-2
+    double rTA = 0.0, rTB = 0.0;                     // M/: centred-difference terms of cell c-2
     double dtloc = __longlong_as_double((long long)kInfBits);
     int nanflag = 0;
 
@@ -180,6 +195,18 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
             const double eJX = eta * jX, eJY = eta * jY, eJZ = eta * jZ;
             const double eJYBZ = eJY * B.bz, eJZBY = eJZ * B.byc;
             const double eJZBX = eJZ * B.bxc, eJXBZ = eJX * B.bz;
+            // M/flux_solver.cu:88-92,173-177: the face current averages the two
+            // cells' centred differences, ((a_i - b_i) + a_i+1) - b_i+1
+            double tA = 0.0, tB = 0.0;
+            if (MULTI) {
+                if (EXACT) {
+                    tA = (C.byc - A.byc) / two_dx;
+                    tB = (bxc_up - bxc_dn) / two_dy;
+                } else {
+                    tA = (C.byc - A.byc) * inv_2dx;
+                    tB = (bxc_up - bxc_dn) * inv_2dy;
+                }
+            }
 
             Flux8 F{};
             if (c >= i0 + 1) {
@@ -192,9 +219,21 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                 Rx.bz = B.bz - 0.5 * sbz;
                 Rx.p = B.px - 0.5 * sp;
                 F = hlld<EXACT>(Lx, Rx, 0.5 * (A.bxc + B.bxc), ph, inv_gm1);
-                F.f5 -= 0.5 * (rJZ + eJZ);
-                F.f6 += 0.5 * (rJY + eJY);
-                F.f7 += 0.5 * (rJYBZ + eJYBZ) - 0.5 * (rJZBY + eJZBY);
+                if (MULTI) {
+                    // face (c-2+1/2): cells A = c-2 and B = c-1 (M/flux_solver.cu:73-104)
+                    const double fjY = EXACT ? -(B.bz - A.bz) / ph.dx : -(B.bz - A.bz) * (2.0 * inv_2dx);
+                    const double fjZ = 0.5 * (((rTA - rTB) + tA) - tB);
+                    const double fJY = ph.eta * fjY, fJZ = ph.eta * fjZ;
+                    const double fJYBZ = fJY * 0.5 * (A.bz + B.bz);
+                    const double fJZBY = fJZ * 0.5 * (A.byc + B.byc);
+                    F.f5 -= fJZ;
+                    F.f6 += fJY;
+                    F.f7 += fJYBZ - fJZBY;
+                } else {
+                    F.f5 -= 0.5 * (rJZ + eJZ);
+                    F.f6 += 0.5 * (rJY + eJY);
+                    F.f7 += 0.5 * (rJYBZ + eJYBZ) - 0.5 * (rJZBY + eJZBY);
+                }
             }
             Lx.rho = B.rho + 0.5 * sr;
             Lx.u = B.u + 0.5 * su;
@@ -204,6 +243,7 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
             Lx.bz = B.bz + 0.5 * sbz;
             Lx.p = B.px + 0.5 * sp;
             rJY = eJY; rJZ = eJZ; rJYBZ = eJYBZ; rJZBY = eJZBY;
+            rTA = tA; rTB = tB;
 
             // y-face flux of column c-1 at faces j+1/2 (HLLD::calculateFluxG,
             // P/hlld.cu:209-233): rotated frame (u,v,w)->(v,w,u),
@@ -236,17 +276,30 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                 // rotate back (P/hlld.cu:170-194)
                 G.f0 = Gr.f0; G.f1 = Gr.f3; G.f2 = Gr.f1; G.f3 = Gr.f2;
                 G.f4 = Gr.f6; G.f5 = Gr.f4; G.f6 = Gr.f5; G.f7 = Gr.f7;
-                const double eJZ1 = __shfl_down(eJZ, 1, 64), eJX1 = __shfl_down(eJX, 1, 64);
-                const double eJZBX1 = __shfl_down(eJZBX, 1, 64), eJXBZ1 = __shfl_down(eJXBZ, 1, 64);
-                G.f4 += 0.5 * (eJZ + eJZ1);
-                G.f6 -= 0.5 * (eJX + eJX1);
-                G.f7 += 0.5 * (eJZBX + eJZBX1) - 0.5 * (eJXBZ + eJXBZ1);
+                if (MULTI) {
+                    // face (c-1, j+1/2): cells j and j+1 (M/flux_solver.cu:158-189)
+                    const double tA1 = __shfl_down(tA, 1, 64), tB1 = __shfl_down(tB, 1, 64);
+                    const double gjX = EXACT ? (bz_up - B.bz) / ph.dy : (bz_up - B.bz) * (2.0 * inv_2dy);
+                    const double gjZ = 0.5 * (((tA - tB) + tA1) - tB1);
+                    const double gJX = ph.eta * gjX, gJZ = ph.eta * gjZ;
+                    const double gJXBZ = gJX * 0.5 * (B.bz + bz_up);
+                    const double gJZBX = gJZ * 0.5 * (B.bxc + bxc_up);
+                    G.f4 += gJZ;
+                    G.f6 -= gJX;
+                    G.f7 += gJZBX - gJXBZ;
+                } else {
+                    const double eJZ1 = __shfl_down(eJZ, 1, 64), eJX1 = __shfl_down(eJX, 1, 64);
+                    const double eJZBX1 = __shfl_down(eJZBX, 1, 64), eJXBZ1 = __shfl_down(eJXBZ, 1, 64);
+                    G.f4 += 0.5 * (eJZ + eJZ1);
+                    G.f6 -= 0.5 * (eJX + eJX1);
+                    G.f7 += 0.5 * (eJZBX + eJZBX1) - 0.5 * (eJXBZ + eJXBZ1);
+                }
             }
 
             // corner EMF at (c-2+1/2, j+1/2), P/ct.cu:22-41 (stage 1 only: the
             // stage-2 CT reuses stage-1 fluxes, P/resistiveMHD_2D.cu:167-168,190)
             double E = 0.0;
-            if (STAGE == 1 && c >= i0 + 1) {
+            if (!MULTI && STAGE == 1 && c >= i0 + 1) {
                 const double f5up = __shfl_down(F.f5, 1, 64);
                 E = 0.25 * (Gprev.f4 + G.f4 + (-1.0 * F.f5) + (-1.0 * f5up));
             }
@@ -257,7 +310,59 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
                 const double g2m = __shfl_up(Gprev.f2, 1, 64), g3m = __shfl_up(Gprev.f3, 1, 64);
                 const double g6m = __shfl_up(Gprev.f6, 1, 64), g7m = __shfl_up(Gprev.f7, 1, 64);
                 const double Em = __shfl_up(E, 1, 64);
-                if (s < iend && (upd_row || (STAGE == 1 && stale_row) || (STAGE == 2 && stale_row))) {
+                if (MULTI) {
+                    // face-smoothed B of the stage input (shift + back,
+                    // M/resistiveMHD2D.cu:597-690): cells s = A and s+1 = B
+                    const double g4m = __shfl_up(Gprev.f4, 1, 64), g5m = __shfl_up(Gprev.f5, 1, 64);
+                    const double byc_up = __shfl_down(A.byc, 1, 64);
+                    const double sbx = 0.5 * (A.bxc + B.bxc);
+                    const double sby = 0.5 * (A.byc + byc_up);
+                    double obx = 0.0, oby = 0.0;
+                    if (STAGE == 2) {
+                        // face-smoothed B of U^n at cell s
+                        const size_t on = (size_t)s * g.pitch + jl;
+                        const double bxs = PL(uold, 4)[on], bxm = PL(uold, 4)[on - g.pitch];
+                        const double bxp = PL(uold, 4)[on + g.pitch];
+                        const double bys = PL(uold, 5)[on], bym1 = PL(uold, 5)[on - 1];
+                        const double byp = __shfl_down(bys, 1, 64);
+                        const double cxs = 0.5 * (bxs + bxm), cxp = 0.5 * (bxp + bxs);
+                        const double cys = 0.5 * (bys + bym1), cyp = 0.5 * (byp + bys);
+                        obx = 0.5 * (cxs + cxp);
+                        oby = 0.5 * (cys + cyp);
+                    }
+                    if (s < iend && upd_row) {
+                        const size_t os = (size_t)s * g.pitch + jl;
+                        double n0, n1, n2, n3, n4, n5, n6, n7;
+                        if (STAGE == 1) {
+                            // oneStepFirst_kernel, M/resistiveMHD2D.cu:64-103
+                            n0 = Rh[os] - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m);
+                            n1 = Mx[os] - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m);
+                            n2 = My[os] - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m);
+                            n3 = Mz[os] - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m);
+                            n4 = sbx - dtdx * (F.f4 - Fprev.f4) - dtdy * (Gprev.f4 - g4m);
+                            n5 = sby - dtdx * (F.f5 - Fprev.f5) - dtdy * (Gprev.f5 - g5m);
+                            n6 = Bz[os] - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m);
+                            n7 = En[os] - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m);
+                        } else {
+                            // oneStepSecond_kernel, M/resistiveMHD2D.cu:106-144
+                            n0 = 0.5 * (PL(uold, 0)[os] + Rh[os] - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m));
+                            n1 = 0.5 * (PL(uold, 1)[os] + Mx[os] - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m));
+                            n2 = 0.5 * (PL(uold, 2)[os] + My[os] - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m));
+                            n3 = 0.5 * (PL(uold, 3)[os] + Mz[os] - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m));
+                            n4 = 0.5 * (obx + sbx - dtdx * (F.f4 - Fprev.f4) - dtdy * (Gprev.f4 - g4m));
+                            n5 = 0.5 * (oby + sby - dtdx * (F.f5 - Fprev.f5) - dtdy * (Gprev.f5 - g5m));
+                            n6 = 0.5 * (PL(uold, 6)[os] + Bz[os] - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m));
+                            n7 = 0.5 * (PL(uold, 7)[os] + En[os] - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m));
+                        }
+                        PL(uout, 0)[os] = n0; PL(uout, 1)[os] = n1; PL(uout, 2)[os] = n2; PL(uout, 3)[os] = n3;
+                        PL(uout, 4)[os] = n4; PL(uout, 5)[os] = n5; PL(uout, 6)[os] = n6; PL(uout, 7)[os] = n7;
+                        // face fluxes of the CT-contact EMF (M/ct.cu:58-63)
+                        PL(aux, 0)[os] = F.f0;
+                        PL(aux, 1)[os] = F.f5;
+                        PL(aux, 2)[os] = Gprev.f0;
+                        PL(aux, 3)[os] = Gprev.f4;
+                    }
+                } else if (s < iend && (upd_row || stale_row)) {
                     const size_t os = (size_t)s * g.pitch + jl;
                     double n0, n1, n2, n3, n4, n5, n6, n7;
                     if (STAGE == 1) {
@@ -321,7 +426,7 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
         eta_prev = eta_c;
     }
 
-    if (STAGE == 2) {
+    if (STAGE == 2 && !MULTI) {
         accumulate_dt(ctrl, cur ^ 1, dtloc);
         const unsigned long long any = __ballot(nanflag);
         if (lane == 0 && any) atomicOr(&ctrl->crashed, 1);
@@ -453,6 +558,7 @@ __global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int sl
                 const double d = cell_dt<EXACT>(cell[q][0], cell[q][1], cell[q][2], cell[q][3], cell[q][4], cell[q][5],
                                          cell[q][6], cell[q][7], ph);
                 if (d < dloc) dloc = d;
+                if (q == 0 && i == 0 && isnan(d)) ctrl->first_nan[slot] = 1;   // cell (0, 0)
             }
         }
     }
@@ -461,7 +567,9 @@ __global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int sl
 
 // Per-cell dt over all nx*ny cells (P/resistiveMHD_2D.cu:277-286) and the
 // NaN scan of e, for a state set from outside the step.
-template <bool EXACT>
+// INTERIOR: the dt of the MPI library's calculateDt, interior cells only
+// (M/resistiveMHD2D.cu:489-524); the NaN scan still covers every cell.
+template <bool EXACT, bool INTERIOR>
 __global__ __launch_bounds__(256) void dt_all_kernel(const double *__restrict__ u, Ctrl *ctrl, int slot, Grid g,
                                                      Phys ph)
 {
@@ -475,7 +583,10 @@ __global__ __launch_bounds__(256) void dt_all_kernel(const double *__restrict__ 
             const double e = PL(u, 7)[o];
             const double d = cell_dt<EXACT>(PL(u, 0)[o], PL(u, 1)[o], PL(u, 2)[o], PL(u, 3)[o], PL(u, 4)[o],
                                             PL(u, 5)[o], PL(u, 6)[o], e, ph);
-            if (d < dloc) dloc = d;
+            const bool inside = !INTERIOR || (i >= kGhost && i < g.nx - kGhost && j >= kGhost && j < g.ny - kGhost);
+            if (inside && d < dloc) dloc = d;
+            const int first = INTERIOR ? kGhost : 0;
+            if (i == first && j == first && isnan(d)) ctrl->first_nan[slot] = 1;
             nanflag |= isnan(e) ? 1 : 0;   // checkCalculationIsCrashed, P/resistiveMHD_2D.cu:294-309
         }
     }
@@ -488,8 +599,131 @@ __global__ void ctrl_reset_kernel(Ctrl *ctrl, int slot)
 {
     if (threadIdx.x == 0) {
         ctrl->acc[slot] = kInfBits;
+        ctrl->first_nan[slot] = 0;
         ctrl->crashed = 0;
     }
+}
+
+// ------------------------------------------------ MPI library's step (M/)
+// The CT-contact EMF of M/ct.cu and the final CT, on one periodic subdomain.
+
+// Face-smoothed B: shiftUToCenterForCT then backUToCenterHalfForCT
+// (M/resistiveMHD2D.cu:597-690) of the face values around plane offset o.
+__device__ __forceinline__ double smooth_bx(const double *bx, size_t o, size_t pitch)
+{
+    const double c0 = 0.5 * (bx[o] + bx[o - pitch]);
+    const double c1 = 0.5 * (bx[o + pitch] + bx[o]);
+    return 0.5 * (c0 + c1);
+}
+__device__ __forceinline__ double smooth_by(const double *by, size_t o)
+{
+    const double c0 = 0.5 * (by[o] + by[o - 1]);
+    const double c1 = 0.5 * (by[o + 1] + by[o]);
+    return 0.5 * (c0 + c1);
+}
+
+// Cell-centred reference EMF u*bY - v*bX - eta*jZ of CT::setOldFlux2D /
+// setNowFlux2D (M/ct.cu:26-65; the G4 array there is its negation) over the
+// interior.  SMOOTH: of the face-smoothed state (stage 1 sees U^n after
+// backUToCenterHalfForCT); otherwise of the state as stored.
+template <bool SMOOTH>
+__global__ __launch_bounds__(256) void m_ec_kernel(const double *__restrict__ u, double *__restrict__ ec, Grid g,
+                                                   Phys ph)
+{
+    const int j = kGhost + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.ny - kGhost) return;
+    const double *bx = PL(u, 4), *by = PL(u, 5);
+    const size_t P = g.pitch;
+    auto BX = [&](size_t q) { return SMOOTH ? smooth_bx(bx, q, P) : bx[q]; };
+    auto BY = [&](size_t q) { return SMOOTH ? smooth_by(by, q) : by[q]; };
+    for (int i = kGhost + blockIdx.y; i < g.nx - kGhost; i += gridDim.y) {
+        const size_t o = (size_t)i * P + j;
+        const double rho = PL(u, 0)[o];
+        const double uu = PL(u, 1)[o] / rho;
+        const double vv = PL(u, 2)[o] / rho;
+        const double bX = 0.5 * (BX(o) + BX(o - P));
+        const double bY = 0.5 * (BY(o) + BY(o - 1));
+        const double jZ = 0.25 * (
+              (BY(o + P) - BY(o)) / ph.dx - (BX(o + 1) - BX(o)) / ph.dy
+            + (BY(o - 1 + P) - BY(o - 1)) / ph.dx - (BX(o) - BX(o - 1)) / ph.dy
+            + (BY(o - 1) - BY(o - 1 - P)) / ph.dx - (BX(o - P) - BX(o - P - 1)) / ph.dy
+            + (BY(o) - BY(o - P)) / ph.dx - (BX(o + 1 - P) - BX(o - P)) / ph.dy);
+        ec[o] = uu * bY - vv * bX - ph.eta * jZ;
+    }
+}
+
+__device__ __forceinline__ double msign(double x) { return (double)((x > 0.0) - (x < 0.0)); }   // M/sign.cu
+
+// Corner EMF of one flux set (M/ct.cu:147-169).  aux planes: 0 = F.f0,
+// 1 = F.f5 (x-faces), 2 = G.f0, 3 = G.f4 (y-faces); ec = F5, G4 = -ec.
+// k, kN, kE, kNE: offsets of (i, j), (i, j+1), (i+1, j), (i+1, j+1).
+__device__ __forceinline__ double contact_emf(const double *aux, const double *ec, long long plane, size_t k,
+                                              size_t kN, size_t kE, size_t kNE)
+{
+    const double *NF0 = aux, *NF5 = aux + plane, *NG0 = aux + 2 * plane, *NG4 = aux + 3 * plane;
+    const double avg = 0.25 * (-NF5[k] - NF5[kN] + NG4[k] + NG4[kE]);
+    const double sS = msign(NF0[k]), sN = msign(NF0[kN]);
+    const double sW = msign(NG0[k]), sE = msign(NG0[kE]);
+    const double eS = (1.0 + sS) * (NG4[k] - (-ec[k])) + (1.0 - sS) * (NG4[kE] - (-ec[kE]));
+    const double eN = (1.0 + sN) * ((-ec[kN]) - NG4[k]) + (1.0 - sN) * ((-ec[kNE]) - NG4[kE]);
+    const double eW = -(1.0 + sW) * (NF5[k] - ec[k]) - (1.0 - sW) * (NF5[kN] - ec[kN]);
+    const double eE = -(1.0 + sE) * (ec[kE] - NF5[k]) - (1.0 - sE) * (ec[kNE] - NF5[kN]);
+    return avg + 0.125 * (eS - eN + eW - eE);
+}
+
+// E = (E_old + E_now) / 2 at corners (i+1/2, j+1/2), i in [2, nx-3),
+// j in [2, ny-3) (CT::divBClean, M/ct.cu:120-218).  The per-face and
+// per-cell inputs are stored for the interior only; the ghost layer's values
+// are those of the periodic image, which the self-exchange halo makes equal.
+__global__ __launch_bounds__(256) void m_emf_kernel(const double *__restrict__ aux_old,
+                                                    const double *__restrict__ aux_now,
+                                                    const double *__restrict__ ec_old,
+                                                    const double *__restrict__ ec_now, double *__restrict__ E, Grid g)
+{
+    const int j = kGhost - 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= g.ny - kGhost) return;
+    const int lnx = g.nx - 2 * kGhost, lny = g.ny - 2 * kGhost;
+    auto wrap = [](int a, int n, int l) { return a < kGhost ? a + l : (a >= n - kGhost ? a - l : a); };
+    const int wj = wrap(j, g.ny, lny), wj1 = wrap(j + 1, g.ny, lny);
+    for (int i = kGhost - 1 + blockIdx.y; i < g.nx - kGhost; i += gridDim.y) {
+        const size_t wi = (size_t)wrap(i, g.nx, lnx) * g.pitch, wi1 = (size_t)wrap(i + 1, g.nx, lnx) * g.pitch;
+        const size_t k = wi + wj, kN = wi + wj1, kE = wi1 + wj, kNE = wi1 + wj1;
+        const double eOld = contact_emf(aux_old, ec_old, g.plane, k, kN, kE, kNE);
+        const double eNow = contact_emf(aux_now, ec_now, g.plane, k, kN, kE, kNE);
+        E[(size_t)i * g.pitch + j] = 0.5 * (eOld + eNow);
+    }
+}
+
+// Final CT from the step's initial face B (M/ct.cu:172-218) over the
+// interior, then the per-cell dt and NaN flag of the new state (the next
+// step's calculateDt, M/resistiveMHD2D.cu:489-524).
+template <bool EXACT>
+__global__ __launch_bounds__(256) void m_ct_kernel(const double *__restrict__ uold, const double *__restrict__ E,
+                                                   double *__restrict__ u, Ctrl *__restrict__ ctrl, int slot, Grid g,
+                                                   Phys ph)
+{
+    const int j = kGhost + blockIdx.x * blockDim.x + threadIdx.x;
+    const double dt = ctrl->dt;
+    double dloc = __longlong_as_double((long long)kInfBits);
+    int nanflag = 0;
+    if (j < g.ny - kGhost) {
+        for (int i = kGhost + blockIdx.y; i < g.nx - kGhost; i += gridDim.y) {
+            const size_t o = (size_t)i * g.pitch + j;
+            const double nbx = PL(uold, 4)[o] - dt / ph.dy * (E[o] - E[o - 1]);
+            const double nby = PL(uold, 5)[o] + dt / ph.dx * (E[o] - E[o - g.pitch]);
+            PL(u, 4)[o] = nbx;
+            PL(u, 5)[o] = nby;
+            const double e = PL(u, 7)[o];
+            const double d = cell_dt<EXACT>(PL(u, 0)[o], PL(u, 1)[o], PL(u, 2)[o], PL(u, 3)[o], nbx, nby, PL(u, 6)[o],
+                                            e, ph);
+            if (d < dloc) dloc = d;
+            if (i == kGhost && j == kGhost && isnan(d)) ctrl->first_nan[slot] = 1;
+            nanflag |= isnan(e) ? 1 : 0;
+        }
+    }
+    accumulate_dt(ctrl, slot, dloc);
+    const unsigned long long any = __ballot(nanflag);
+    if ((threadIdx.x & 63) == 0 && any) atomicOr(&ctrl->crashed, 1);
 }
 
 // AoS (reference ConservationParameter order) <-> SoA planes.
@@ -537,20 +771,54 @@ static int grid_for(long long n, int block)
     return (int)b;
 }
 
-hipError_t launch_stage(int stage, bool exact, const double *uin, double *uout, const double *uold,
-                        const double *eta, double *side, Ctrl *ctrl, const Grid &g, const Phys &ph, int cur,
-                        hipStream_t s)
+hipError_t launch_stage(int stage, bool exact, int scheme, const double *uin, double *uout, const double *uold,
+                        const double *eta, double *side, double *aux, Ctrl *ctrl, const Grid &g, const Phys &ph,
+                        int cur, hipStream_t s)
 {
     const int blocks = g.xcds * g.items_per_xcd;
-#define MHD_LAUNCH_STAGE(ST, EX)                                                                               \
-    hipLaunchKernelGGL((rk_stage_kernel<ST, EX>), dim3(blocks), dim3(64), 0, s, uin, uout, uold, eta, side, ctrl, \
-                       g, ph, cur)
-    if (stage == 1) {
-        if (exact) MHD_LAUNCH_STAGE(1, true); else MHD_LAUNCH_STAGE(1, false);
+#define MHD_LAUNCH_STAGE(ST, EX, SC)                                                                            \
+    hipLaunchKernelGGL((rk_stage_kernel<ST, EX, SC>), dim3(blocks), dim3(64), 0, s, uin, uout, uold, eta, side, \
+                       aux, ctrl, g, ph, cur)
+#define MHD_LAUNCH_STAGE_EX(ST, SC) \
+    do { if (exact) MHD_LAUNCH_STAGE(ST, true, SC); else MHD_LAUNCH_STAGE(ST, false, SC); } while (0)
+    if (scheme == 0) {
+        if (stage == 1) MHD_LAUNCH_STAGE_EX(1, 0); else MHD_LAUNCH_STAGE_EX(2, 0);
     } else {
-        if (exact) MHD_LAUNCH_STAGE(2, true); else MHD_LAUNCH_STAGE(2, false);
+        if (stage == 1) MHD_LAUNCH_STAGE_EX(1, 1); else MHD_LAUNCH_STAGE_EX(2, 1);
     }
+#undef MHD_LAUNCH_STAGE_EX
 #undef MHD_LAUNCH_STAGE
+    return hipGetLastError();
+}
+
+// 2D launch shape of the interior elementwise kernels: threads along y
+// (contiguous), blocks along x folded over at most 16384 rows.
+static dim3 interior_grid(const Grid &g, int extra)
+{
+    const int cols = g.ny - 2 * kGhost + extra;
+    const int rows = g.nx - 2 * kGhost + extra;
+    return dim3((cols + 255) / 256, rows < 16384 ? rows : 16384);
+}
+
+hipError_t launch_m_ec(bool smooth, const double *u, double *ec, const Grid &g, const Phys &ph, hipStream_t s)
+{
+    if (smooth) hipLaunchKernelGGL((m_ec_kernel<true>), interior_grid(g, 0), dim3(256), 0, s, u, ec, g, ph);
+    else hipLaunchKernelGGL((m_ec_kernel<false>), interior_grid(g, 0), dim3(256), 0, s, u, ec, g, ph);
+    return hipGetLastError();
+}
+
+hipError_t launch_m_emf(const double *aux_old, const double *aux_now, const double *ec_old, const double *ec_now,
+                        double *E, const Grid &g, hipStream_t s)
+{
+    hipLaunchKernelGGL(m_emf_kernel, interior_grid(g, 1), dim3(256), 0, s, aux_old, aux_now, ec_old, ec_now, E, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *u, Ctrl *ctrl, int slot,
+                       const Grid &g, const Phys &ph, hipStream_t s)
+{
+    if (exact) hipLaunchKernelGGL((m_ct_kernel<true>), interior_grid(g, 0), dim3(256), 0, s, uold, E, u, ctrl, slot, g, ph);
+    else hipLaunchKernelGGL((m_ct_kernel<false>), interior_grid(g, 0), dim3(256), 0, s, uold, E, u, ctrl, slot, g, ph);
     return hipGetLastError();
 }
 
@@ -582,13 +850,15 @@ hipError_t launch_bc_y(int kind, bool exact, double *u, Ctrl *ctrl, int slot, co
     return hipGetLastError();
 }
 
-hipError_t launch_dt_all(bool exact, const double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
-                         hipStream_t s)
+hipError_t launch_dt_all(bool exact, bool interior, const double *u, Ctrl *ctrl, int slot, const Grid &g,
+                         const Phys &ph, hipStream_t s)
 {
     hipLaunchKernelGGL(ctrl_reset_kernel, dim3(1), dim3(64), 0, s, ctrl, slot);
     const dim3 grid(g.nx < 1024 ? g.nx : 1024);
-    if (exact) hipLaunchKernelGGL((dt_all_kernel<true>), grid, dim3(256), 0, s, u, ctrl, slot, g, ph);
-    else hipLaunchKernelGGL((dt_all_kernel<false>), grid, dim3(256), 0, s, u, ctrl, slot, g, ph);
+#define MHD_LAUNCH_DT(EX, IN) hipLaunchKernelGGL((dt_all_kernel<EX, IN>), grid, dim3(256), 0, s, u, ctrl, slot, g, ph)
+    if (exact) { if (interior) MHD_LAUNCH_DT(true, true); else MHD_LAUNCH_DT(true, false); }
+    else { if (interior) MHD_LAUNCH_DT(false, true); else MHD_LAUNCH_DT(false, false); }
+#undef MHD_LAUNCH_DT
     return hipGetLastError();
 }
 
@@ -596,11 +866,11 @@ int stage_waves_per_cu(bool exact)
 {
     int a = 0, b = 0;
     if (exact) {
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_kernel<1, true>, 64, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_kernel<2, true>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_kernel<1, true, 0>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_kernel<2, true, 0>, 64, 0);
     } else {
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_kernel<1, false>, 64, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_kernel<2, false>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_kernel<1, false, 0>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_kernel<2, false, 0>, 64, 0);
     }
     const int w = a < b ? a : b;
     return w > 0 ? w : 1;

@@ -28,11 +28,14 @@ EXPORTS = (
     "mhd_get_time", "mhd_set_time", "mhd_check_crashed", "mhd_save", "mhd_load", "mhd_synchronize",
     "mhd_fill_orszag_tang", "mhd_fill_harris", "mhd_fill_harris_eta", "mhd_init_orszag_tang",
     "mhd_init_harris", "mhd_set_sweep_length", "mhd_get_sweep_length", "mhd_profile_enable",
-    "mhd_profile_read", "mhd_set_numerics", "mhd_get_numerics",
+    "mhd_profile_read", "mhd_set_numerics", "mhd_get_numerics", "mhd_fill_turbulence",
+    "mhd_init_turbulence", "mhd_set_scheme", "mhd_get_scheme",
 )
 
 MHD_NUMERICS_FAST = 0
 MHD_NUMERICS_EXACT = 1
+MHD_SCHEME_SINGLE = 0
+MHD_SCHEME_MULTI_SHARE = 1
 
 
 class MhdConfig(ctypes.Structure):
@@ -102,6 +105,10 @@ def load_library(path=LIB_PATH):
         "mhd_profile_read": (I, [P, D, D, ctypes.POINTER(I)]),
         "mhd_set_numerics": (I, [P, I]),
         "mhd_get_numerics": (I, [P, ctypes.POINTER(I)]),
+        "mhd_fill_turbulence": (I, [C, ctypes.c_ulonglong, I, ctypes.c_double, ctypes.c_double, D]),
+        "mhd_init_turbulence": (I, [P, ctypes.c_ulonglong, I, ctypes.c_double, ctypes.c_double]),
+        "mhd_set_scheme": (I, [P, I]),
+        "mhd_get_scheme": (I, [P, ctypes.POINTER(I)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -145,6 +152,12 @@ def harris_config(nx=4096, ny=2048, eta_max=6e-3):
                           bc_x=MHD_BC_PERIODIC, bc_y=MHD_BC_SYMMETRIC)
 
 
+def turbulence_config(n=8192, eta=1e-4):
+    """Decaying turbulence on [0, 2pi]^2, periodic x and y (one GPU's
+    subdomain of the MPI library's decomposition)."""
+    return default_config(n, n, 0.0, 2.0 * np.pi, 0.0, 2.0 * np.pi, eta=eta)
+
+
 def fill_orszag_tang(cfg):
     out = np.empty((cfg.nx, cfg.ny, NVAR), dtype=np.float64)
     _check(load_library().mhd_fill_orszag_tang(ctypes.byref(cfg), _dptr(out)))
@@ -154,6 +167,12 @@ def fill_orszag_tang(cfg):
 def fill_harris(cfg, seed=0, noise_amp=1e-3):
     out = np.empty((cfg.nx, cfg.ny, NVAR), dtype=np.float64)
     _check(load_library().mhd_fill_harris(ctypes.byref(cfg), seed, noise_amp, _dptr(out)))
+    return out
+
+
+def fill_turbulence(cfg, seed=0, kmax=8, rms_v=0.5, rms_b=0.5):
+    out = np.empty((cfg.nx, cfg.ny, NVAR), dtype=np.float64)
+    _check(load_library().mhd_fill_turbulence(ctypes.byref(cfg), seed, kmax, rms_v, rms_b, _dptr(out)))
     return out
 
 
@@ -302,7 +321,7 @@ class ResistiveMHD2D:
         self._ok(self._lib.mhd_synchronize(self._h))
 
     def set_numerics(self, mode):
-        """MHD_NUMERICS_FAST (default) or MHD_NUMERICS_EXACT."""
+        """MHD_NUMERICS_EXACT (default) or MHD_NUMERICS_FAST ("exact"/"fast")."""
         if isinstance(mode, str):
             mode = {"fast": MHD_NUMERICS_FAST, "exact": MHD_NUMERICS_EXACT}[mode]
         self._ok(self._lib.mhd_set_numerics(self._h, int(mode)))
@@ -335,3 +354,17 @@ class ResistiveMHD2D:
 
     def initHarris(self, seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3):
         self._ok(self._lib.mhd_init_harris(self._h, seed, noise_amp, eta0, eta1))
+
+    def initTurbulence(self, seed=0, kmax=8, rms_v=0.5, rms_b=0.5):
+        self._ok(self._lib.mhd_init_turbulence(self._h, seed, kmax, rms_v, rms_b))
+
+    def set_scheme(self, scheme):
+        """MHD_SCHEME_SINGLE (default) or MHD_SCHEME_MULTI_SHARE ("single"/"multi")."""
+        if isinstance(scheme, str):
+            scheme = {"single": MHD_SCHEME_SINGLE, "multi": MHD_SCHEME_MULTI_SHARE}[scheme]
+        self._ok(self._lib.mhd_set_scheme(self._h, int(scheme)))
+
+    def scheme(self):
+        v = ctypes.c_int()
+        self._ok(self._lib.mhd_get_scheme(self._h, ctypes.byref(v)))
+        return "multi" if v.value == MHD_SCHEME_MULTI_SHARE else "single"

@@ -87,11 +87,45 @@ def test_harris_setup_and_eta():
     np.testing.assert_allclose(eta, 1e-3 + 5e-3 * np.exp(-(X ** 2 + Y ** 2) / 4), rtol=1e-14)
 
 
+def test_turbulence_setup():
+    """rho = p = 1, interior rms |v| and |B| as requested, face divergence of
+    B zero to round-off (B is the discrete curl of Az), modes |k| <= kmax,
+    seed-reproducible."""
+    cfg = mhd.default_config(64, 48, 0.0, 2.0 * np.pi, 0.0, 1.5 * np.pi, eta=1e-4)
+    U = mhd.fill_turbulence(cfg, seed=5, kmax=8, rms_v=0.5, rms_b=0.25)
+    g = mhd.GHOST
+    I = U[g:-g, g:-g]
+    assert np.all(U[..., 0] == 1.0) and np.all(U[..., 3] == 0.0) and np.all(U[..., 6] == 0.0)
+    np.testing.assert_allclose(np.sqrt((I[..., 1] ** 2 + I[..., 2] ** 2).mean()), 0.5, rtol=1e-12)
+    np.testing.assert_allclose(np.sqrt((I[..., 4] ** 2 + I[..., 5] ** 2).mean()), 0.25, rtol=1e-12)
+    bx, by = U[..., 4], U[..., 5]
+    div = ((bx[g:-g, g:-g] - bx[g - 1:-g - 1, g:-g]) / cfg.dx
+           + (by[g:-g, g:-g] - by[g:-g, g - 1:-g - 1]) / cfg.dy)
+    assert np.abs(div).max() < 1e-13 * np.abs(bx).max() / cfg.dx
+    # energy from p = 1 and the cell-centred field
+    bxc = 0.5 * (bx[1:, :] + bx[:-1, :])[:, 1:]
+    byc = 0.5 * (by[:, 1:] + by[:, :-1])[1:, :]
+    e = 1.0 / (2.0 / 3.0) + 0.5 * (U[1:, 1:, 1] ** 2 + U[1:, 1:, 2] ** 2) + 0.5 * (bxc ** 2 + byc ** 2)
+    np.testing.assert_allclose(U[1:, 1:, 7], e, rtol=1e-13)
+    # spectrum of the interior velocity: nothing beyond |n| = kmax
+    spec = np.abs(np.fft.fft2(I[..., 1])) ** 2
+    nx = np.fft.fftfreq(64, 1.0 / 64)[:, None]
+    ny = np.fft.fftfreq(48, 1.0 / 48)[None, :]
+    outside = (nx ** 2 + ny ** 2) > 64
+    assert spec[outside].sum() < 1e-20 * spec.sum()
+    assert spec[~outside].sum() > 0
+    assert np.array_equal(U, mhd.fill_turbulence(cfg, seed=5, kmax=8, rms_v=0.5, rms_b=0.25))
+    assert not np.array_equal(U, mhd.fill_turbulence(cfg, seed=6, kmax=8, rms_v=0.5, rms_b=0.25))
+
+
 def test_fill_rejects_bad_config():
     cfg = mhd.orszag_tang_config(32)
     cfg.nx = 5
     with pytest.raises(mhd.MHDError):
         mhd.fill_orszag_tang(cfg)
+    cfg = mhd.orszag_tang_config(16)
+    with pytest.raises(mhd.MHDError):   # kmax = 8 is not resolved by 16 cells
+        mhd.fill_turbulence(cfg, kmax=8)
 
 
 CPP_MAIN = r"""
