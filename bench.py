@@ -93,10 +93,98 @@ def cpu_baseline(nx, ny, steps, workload="harris"):
     }
 
 
+BATCH_SIZES = (64, 128, 192, 256, 384, 512)
+
+
+def batch_configs(mhd, nruns, seed0=0):
+    """The batched workload: run s draws (setup, N, eta) from a generator
+    seeded with seed0 + s: Orszag-Tang or Harris sheet (50/50) on N x N
+    cells, N uniform over BATCH_SIZES, eta log-uniform in [1e-4, 1e-2]."""
+    import numpy as np
+    runs = []
+    for s in range(nruns):
+        rng = np.random.default_rng(seed0 + s)
+        harris = bool(rng.integers(2))
+        n = int(rng.choice(BATCH_SIZES))
+        eta = float(10.0 ** rng.uniform(-4.0, -2.0))
+        cfg = mhd.harris_config(n, n, eta_max=eta) if harris else mhd.orszag_tang_config(n, eta=eta)
+        cfg.eta = eta
+        runs.append((cfg, "harris" if harris else "orszag_tang", seed0 + s))
+    return runs
+
+
+def batch_bench(nruns, steps, plain_steps, run_steps, numerics):
+    """Runs/s of the batched form against a loop of single handles advancing
+    the same runs one after another, both with states resident on the GPU.
+    runs/s = runs * (timed steps / steps per run) / seconds."""
+    import torch
+    import resistivemhd_gpu_amd as mhd
+    runs = batch_configs(mhd, nruns)
+    b = mhd.ResistiveMHD2DBatch([c for c, _, _ in runs])
+    b.set_numerics(numerics)
+    for r, (c, kind, seed) in enumerate(runs):
+        if kind == "harris":
+            b.initHarris(r, seed=seed, noise_amp=1e-3)
+        else:
+            b.initOrszagTang(r)
+    b.applyBoundary()
+    t0 = time.perf_counter()
+    sweep = b.autotune(steps=2)
+    t_tune = time.perf_counter() - t0
+    b.steps(4)
+    b.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.steps(steps)
+    b.synchronize()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    crashed = any(b.checkCalculationIsCrashed(r) for r in range(nruns))
+    del b
+
+    singles = []
+    for c, kind, seed in runs:
+        s = mhd.ResistiveMHD2D(c)
+        s.set_numerics(numerics)
+        if kind == "harris":
+            s.initializeU(mhd.fill_harris(c, seed=seed, noise_amp=1e-3))
+        else:
+            s.initOrszagTang()
+        s.steps(2)
+        singles.append(s)
+    for s in singles:
+        s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in singles:              # the plain per-instance loop: one run after another
+        s.steps(plain_steps)
+        s.synchronize()
+    t_plain = time.perf_counter() - t0
+    del singles
+    cells = sum((c.nx - 6) * (c.ny - 6) for c, _, _ in runs)
+    batched = nruns * steps / run_steps / t_batch
+    plain = nruns * plain_steps / run_steps / t_plain
+    return {
+        "runs": nruns, "steps_per_run": run_steps, "numerics": numerics,
+        "interior_cells_total": cells,
+        "batched_runs_per_s": batched, "plain_runs_per_s": plain, "speedup_vs_plain": batched / plain,
+        "batched_ms_per_step": t_batch / steps * 1e3, "plain_ms_per_step": t_plain / plain_steps * 1e3,
+        "batched_zone_cycles_per_s": cells * steps / t_batch,
+        "timed_steps": {"batched": steps, "plain": plain_steps},
+        "autotuned_sweep_columns": sweep, "autotune_s": t_tune, "crashed": crashed,
+        "workload": f"{nruns} runs: Orszag-Tang / Harris 50/50, N in {list(BATCH_SIZES)} squared, "
+                    "eta log-uniform [1e-4, 1e-2], seeded per run",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["harris", "turbulence"], default="harris")
+    ap.add_argument("--workload", choices=["harris", "turbulence", "batch"], default="harris")
+    ap.add_argument("--batch-runs", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=40, help="timed batched steps")
+    ap.add_argument("--plain-steps", type=int, default=10, help="timed steps per run of the plain loop")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form line in the default run")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=0, help="interior cells along x (0: 4096 harris, 8192 turbulence)")
@@ -122,6 +210,14 @@ def main():
     sys.path.insert(0, root)
     import torch
     import resistivemhd_gpu_amd as mhd
+
+    if args.workload == "batch":
+        res = batch_bench(args.batch_runs, args.batch_steps, args.plain_steps, 500, args.numerics)
+        print(json.dumps({"metric": "batched runs/s vs plain per-instance loop", "value": res["batched_runs_per_s"],
+                          "unit": "runs/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "data": "synthetic", "config": {"workload": res["workload"]}, "batched": res}),
+              flush=True)
+        return
 
     multi = args.workload == "turbulence"
     if multi:
@@ -219,6 +315,8 @@ def main():
         torch.cuda.synchronize()
         line["calibration_copy_bytes"] = n * 8
         del src, dst
+    if not multi and not args.no_batch and not args.no_cpu:
+        line["batched"] = batch_bench(args.batch_runs, args.batch_steps, args.plain_steps, 500, args.numerics)
     if not args.no_cpu:
         if multi:
             line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_nx, args.cpu_steps, "turbulence")

@@ -204,6 +204,49 @@ int mhd_get_sweep_length(const mhd_handle *h, int *columns);
 int mhd_profile_enable(mhd_handle *h, int on);
 int mhd_profile_read(mhd_handle *h, double *ms_stage1, double *ms_stage2, int *steps);
 
+/* ------------------------------------------------------------------ batch
+ * Batched form (no counterpart in the reference, which runs one simulation
+ * per process because dt is a global, P/const.hpp:20,44): nruns independent
+ * runs, each with its own mhd_config (grid size, dx, dy, gamma, CFL, eta,
+ * boundaries), state, optional eta field and dt, advanced together.  Each RK
+ * stage is one launch over a work list of every run's (strip, segment)
+ * items and each boundary pass one launch over all runs; steps replay a
+ * captured hipGraph.  Per run the arithmetic is that of mhd_step_rk2 with
+ * MHD_SCHEME_SINGLE: with the same numerics a run's state is bitwise the one
+ * a single handle produces.  States use the same AoS layout as
+ * mhd_set_state. */
+typedef struct mhd_batch mhd_batch;
+int mhd_batch_create(const mhd_config *cfgs, int nruns, mhd_batch **out);
+int mhd_batch_destroy(mhd_batch *b);
+const char *mhd_batch_last_error(const mhd_batch *b);   /* b may be NULL */
+int mhd_batch_size(const mhd_batch *b, int *nruns);
+int mhd_batch_set_state(mhd_batch *b, int run, const double *host_aos);
+int mhd_batch_get_state(mhd_batch *b, int run, double *host_aos);
+int mhd_batch_set_eta_field(mhd_batch *b, int run, const double *host_eta);   /* NULL: uniform cfg.eta */
+/* the configured boundary of every run (x then y) */
+int mhd_batch_apply_boundary(mhd_batch *b);
+/* setups of mhd_fill_orszag_tang / mhd_fill_harris uploaded to one run
+ * (the boundary is applied by mhd_batch_apply_boundary) */
+int mhd_batch_init_orszag_tang(mhd_batch *b, int run);
+int mhd_batch_init_harris(mhd_batch *b, int run, unsigned long long seed, double noise_amp);
+/* nsteps RK2 steps of every run, no host synchronisation */
+int mhd_batch_step(mhd_batch *b, int nsteps);
+int mhd_batch_synchronize(mhd_batch *b);
+int mhd_batch_get_dt(mhd_batch *b, int run, double *dt);
+int mhd_batch_get_time(mhd_batch *b, int run, double *t);
+int mhd_batch_check_crashed(mhd_batch *b, int run, int *crashed);
+int mhd_batch_set_numerics(mhd_batch *b, int mode);
+/* sweep length of the work items (0: chosen from the occupancy) */
+int mhd_batch_set_sweep_length(mhd_batch *b, int columns);
+int mhd_batch_get_sweep_length(const mhd_batch *b, int *columns);
+/* replay steps from a captured hipGraph (default on) */
+int mhd_batch_set_graph(mhd_batch *b, int on);
+/* Autotuner: times 2*steps batch steps for each candidate sweep length on
+ * the batch's current states, keeps the fastest (written to *columns if not
+ * NULL) and restores every state, time and dt, so the runs continue as if
+ * it had not been called. */
+int mhd_batch_autotune(mhd_batch *b, int steps, int *columns);
+
 #ifdef __cplusplus
 }
 #endif

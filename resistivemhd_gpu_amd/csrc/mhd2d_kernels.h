@@ -19,6 +19,18 @@ struct Grid {
     int items_per_xcd;   // ceil(nstrips * nsegs / xcds)
 };
 
+// One run of a batch: its grid, constants, buffers and control block, and
+// the index of its first item in the batch's work list.
+struct RunDesc {
+    Grid g;
+    Phys ph;
+    double *u, *us, *eta, *side;
+    Ctrl *ctrl;
+    int bc_x, bc_y;
+    int first_item;
+    int pad;
+};
+
 // exact = true: operation-for-operation reference arithmetic (built without
 // contraction it rounds as the reference does); false: fast numerics.
 // scheme 0: the single-GPU library's stage; 1: the MPI library's stage on
@@ -42,6 +54,14 @@ hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *
                        const Grid &g, const Phys &ph, hipStream_t s);
 // Resident 64-lane stage workgroups per CU (the lower of the two stages).
 int stage_waves_per_cu(bool exact);
+// Batched stage over a work list of total_items items (item_run: run of
+// each item) and the batched boundary pair of every run's U* (stage_state)
+// or U, optionally reducing the ghost cells' dt into slot.
+hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const int *item_run, int total_items,
+                              int xcds, int items_per_xcd, int cur, hipStream_t s);
+hipError_t launch_bc_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int max_ny, bool stage_state,
+                           bool with_dt, int slot, hipStream_t s);
+int stage_batch_waves_per_cu(bool exact);
 hipError_t launch_unpack(const double *aos, double *u, const Grid &g, hipStream_t s);
 hipError_t launch_pack(const double *u, double *aos, const Grid &g, hipStream_t s);
 hipError_t launch_scatter_plane(const double *src, double *dst, const Grid &g, hipStream_t s);

@@ -84,22 +84,17 @@ __device__ __forceinline__ Cell make_cell(double rho, double mx, double my, doub
 // no CT in the stages; the face fluxes the CT-contact EMF needs are stored
 // in `aux` (planes F0, F5 at x-faces and G0, G4 at y-faces, owned by the
 // cell on their low side) and the stage-2 state goes to uout != uold.
+//
+// stage_item is one work item (strip sy, segment sx) of one run; the
+// single-run kernel and the batched work-list kernel both call it.
 template <int STAGE, bool EXACT, int SCHEME>
-__global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
-void rk_stage_kernel(const double *__restrict__ uin, double *uout,
-                                                      const double *uold, const double *__restrict__ etaf,
-                                                      double *__restrict__ side, double *__restrict__ aux,
-                                                      Ctrl *__restrict__ ctrl, Grid g, Phys ph, int cur)
+__device__ __forceinline__ void stage_item(int item, const double *__restrict__ uin, double *uout, const double *uold,
+                                           const double *__restrict__ etaf, double *__restrict__ side,
+                                           double *__restrict__ aux, Ctrl *__restrict__ ctrl, const Grid &g,
+                                           const Phys &ph, int cur)
 {
     constexpr bool MULTI = SCHEME == 1;
     const int lane = threadIdx.x;
-    // Workgroups are dealt to the XCDs round-robin (block b runs on XCD
-    // b % xcds).  Each XCD gets a contiguous range of items (strips fastest),
-    // so neighbouring strips and segments, which read the same halo lines,
-    // share that XCD's L2.
-    const int item = (blockIdx.x % g.xcds) * g.items_per_xcd + blockIdx.x / g.xcds;
-    if (item >= g.nstrips * g.nsegs) return;
     const int sy = item % g.nstrips;
     const int sx = item / g.nstrips;
     const int j0 = kGhost + sy * kStripRows;
@@ -433,14 +428,55 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout,
     }
 }
 
+// Workgroups are dealt to the XCDs round-robin (block b runs on XCD
+// b % xcds).  Each XCD gets a contiguous range of items (strips fastest), so
+// neighbouring strips and segments, which read the same halo lines, share
+// that XCD's L2.
+__device__ __forceinline__ int xcd_item(int xcds, int items_per_xcd)
+{
+    return (int)(blockIdx.x % xcds) * items_per_xcd + (int)(blockIdx.x / xcds);
+}
+
+template <int STAGE, bool EXACT, int SCHEME>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
+void rk_stage_kernel(const double *__restrict__ uin, double *uout, const double *uold,
+                     const double *__restrict__ etaf, double *__restrict__ side, double *__restrict__ aux,
+                     Ctrl *__restrict__ ctrl, Grid g, Phys ph, int cur)
+{
+    const int item = xcd_item(g.xcds, g.items_per_xcd);
+    if (item >= g.nstrips * g.nsegs) return;
+    stage_item<STAGE, EXACT, SCHEME>(item, uin, uout, uold, etaf, side, aux, ctrl, g, ph, cur);
+}
+
+// Batched form: one launch advances every run of a batch by one stage.  The
+// work list holds the items of all runs back to back; item_run maps an item
+// to its run, whose descriptor gives the grid, constants and buffers.  Each
+// run keeps its own dt (its own control block).
+template <int STAGE, bool EXACT>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
+void rk_stage_batch_kernel(const RunDesc *__restrict__ runs, const int *__restrict__ item_run, int total_items,
+                           int xcds, int items_per_xcd, int cur)
+{
+    const int item = xcd_item(xcds, items_per_xcd);
+    if (item >= total_items) return;
+    const RunDesc &d = runs[item_run[item]];
+    const Grid g = d.g;
+    const Phys ph = d.ph;
+    if (STAGE == 1)
+        stage_item<1, EXACT, 0>(item - d.first_item, d.u, d.us, d.u, d.eta, d.side, nullptr, d.ctrl, g, ph, cur);
+    else
+        stage_item<2, EXACT, 0>(item - d.first_item, d.us, d.u, d.u, d.eta, d.side, nullptr, d.ctrl, g, ph, cur);
+}
+
 // ------------------------------------------------------------ boundaries
 // Periodic x (P/boundary.cu:4-71) or symmetric x (S/boundary.cu:4-104), one
 // thread per row.  The symmetric bX rebuild reads By of the interior columns
 // the mirror copied from (identical values), so one pass suffices.
 template <int KIND, bool EXACT>
-__global__ __launch_bounds__(256) void bc_x_kernel(double *u, Ctrl *ctrl, int slot, Grid g, Phys ph)
+__device__ __forceinline__ void bc_x_row(int j, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
     double dloc = __longlong_as_double((long long)kInfBits);
     if (j < g.ny) {
         const int nx = g.nx;
@@ -499,12 +535,17 @@ __global__ __launch_bounds__(256) void bc_x_kernel(double *u, Ctrl *ctrl, int sl
     if (ctrl) accumulate_dt(ctrl, slot, dloc);
 }
 
+template <int KIND, bool EXACT>
+__global__ __launch_bounds__(256) void bc_x_kernel(double *u, Ctrl *ctrl, int slot, Grid g, Phys ph)
+{
+    bc_x_row<KIND, EXACT>(blockIdx.x * blockDim.x + threadIdx.x, u, ctrl, slot, g, ph);
+}
+
 // Periodic y (P/boundary.cu:74-141) or symmetric y (S/boundary.cu:107-205),
 // one thread per column, after the x boundary.
 template <int KIND, bool EXACT>
-__global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int slot, Grid g, Phys ph)
+__device__ __forceinline__ void bc_y_col(int i, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double dloc = __longlong_as_double((long long)kInfBits);
     if (i < g.nx) {
         const int ny = g.ny;
@@ -563,6 +604,44 @@ __global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int sl
         }
     }
     if (ctrl) accumulate_dt(ctrl, slot, dloc);
+}
+
+template <int KIND, bool EXACT>
+__global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int slot, Grid g, Phys ph)
+{
+    bc_y_col<KIND, EXACT>(blockIdx.x * blockDim.x + threadIdx.x, u, ctrl, slot, g, ph);
+}
+
+// Batched boundaries: blockIdx.y is the run; which state (U or U*) and
+// whether the dt of the ghost cells is reduced are the same for every run.
+template <bool EXACT>
+__global__ __launch_bounds__(256) void bc_x_batch_kernel(const RunDesc *__restrict__ runs, int stage_state,
+                                                         int with_dt, int slot)
+{
+    const RunDesc &d = runs[blockIdx.y];
+    const Grid g = d.g;
+    const Phys ph = d.ph;
+    if ((int)(blockIdx.x * blockDim.x) >= g.ny) return;   // whole workgroup past this run's rows
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    double *u = stage_state ? d.us : d.u;
+    Ctrl *ctrl = with_dt ? d.ctrl : nullptr;
+    if (d.bc_x == 0) bc_x_row<0, EXACT>(j, u, ctrl, slot, g, ph);
+    else bc_x_row<1, EXACT>(j, u, ctrl, slot, g, ph);
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void bc_y_batch_kernel(const RunDesc *__restrict__ runs, int stage_state,
+                                                         int with_dt, int slot)
+{
+    const RunDesc &d = runs[blockIdx.y];
+    const Grid g = d.g;
+    const Phys ph = d.ph;
+    if ((int)(blockIdx.x * blockDim.x) >= g.nx) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double *u = stage_state ? d.us : d.u;
+    Ctrl *ctrl = with_dt ? d.ctrl : nullptr;
+    if (d.bc_y == 0) bc_y_col<0, EXACT>(i, u, ctrl, slot, g, ph);
+    else bc_y_col<1, EXACT>(i, u, ctrl, slot, g, ph);
 }
 
 // Per-cell dt over all nx*ny cells (P/resistiveMHD_2D.cu:277-286) and the
@@ -791,6 +870,51 @@ hipError_t launch_stage(int stage, bool exact, int scheme, const double *uin, do
     return hipGetLastError();
 }
 
+hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const int *item_run, int total_items,
+                              int xcds, int items_per_xcd, int cur, hipStream_t s)
+{
+    const int blocks = xcds * items_per_xcd;
+#define MHD_LAUNCH_BATCH(ST, EX)                                                                                \
+    hipLaunchKernelGGL((rk_stage_batch_kernel<ST, EX>), dim3(blocks), dim3(64), 0, s, runs, item_run, total_items, \
+                       xcds, items_per_xcd, cur)
+    if (stage == 1) {
+        if (exact) MHD_LAUNCH_BATCH(1, true); else MHD_LAUNCH_BATCH(1, false);
+    } else {
+        if (exact) MHD_LAUNCH_BATCH(2, true); else MHD_LAUNCH_BATCH(2, false);
+    }
+#undef MHD_LAUNCH_BATCH
+    return hipGetLastError();
+}
+
+hipError_t launch_bc_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int max_ny, bool stage_state,
+                           bool with_dt, int slot, hipStream_t s)
+{
+    const dim3 gx((max_ny + 255) / 256, nruns), gy((max_nx + 255) / 256, nruns);
+    const int st = stage_state ? 1 : 0, wd = with_dt ? 1 : 0;
+    if (exact) {
+        hipLaunchKernelGGL(bc_x_batch_kernel<true>, gx, dim3(256), 0, s, runs, st, wd, slot);
+        hipLaunchKernelGGL(bc_y_batch_kernel<true>, gy, dim3(256), 0, s, runs, st, wd, slot);
+    } else {
+        hipLaunchKernelGGL(bc_x_batch_kernel<false>, gx, dim3(256), 0, s, runs, st, wd, slot);
+        hipLaunchKernelGGL(bc_y_batch_kernel<false>, gy, dim3(256), 0, s, runs, st, wd, slot);
+    }
+    return hipGetLastError();
+}
+
+int stage_batch_waves_per_cu(bool exact)
+{
+    int a = 0, b = 0;
+    if (exact) {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_batch_kernel<1, true>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_batch_kernel<2, true>, 64, 0);
+    } else {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_batch_kernel<1, false>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_batch_kernel<2, false>, 64, 0);
+    }
+    const int w = a < b ? a : b;
+    return w > 0 ? w : 1;
+}
+
 // 2D launch shape of the interior elementwise kernels: threads along y
 // (contiguous), blocks along x folded over at most 16384 rows.
 static dim3 interior_grid(const Grid &g, int extra)
@@ -817,8 +941,13 @@ hipError_t launch_m_emf(const double *aux_old, const double *aux_now, const doub
 hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *u, Ctrl *ctrl, int slot,
                        const Grid &g, const Phys &ph, hipStream_t s)
 {
-    if (exact) hipLaunchKernelGGL((m_ct_kernel<true>), interior_grid(g, 0), dim3(256), 0, s, uold, E, u, ctrl, slot, g, ph);
-    else hipLaunchKernelGGL((m_ct_kernel<false>), interior_grid(g, 0), dim3(256), 0, s, uold, E, u, ctrl, slot, g, ph);
+    // each thread folds many rows before the one dt atomic per wavefront:
+    // about 8k workgroups in all
+    dim3 grid = interior_grid(g, 0);
+    const unsigned rows = (unsigned)(8192 / grid.x > 0 ? 8192 / grid.x : 1);
+    if (grid.y > rows) grid.y = rows;
+    if (exact) hipLaunchKernelGGL((m_ct_kernel<true>), grid, dim3(256), 0, s, uold, E, u, ctrl, slot, g, ph);
+    else hipLaunchKernelGGL((m_ct_kernel<false>), grid, dim3(256), 0, s, uold, E, u, ctrl, slot, g, ph);
     return hipGetLastError();
 }
 

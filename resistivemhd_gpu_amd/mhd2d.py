@@ -30,6 +30,11 @@ EXPORTS = (
     "mhd_init_harris", "mhd_set_sweep_length", "mhd_get_sweep_length", "mhd_profile_enable",
     "mhd_profile_read", "mhd_set_numerics", "mhd_get_numerics", "mhd_fill_turbulence",
     "mhd_init_turbulence", "mhd_set_scheme", "mhd_get_scheme",
+    "mhd_batch_create", "mhd_batch_destroy", "mhd_batch_last_error", "mhd_batch_size", "mhd_batch_set_state",
+    "mhd_batch_get_state", "mhd_batch_set_eta_field", "mhd_batch_apply_boundary", "mhd_batch_init_orszag_tang",
+    "mhd_batch_init_harris", "mhd_batch_step", "mhd_batch_synchronize", "mhd_batch_get_dt", "mhd_batch_get_time",
+    "mhd_batch_check_crashed", "mhd_batch_set_numerics", "mhd_batch_set_sweep_length",
+    "mhd_batch_get_sweep_length", "mhd_batch_set_graph", "mhd_batch_autotune",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -109,6 +114,26 @@ def load_library(path=LIB_PATH):
         "mhd_init_turbulence": (I, [P, ctypes.c_ulonglong, I, ctypes.c_double, ctypes.c_double]),
         "mhd_set_scheme": (I, [P, I]),
         "mhd_get_scheme": (I, [P, ctypes.POINTER(I)]),
+        "mhd_batch_create": (I, [C, I, ctypes.POINTER(P)]),
+        "mhd_batch_destroy": (I, [P]),
+        "mhd_batch_last_error": (ctypes.c_char_p, [P]),
+        "mhd_batch_size": (I, [P, ctypes.POINTER(I)]),
+        "mhd_batch_set_state": (I, [P, I, D]),
+        "mhd_batch_get_state": (I, [P, I, D]),
+        "mhd_batch_set_eta_field": (I, [P, I, D]),
+        "mhd_batch_apply_boundary": (I, [P]),
+        "mhd_batch_init_orszag_tang": (I, [P, I]),
+        "mhd_batch_init_harris": (I, [P, I, ctypes.c_ulonglong, ctypes.c_double]),
+        "mhd_batch_step": (I, [P, I]),
+        "mhd_batch_synchronize": (I, [P]),
+        "mhd_batch_get_dt": (I, [P, I, D]),
+        "mhd_batch_get_time": (I, [P, I, D]),
+        "mhd_batch_check_crashed": (I, [P, I, ctypes.POINTER(I)]),
+        "mhd_batch_set_numerics": (I, [P, I]),
+        "mhd_batch_set_sweep_length": (I, [P, I]),
+        "mhd_batch_get_sweep_length": (I, [P, ctypes.POINTER(I)]),
+        "mhd_batch_set_graph": (I, [P, I]),
+        "mhd_batch_autotune": (I, [P, I, ctypes.POINTER(I)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -368,3 +393,117 @@ class ResistiveMHD2D:
         v = ctypes.c_int()
         self._ok(self._lib.mhd_get_scheme(self._h, ctypes.byref(v)))
         return "multi" if v.value == MHD_SCHEME_MULTI_SHARE else "single"
+
+
+class ResistiveMHD2DBatch:
+    """Many independent runs advanced together (mhd_batch_* of the C ABI):
+    one launch per RK stage over a work list of every run's items, steps
+    replayed from a hipGraph.  Each run has its own configuration, state,
+    resistivity and dt; per run the result is bitwise that of
+    ResistiveMHD2D with the same numerics."""
+
+    def __init__(self, cfgs):
+        self._lib = load_library()
+        self.cfgs = [MhdConfig.from_buffer_copy(c) for c in cfgs]
+        arr = (MhdConfig * len(self.cfgs))(*self.cfgs)
+        h = ctypes.c_void_p()
+        rc = self._lib.mhd_batch_create(arr, len(self.cfgs), ctypes.byref(h))
+        if rc != 0:
+            msg = self._lib.mhd_batch_last_error(None)
+            raise MHDError(f"libmhd2d error {rc}: {msg.decode() if msg else ''}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mhd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.cfgs)
+
+    def _ok(self, rc):
+        if rc != 0:
+            msg = self._lib.mhd_batch_last_error(self._h)
+            raise MHDError(f"libmhd2d error {rc}: {msg.decode() if msg else ''}")
+
+    def shape(self, run):
+        c = self.cfgs[run]
+        return (c.nx, c.ny, NVAR)
+
+    def setU(self, run, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != self.shape(run):
+            raise ValueError(f"state shape {U.shape} != {self.shape(run)}")
+        self._ok(self._lib.mhd_batch_set_state(self._h, run, _dptr(U)))
+
+    def getU(self, run):
+        out = np.empty(self.shape(run), dtype=np.float64)
+        self._ok(self._lib.mhd_batch_get_state(self._h, run, _dptr(out)))
+        return out
+
+    def setEtaField(self, run, eta):
+        if eta is None:
+            self._ok(self._lib.mhd_batch_set_eta_field(self._h, run, None))
+            return
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        c = self.cfgs[run]
+        if eta.shape != (c.nx, c.ny):
+            raise ValueError(f"eta shape {eta.shape} != {(c.nx, c.ny)}")
+        self._ok(self._lib.mhd_batch_set_eta_field(self._h, run, _dptr(eta)))
+
+    def applyBoundary(self):
+        self._ok(self._lib.mhd_batch_apply_boundary(self._h))
+
+    def initOrszagTang(self, run):
+        self._ok(self._lib.mhd_batch_init_orszag_tang(self._h, run))
+
+    def initHarris(self, run, seed=0, noise_amp=1e-3):
+        self._ok(self._lib.mhd_batch_init_harris(self._h, run, seed, noise_amp))
+
+    def steps(self, n):
+        self._ok(self._lib.mhd_batch_step(self._h, int(n)))
+
+    def synchronize(self):
+        self._ok(self._lib.mhd_batch_synchronize(self._h))
+
+    def dt(self, run):
+        v = ctypes.c_double()
+        self._ok(self._lib.mhd_batch_get_dt(self._h, run, ctypes.byref(v)))
+        return v.value
+
+    def totalTime(self, run):
+        v = ctypes.c_double()
+        self._ok(self._lib.mhd_batch_get_time(self._h, run, ctypes.byref(v)))
+        return v.value
+
+    def checkCalculationIsCrashed(self, run):
+        c = ctypes.c_int()
+        self._ok(self._lib.mhd_batch_check_crashed(self._h, run, ctypes.byref(c)))
+        return bool(c.value)
+
+    def set_numerics(self, mode):
+        if isinstance(mode, str):
+            mode = {"fast": MHD_NUMERICS_FAST, "exact": MHD_NUMERICS_EXACT}[mode]
+        self._ok(self._lib.mhd_batch_set_numerics(self._h, int(mode)))
+
+    def set_sweep_length(self, columns):
+        self._ok(self._lib.mhd_batch_set_sweep_length(self._h, int(columns)))
+
+    def sweep_length(self):
+        v = ctypes.c_int()
+        self._ok(self._lib.mhd_batch_get_sweep_length(self._h, ctypes.byref(v)))
+        return v.value
+
+    def set_graph(self, on=True):
+        self._ok(self._lib.mhd_batch_set_graph(self._h, 1 if on else 0))
+
+    def autotune(self, steps=2):
+        v = ctypes.c_int()
+        self._ok(self._lib.mhd_batch_autotune(self._h, int(steps), ctypes.byref(v)))
+        return v.value
