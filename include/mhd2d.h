@@ -247,6 +247,44 @@ int mhd_batch_set_graph(mhd_batch *b, int on);
  * it had not been called. */
 int mhd_batch_autotune(mhd_batch *b, int steps, int *columns);
 
+/* ---------------------------------------------------------------- adjoint
+ * Discrete adjoint of the MHD_SCHEME_SINGLE step (no counterpart in the
+ * reference, which has no differentiation), for a batch of runs as in
+ * mhd_batch_*.  A rollout U_N = S_{N-1}(... S_0(BC(U0))) is run by
+ * mhd_adjoint_forward, which records every step's dt and keeps a checkpoint
+ * every `stride` steps (0: ceil(sqrt(nsteps))).  mhd_adjoint_backward takes
+ * the seed dL/dU_N of each run and returns dL/dU0 (of the state passed to
+ * mhd_adjoint_set_state, before the boundary fill) and dL/deta (per cell,
+ * ghosts included), with every step's dt held at its recorded value
+ * ("frozen dt").  With frozen_dt = 1 the forward replays dt values set by
+ * mhd_adjoint_set_dts instead of the CFL ones (for finite-difference checks
+ * with the same dt sequence).  Exact numerics only; periodic x with periodic
+ * or symmetric y (MHD_ERR_INVALID otherwise).  States use the AoS layout of
+ * mhd_set_state. */
+typedef struct mhd_adjoint mhd_adjoint;
+int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out);
+int mhd_adjoint_destroy(mhd_adjoint *a);
+const char *mhd_adjoint_last_error(const mhd_adjoint *a);   /* a may be NULL */
+int mhd_adjoint_set_eta_field(mhd_adjoint *a, int run, const double *host_eta);   /* NULL: uniform cfg.eta */
+int mhd_adjoint_set_state(mhd_adjoint *a, int run, const double *host_aos);       /* U0 */
+int mhd_adjoint_set_dts(mhd_adjoint *a, int run, const double *host_dts, int nsteps);
+int mhd_adjoint_get_dts(mhd_adjoint *a, int run, double *host_dts, int nsteps);
+int mhd_adjoint_forward(mhd_adjoint *a, int nsteps, int stride, int frozen_dt);
+int mhd_adjoint_get_state(mhd_adjoint *a, int run, double *host_aos);   /* U_N, valid until the backward pass */
+int mhd_adjoint_set_seed(mhd_adjoint *a, int run, const double *host_aos);   /* dL/dU_N */
+int mhd_adjoint_backward(mhd_adjoint *a);
+int mhd_adjoint_get_grad_state(mhd_adjoint *a, int run, double *host_aos);   /* dL/dU0 */
+int mhd_adjoint_get_grad_eta(mhd_adjoint *a, int run, double *host_eta);     /* dL/deta */
+/* Tangent-linear model of the same rollout (the FP64 reference the adjoint
+ * is checked against): with the direction (dU0, deta) of each run (NULL: 0),
+ * mhd_adjoint_tangent replays nsteps steps from U0 with the recorded dt and
+ * carries dU along; mhd_adjoint_get_tangent returns dU_N.  For any seed W,
+ * sum(W * dU_N) = sum(dL/dU0 * dU0) + sum(dL/deta * deta). */
+int mhd_adjoint_set_direction(mhd_adjoint *a, int run, const double *host_dU0, const double *host_deta);
+int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps);
+int mhd_adjoint_get_tangent(mhd_adjoint *a, int run, double *host_dUN);
+int mhd_adjoint_synchronize(mhd_adjoint *a);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmhd2d.so")
 
-SOURCES = ["mhd2d_kernels.hip", "mhd2d_capi.hip", "mhd2d_batch.hip", "mhd2d_setups.cpp"]
+SOURCES = ["mhd2d_kernels.hip", "mhd2d_capi.hip", "mhd2d_batch.hip", "mhd2d_adjoint.hip", "mhd2d_setups.cpp"]
 
 # -ffp-contract=off: no fused multiply-add contraction, so each operation
 # rounds as written in the reference's expressions.

@@ -1,0 +1,294 @@
+#!/usr/bin/env python3
+"""Reverse- and forward-mode source transformation of the single-assignment
+flux listings (*.ssa) into C++ (mhd2d_face_ad.h).
+
+Each listing statement `x = expr` (Python expression syntax, which parses
++ - * / and unary minus with C's precedence and left associativity) is
+lowered to three-address operations in the expression's own evaluation
+order, so the emitted primal rounds exactly as the written C expression
+does.  For every listing three functions are emitted:
+
+  NAME(in, par, out)                  primal
+  NAME_tlm(in, in_d, par, out, out_d) tangent-linear (one direction)
+  NAME_adj(in, par, out_b, in_b)      adjoint: in_b = (d out / d in)^T out_b
+
+Elementary operations and their derivatives:
+  a+b, a-b, a*b, a/b, -a       the usual rules
+  sqrt(a)                      1/(2 sqrt a), taken as 0 where sqrt a = 0
+  fabs(a)                      sign(a) (0 at 0)
+  tmin(a, b) = b < a ? b : a   routes to the selected argument
+  tmax(a, b) = a < b ? b : a   routes to the selected argument
+  sgn(a), ind(condition)       piecewise constant: derivative 0
+  minmod(a, b)                 expanded as s = sgn(a); s * tmax(tmin(|a|, s b), eps)
+The branch decisions (minmod, tmin/tmax, the HLLD region indicators) are
+taken from the primal values, which the adjoint recomputes first, so the
+adjoint differentiates exactly the branch the forward step took.
+
+Usage: gen_face_ad.py OUT.h LISTING.ssa[:NAME] ...
+"""
+import ast
+import os
+import sys
+
+
+class Lowering:
+    def __init__(self, params):
+        self.params = set(params)
+        self.ops = []          # (dst, op, args) ; args are names or float literals
+        self.ntmp = 0
+
+    def tmp(self):
+        self.ntmp += 1
+        return f"t{self.ntmp}"
+
+    def operand(self, node):
+        if isinstance(node, ast.Constant):
+            return repr(float(node.value))
+        if isinstance(node, ast.Name):
+            return node.id
+        return self.expr(node)
+
+    def emit(self, op, args, dst=None):
+        dst = dst or self.tmp()
+        self.ops.append((dst, op, args))
+        return dst
+
+    def expr(self, node, dst=None):
+        if isinstance(node, ast.BinOp):
+            a = self.operand(node.left)
+            b = self.operand(node.right)
+            op = {ast.Add: "add", ast.Sub: "sub", ast.Mult: "mul", ast.Div: "div"}[type(node.op)]
+            return self.emit(op, [a, b], dst)
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+            return self.emit("neg", [self.operand(node.operand)], dst)
+        if isinstance(node, ast.Call):
+            fn = node.func.id
+            if fn == "ind":
+                return self.emit("ind", [self.cond(node.args[0])], dst)
+            if fn == "minmod":
+                a = self.operand(node.args[0])
+                b = self.operand(node.args[1])
+                s = self.emit("sgn", [a])
+                ax = self.emit("fabs", [a])
+                sy = self.emit("mul", [s, b])
+                m = self.emit("tmin", [ax, sy])
+                mm = self.emit("tmax", [m, "eps"])
+                return self.emit("mul", [s, mm], dst)
+            args = [self.operand(x) for x in node.args]
+            return self.emit(fn, args, dst)
+        if isinstance(node, (ast.Name, ast.Constant)):
+            return self.emit("copy", [self.operand(node)], dst)
+        raise ValueError(f"unsupported expression {ast.dump(node)}")
+
+    def cond(self, node):
+        if isinstance(node, ast.BoolOp):
+            j = " && " if isinstance(node.op, ast.And) else " || "
+            return "(" + j.join(self.cond(v) for v in node.values) + ")"
+        if isinstance(node, ast.Compare) and len(node.ops) == 1:
+            sym = {ast.Lt: "<", ast.LtE: "<=", ast.Gt: ">", ast.GtE: ">="}[type(node.ops[0])]
+            return f"({self.operand(node.left)} {sym} {self.operand(node.comparators[0])})"
+        raise ValueError(f"unsupported condition {ast.dump(node)}")
+
+
+def parse(path):
+    params, inputs, outputs, stmts = [], [], [], []
+    for line in open(path):
+        line = line.split("#", 1)[0].strip()
+        if not line:
+            continue
+        if line.startswith("params:"):
+            params = line.split(":", 1)[1].split()
+        elif line.startswith("inputs:"):
+            inputs = line.split(":", 1)[1].split()
+        elif line.startswith("outputs:"):
+            outputs = line.split(":", 1)[1].split()
+        else:
+            stmts.append(ast.parse(line).body[0])
+    low = Lowering(params)
+    for st in stmts:
+        low.expr(st.value, dst=st.targets[0].id)
+    return params, inputs, outputs, low.ops
+
+
+C_OP = {"add": "{0} + {1}", "sub": "{0} - {1}", "mul": "{0} * {1}", "div": "{0} / {1}", "neg": "-{0}",
+        "copy": "{0}", "sqrt": "sqrt({0})", "fabs": "fabs({0})", "tmin": "(({1}) < ({0}) ? ({1}) : ({0}))",
+        "tmax": "(({0}) < ({1}) ? ({1}) : ({0}))", "sgn": "(double)(({0} > 0.0) - ({0} < 0.0))",
+        "ind": "(double){0}"}
+
+
+def is_lit(a):
+    return a[0].isdigit() or a[0] == "-"
+
+
+def primal_lines(params, inputs, ops):
+    out = [f"    const double {p} = par[{i}];" for i, p in enumerate(params)]
+    out += [f"    const double {v} = in[{i}];" for i, v in enumerate(inputs)]
+    for dst, op, args in ops:
+        out.append(f"    const double {dst} = {C_OP[op].format(*args)};")
+    return out
+
+
+def tangent_lines(params, inputs, ops):
+    cst = set(params)
+    d = lambda a: "0.0" if (is_lit(a) or a in cst) else f"{a}_d"
+    out = []
+    for i, v in enumerate(inputs):
+        out.append(f"    const double {v}_d = in_d[{i}];")
+    for dst, op, args in ops:
+        a = args[0] if args else None
+        b = args[1] if len(args) > 1 else None
+        if op == "add":
+            e = f"{d(a)} + {d(b)}"
+        elif op == "sub":
+            e = f"{d(a)} - {d(b)}"
+        elif op == "mul":
+            e = f"{d(a)} * {b} + {a} * {d(b)}"
+        elif op == "div":
+            e = f"({d(a)} - {dst} * {d(b)}) / {b}"
+        elif op == "neg":
+            e = f"-{d(a)}"
+        elif op == "copy":
+            e = d(a)
+        elif op == "sqrt":
+            e = f"({dst} > 0.0 ? 0.5 * {d(a)} / {dst} : 0.0)"
+        elif op == "fabs":
+            e = f"({a} > 0.0 ? {d(a)} : ({a} < 0.0 ? -{d(a)} : 0.0))"
+        elif op == "tmin":
+            e = f"(({b}) < ({a}) ? {d(b)} : {d(a)})"
+        elif op == "tmax":
+            e = f"(({a}) < ({b}) ? {d(b)} : {d(a)})"
+        elif op in ("sgn", "ind"):
+            e = "0.0"
+        else:
+            raise ValueError(op)
+        out.append(f"    const double {dst}_d = {e};")
+    return out
+
+
+def adjoint_lines(params, inputs, outputs, ops):
+    cst = set(params)
+    active = lambda a: not (is_lit(a) or a in cst)
+    out = []
+    names = list(inputs) + [dst for dst, _, _ in ops]
+    for n in names:
+        out.append(f"    double {n}_b = 0.0;")
+    for i, v in enumerate(outputs):
+        out.append(f"    {v}_b += out_b[{i}];")
+    for dst, op, args in reversed(ops):
+        a = args[0] if args else None
+        b = args[1] if len(args) > 1 else None
+        t = f"{dst}_b"
+        acc = []
+        if op == "add":
+            acc = [(a, t), (b, t)]
+        elif op == "sub":
+            acc = [(a, t), (b, f"-{t}")]
+        elif op == "mul":
+            acc = [(a, f"{t} * {b}"), (b, f"{t} * {a}")]
+        elif op == "div":
+            acc = [(a, f"{t} / {b}"), (b, f"-{t} * {dst} / {b}")]
+        elif op == "neg":
+            acc = [(a, f"-{t}")]
+        elif op == "copy":
+            acc = [(a, t)]
+        elif op == "sqrt":
+            acc = [(a, f"({dst} > 0.0 ? 0.5 * {t} / {dst} : 0.0)")]
+        elif op == "fabs":
+            acc = [(a, f"({a} > 0.0 ? {t} : ({a} < 0.0 ? -{t} : 0.0))")]
+        elif op == "tmin":
+            if active(a) and active(b):
+                out.append(f"    if (({b}) < ({a})) {b}_b += {t}; else {a}_b += {t};")
+            elif active(a):
+                out.append(f"    if (!(({b}) < ({a}))) {a}_b += {t};")
+            elif active(b):
+                out.append(f"    if (({b}) < ({a})) {b}_b += {t};")
+            continue
+        elif op == "tmax":
+            if active(a) and active(b):
+                out.append(f"    if (({a}) < ({b})) {b}_b += {t}; else {a}_b += {t};")
+            elif active(a):
+                out.append(f"    if (!(({a}) < ({b}))) {a}_b += {t};")
+            elif active(b):
+                out.append(f"    if (({a}) < ({b})) {b}_b += {t};")
+            continue
+        elif op in ("sgn", "ind"):
+            continue
+        else:
+            raise ValueError(op)
+        for v, e in acc:
+            if active(v):
+                out.append(f"    {v}_b += {e};")
+    for i, v in enumerate(inputs):
+        out.append(f"    in_b[{i}] = {v}_b;")
+    return out
+
+
+def emit(name, path):
+    params, inputs, outputs, ops = parse(path)
+    src = os.path.basename(path)
+    n_in, n_out, n_par = len(inputs), len(outputs), len(params)
+    lines = [f"// ---- {name}: from {src}",
+             f"// in[{n_in}]: {' '.join(inputs)}",
+             f"// par[{n_par}]: {' '.join(params)}",
+             f"// out[{n_out}]: {' '.join(outputs)}",
+             f"constexpr int {name}_nin = {n_in};",
+             f"constexpr int {name}_nout = {n_out};",
+             f"MHD_HD inline void {name}(const double *in, const double *par, double *out)", "{"]
+    lines += primal_lines(params, inputs, ops)
+    lines += [f"    out[{i}] = {v};" for i, v in enumerate(outputs)]
+    lines += ["}", "",
+              f"MHD_HD inline void {name}_tlm(const double *in, const double *in_d, const double *par, double *out,"
+              f" double *out_d)", "{"]
+    lines += primal_lines(params, inputs, ops)
+    lines += tangent_lines(params, inputs, ops)
+    lines += [f"    out[{i}] = {v};" for i, v in enumerate(outputs)]
+    lines += [f"    out_d[{i}] = {v}_d;" for i, v in enumerate(outputs)]
+    lines += ["}", "",
+              f"MHD_HD inline void {name}_adj(const double *in, const double *par, const double *out_b,"
+              f" double *in_b)", "{"]
+    lines += primal_lines(params, inputs, ops)
+    lines += adjoint_lines(params, inputs, outputs, ops)
+    lines += ["}", ""]
+    return lines
+
+
+def main(argv):
+    out_path = argv[1]
+    body = ["// mhd2d_face_ad.h -- GENERATED by csrc/adjoint/gen_face_ad.py from the *.ssa",
+            "// listings beside it; do not edit.  Primal, tangent-linear and adjoint of",
+            "// the per-face flux pieces of the RK step (see the listings for the",
+            "// reference lines each follows).",
+            "#pragma once",
+            "#include <math.h>",
+            "#if defined(__HIPCC__)",
+            "#define MHD_HD __host__ __device__",
+            "#else",
+            "#define MHD_HD",
+            "#endif",
+            "",
+            "#if defined(__clang__)",
+            "#pragma clang diagnostic push",
+            "#pragma clang diagnostic ignored \"-Wunused-variable\"",
+            "#pragma clang diagnostic ignored \"-Wunused-but-set-variable\"",
+            "#elif defined(__GNUC__)",
+            "#pragma GCC diagnostic push",
+            "#pragma GCC diagnostic ignored \"-Wunused-variable\"",
+            "#pragma GCC diagnostic ignored \"-Wunused-but-set-variable\"",
+            "#endif",
+            "",
+            "namespace mhd_ad {",
+            ""]
+    for spec in argv[2:]:
+        path, _, name = spec.partition(":")
+        name = name or os.path.splitext(os.path.basename(path))[0]
+        body += emit(name, path)
+    body += ["}  // namespace mhd_ad", "",
+             "#if defined(__clang__)", "#pragma clang diagnostic pop",
+             "#elif defined(__GNUC__)", "#pragma GCC diagnostic pop", "#endif", ""]
+    text = "\n".join(body)
+    if not os.path.exists(out_path) or open(out_path).read() != text:
+        open(out_path, "w").write(text)
+
+
+if __name__ == "__main__":
+    main(sys.argv)

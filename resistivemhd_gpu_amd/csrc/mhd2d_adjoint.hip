@@ -1,0 +1,1046 @@
+// mhd2d_adjoint.hip -- discrete adjoint of the single-GPU library's RK2 step
+// (ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_periodicXY/
+// resistiveMHD_2D.cu:133-195) for a batch of runs, with dt frozen at the
+// values the forward pass used, and sqrt(N) checkpointing (mhd_adjoint_* in
+// include/mhd2d.h).
+//
+// Forward: the batched sweep kernels advance every run, recording each
+// step's dt and keeping the state every `stride` steps.  Backward, segment by
+// segment from the last: the segment's states are recomputed from its
+// checkpoint with the recorded dt (bitwise the forward's), then each step is
+// reversed.  The adjoint of one step, with lam the adjoint of U^{n+1}:
+//
+//   U^{n+1} = BC(U2), U2_nonB = (U + U* - dt dF(U*) - dt dG(U*)) / 2,
+//   U2_B = B* ; U* = BC(U1), U1_nonB = U - dt dF(U) - dt dG(U),
+//   U1_B = B* = B - dt curl E(F(U), G(U))
+//
+//   1. lam <- BC^T lam
+//   2. stage-2 faces: lamF from lam, then the face adjoint on the centred U*
+//      (flux_full_adj: MUSCL, HLLD, eta J) scattered to lamC and g_eta
+//   3. lamBar = shift^T lamC + lam_nonB / 2 ; lamU_nonB = lam_nonB / 2
+//   4. lamBar <- BC^T lamBar
+//   5. lamB* = lamBar_B + lam_B ; lamE = -dt curl^T lamB* ; lamU_B = lamB* ;
+//      lamU_nonB += lamBar_nonB
+//   6. stage-1 faces: lamF from lamBar and lamE, face adjoint on the centred U
+//   7. lam = lamU + shift^T lamC  (the adjoint of U^n)
+//
+// The face pieces and their branch decisions (minmod, HLLD region, min/max)
+// come from the primal recomputed in exact arithmetic, so the adjoint
+// differentiates the branch the forward step took.  Contributions of faces
+// to cells are summed with hardware FP64 atomics (the gradient's last bits
+// depend on their order).  Boundaries: periodic x with periodic or symmetric
+// y (the symmetric-x fill reads a stale CT face value and is not
+// differentiated).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mhd2d.h"
+#include "mhd2d_adjoint_face.h"
+#include "mhd2d_batch_impl.h"
+#include "mhd2d_kernels.h"
+
+using mhd::Ctrl;
+using mhd::Grid;
+using mhd::Phys;
+
+namespace {
+
+// One run of the adjoint batch.
+struct AdjDesc {
+    Grid g;
+    Phys ph;
+    int bc_y;
+    int ntj;                      // 64-row tiles per column
+    double *u, *us;               // U^n and U* (the forward batch's buffers)
+    const double *eta;            // per-cell resistivity or nullptr (uniform ph.eta)
+    double *lam, *lamBar, *lamU, *lamC, *lamE, *geta;
+    double *u0, *hist, *ckpt;     // initial state, segment states, checkpoints
+    double *dts;                  // dt of every step of the rollout
+    // tangent-linear model: direction (dU0, deta), tangent state dU, dU*,
+    // tangent face fluxes dF, dG and corner EMF dE
+    double *du0, *deta, *du, *dus, *dF, *dG, *dE;
+};
+
+// ---------------------------------------------------------------- kernels
+// Work list: run r owns tiles [first[r], first[r+1]); a tile is 64
+// consecutive rows of one column, one wavefront.
+__device__ __forceinline__ bool tile_cell(const long long *first, int nruns, int &r, int &i, int &j,
+                                          const AdjDesc *runs)
+{
+    const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= first[nruns]) return false;
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    r = lo;
+    const long long lt = t - first[lo];
+    const int ntj = runs[lo].ntj;
+    i = (int)(lt / ntj);
+    j = (int)(lt % ntj) * 64 + (int)(threadIdx.x & 63);
+    return true;
+}
+
+struct Soa {
+    const double *u;
+    int pitch;
+    long long plane;
+    __device__ double operator()(int v, int i, int j) const { return u[v * plane + (size_t)i * pitch + j]; }
+};
+
+struct EtaAcc {
+    const double *e;
+    int pitch;
+    double uniform;
+    __device__ double operator()(int i, int j) const { return e ? e[(size_t)i * pitch + j] : uniform; }
+};
+
+struct AtomicAdd {
+    double *lamC, *geta;
+    int pitch;
+    long long plane;
+    __device__ void operator()(int i, int j, int v, double b) const
+    {
+        if (v == 8) unsafeAtomicAdd(geta + (size_t)i * pitch + j, b);
+        else unsafeAtomicAdd(lamC + v * plane + (size_t)i * pitch + j, b);
+    }
+};
+
+enum Buf { kU0 = 0, kU = 1, kHist = 2, kCkpt = 3, kLam = 4, kLamBar = 5, kLamU = 6, kLamC = 7 };
+
+__device__ __forceinline__ double *buf_ptr(const AdjDesc &d, int which, int slot)
+{
+    const size_t s8 = (size_t)8 * d.g.plane;
+    switch (which) {
+    case kU0: return d.u0;
+    case kU: return d.u;
+    case kHist: return d.hist + slot * s8;
+    case kCkpt: return d.ckpt + slot * s8;
+    case kLam: return d.lam;
+    case kLamBar: return d.lamBar;
+    case kLamU: return d.lamU;
+    default: return d.lamC;
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(const AdjDesc *runs, const long long *first, int nruns, int src,
+                                                   int src_slot, int dst, int dst_slot)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    if (j >= d.g.ny) return;
+    const double *a = buf_ptr(d, src, src_slot);
+    double *b = buf_ptr(d, dst, dst_slot);
+    const size_t o = (size_t)i * d.g.pitch + j;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) b[v * d.g.plane + o] = a[v * d.g.plane + o];
+}
+
+// mask bits: 1 lamC, 2 geta
+__global__ __launch_bounds__(256) void zero_kernel(const AdjDesc *runs, const long long *first, int nruns, int mask)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    if (j >= d.g.ny) return;
+    const size_t o = (size_t)i * d.g.pitch + j;
+    if (mask & 1) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) d.lamC[v * d.g.plane + o] = 0.0;
+    }
+    if (mask & 2) d.geta[o] = 0.0;
+}
+
+// Transpose of the y boundary fill (periodic: P/boundary.cu:74-141;
+// symmetric: S/boundary.cu:107-205), one thread per column; blockIdx.y = run.
+__global__ __launch_bounds__(256) void bct_y_kernel(const AdjDesc *runs, int which)
+{
+    const AdjDesc &d = runs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.g.nx) return;
+    double *L = buf_ptr(d, which, 0);
+    const long long pl = d.g.plane;
+    const int ny = d.g.ny;
+    const size_t row = (size_t)i * d.g.pitch;
+    const int dst[6] = {0, 1, 2, ny - 3, ny - 2, ny - 1};
+    if (d.bc_y == MHD_BC_PERIODIC) {
+        const int src[6] = {ny - 6, ny - 5, ny - 4, 3, 4, 5};
+        for (int v = 0; v < 8; ++v)
+            for (int q = 0; q < 6; ++q) {
+                L[v * pl + row + src[q]] += L[v * pl + row + dst[q]];
+                L[v * pl + row + dst[q]] = 0.0;
+            }
+        return;
+    }
+    const int src[6] = {5, 4, 3, ny - 4, ny - 5, ny - 6};
+    for (int v = 0; v < 8; ++v) {
+        if (v == 5) continue;
+        for (int q = 0; q < 6; ++q) {
+            const double b = L[v * pl + row + dst[q]];
+            L[v * pl + row + dst[q]] = 0.0;
+            if (v == 4) unsafeAtomicAdd(&L[4 * pl + row + src[q]], b);   // other columns add to Bx too
+            else L[v * pl + row + src[q]] += b;
+        }
+    }
+    double *by = L + 5 * pl + row;
+    const double c = d.ph.dy / d.ph.dx;
+    double lb0 = by[0], lb1 = by[1], lb2 = by[2];
+    double lt0 = by[ny - 3], lt1 = by[ny - 2], lt2 = by[ny - 1];
+    by[0] = by[1] = by[2] = 0.0;
+    by[ny - 3] = by[ny - 2] = by[ny - 1] = 0.0;
+    // bottom: by2 = by3 + c d3, by1 = by2 + c d3, by0 = by1 + c d4
+    lb1 += lb0;
+    const double ld4 = c * lb0;
+    lb2 += lb1;
+    double ld3 = c * lb1 + c * lb2;
+    by[3] += lb2;
+    // top: by(ny-3) = by(ny-4) - c dA, by(ny-2) = .. - c dB, by(ny-1) = .. - c dC
+    lt1 += lt2;
+    const double ldC = -c * lt2;
+    lt0 += lt1;
+    const double ldB = -c * lt1;
+    by[ny - 4] += lt0;
+    const double ldA = -c * lt0;
+    if (i > 0) {
+        // d_r = Bx(i, r) - Bx(i-1, r)
+        double *bx = L + 4 * pl;
+        const size_t prev = (size_t)(i - 1) * d.g.pitch;
+        const int rows[5] = {3, 4, ny - 4, ny - 5, ny - 6};
+        const double lds[5] = {ld3, ld4, ldA, ldB, ldC};
+        for (int q = 0; q < 5; ++q) {
+            unsafeAtomicAdd(&bx[row + rows[q]], lds[q]);
+            unsafeAtomicAdd(&bx[prev + rows[q]], -lds[q]);
+        }
+    }
+}
+
+// Transpose of the periodic x fill (P/boundary.cu:4-71), one thread per row.
+__global__ __launch_bounds__(256) void bct_x_kernel(const AdjDesc *runs, int which)
+{
+    const AdjDesc &d = runs[blockIdx.y];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.g.ny) return;
+    double *L = buf_ptr(d, which, 0);
+    const long long pl = d.g.plane;
+    const int nx = d.g.nx;
+    const size_t P = d.g.pitch;
+    const int dst[6] = {0, 1, 2, nx - 3, nx - 2, nx - 1};
+    const int src[6] = {nx - 6, nx - 5, nx - 4, 3, 4, 5};
+    for (int v = 0; v < 8; ++v)
+        for (int q = 0; q < 6; ++q) {
+            L[v * pl + src[q] * P + j] += L[v * pl + dst[q] * P + j];
+            L[v * pl + dst[q] * P + j] = 0.0;
+        }
+}
+
+__device__ __forceinline__ bool non_b(int v) { return v != 4 && v != 5; }
+
+// Face adjoints of one stage and direction: face (i, j) sits between cell
+// (i, j) and (i+1, j) (DIR 0) or (i, j+1) (DIR 1).
+template <int STAGE, int DIR>
+__global__ __launch_bounds__(256) void face_adj_kernel(const AdjDesc *runs, const long long *first, int nruns,
+                                                       int step)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    const Grid &g = d.g;
+    if (DIR == 0 ? (i < 2 || i > g.nx - 4 || j < 2 || j > g.ny - 3) : (i < 2 || i > g.nx - 3 || j < 2 || j > g.ny - 4))
+        return;
+    const double dt = d.dts[step];
+    const double coef = (STAGE == 2 ? 0.5 : 1.0) * dt / (DIR == 0 ? d.ph.dx : d.ph.dy);
+    const double *L = STAGE == 2 ? d.lam : d.lamBar;
+    const size_t o = (size_t)i * g.pitch + j;
+    const size_t on = DIR == 0 ? o + g.pitch : o + 1;
+    double lamF[8];
+    bool any = false;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        lamF[v] = non_b(v) ? coef * (L[v * g.plane + on] - L[v * g.plane + o]) : 0.0;
+        any |= lamF[v] != 0.0;
+    }
+    if (STAGE == 1) {
+        // E(i, j) = (G4(i, j) + G4(i+1, j) - F5(i, j) - F5(i, j+1)) / 4
+        if (DIR == 0) lamF[5] = -0.25 * (d.lamE[o] + d.lamE[o - 1]);
+        else lamF[4] = 0.25 * (d.lamE[o] + d.lamE[o - g.pitch]);
+        any |= lamF[4] != 0.0 || lamF[5] != 0.0;
+    }
+    if (!any) return;
+    const Soa S{STAGE == 2 ? d.us : d.u, g.pitch, g.plane};
+    const EtaAcc E{d.eta, g.pitch, d.ph.eta};
+    AtomicAdd add{d.lamC, d.geta, g.pitch, g.plane};
+    const double par[3] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps};
+    const double rpar[2] = {d.ph.dx, d.ph.dy};
+    mhd_ad::flux_full_adj(S, E, DIR, i, j, par, rpar, lamF, add);
+}
+
+// Adjoint of the face-to-centre averaging (shiftUToCenterForCT) at cell
+// (i, j): component v of the stored state.
+__device__ __forceinline__ double shift_t(const AdjDesc &d, int v, int i, int j)
+{
+    const Grid &g = d.g;
+    const double *C = d.lamC + v * g.plane;
+    const size_t o = (size_t)i * g.pitch + j;
+    if (v == 4) return (i >= 1 ? 0.5 * C[o] : C[o]) + (i + 1 < g.nx ? 0.5 * C[o + g.pitch] : 0.0);
+    if (v == 5) return (j >= 1 ? 0.5 * C[o] : C[o]) + (j + 1 < g.ny ? 0.5 * C[o + 1] : 0.0);
+    return C[o];
+}
+
+// step 3: lamBar = shift^T lamC + lam_nonB / 2 ; lamU_nonB = lam_nonB / 2
+__global__ __launch_bounds__(256) void cell2_kernel(const AdjDesc *runs, const long long *first, int nruns)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    if (j >= d.g.ny) return;
+    const size_t o = (size_t)i * d.g.pitch + j;
+    for (int v = 0; v < 8; ++v) {
+        const double half = non_b(v) ? 0.5 * d.lam[v * d.g.plane + o] : 0.0;
+        d.lamBar[v * d.g.plane + o] = shift_t(d, v, i, j) + half;
+        d.lamU[v * d.g.plane + o] = half;
+    }
+}
+
+// step 5: CT adjoint.  B* = B - dt curl E for cells i >= 1, j >= 1
+// (P/ct.cu:44-59): bX* = bX - dt/dy (E(i,j) - E(i,j-1)),
+// bY* = bY + dt/dx (E(i,j) - E(i-1,j)).
+__device__ __forceinline__ double lam_bstar(const AdjDesc &d, int v, int i, int j)
+{
+    if (i < 1 || j < 1 || i >= d.g.nx || j >= d.g.ny) return 0.0;
+    const size_t o = v * d.g.plane + (size_t)i * d.g.pitch + j;
+    return d.lamBar[o] + d.lam[o];
+}
+
+__global__ __launch_bounds__(256) void ct_adj_kernel(const AdjDesc *runs, const long long *first, int nruns, int step)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    const Grid &g = d.g;
+    if (j >= g.ny) return;
+    const size_t o = (size_t)i * g.pitch + j;
+    const double dt = d.dts[step];
+    const double cy = dt / d.ph.dy, cx = dt / d.ph.dx;
+    double le = 0.0;
+    if (i < g.nx - 1 && j < g.ny - 1)
+        le = -cy * lam_bstar(d, 4, i, j) + cy * lam_bstar(d, 4, i, j + 1) + cx * lam_bstar(d, 5, i, j)
+           - cx * lam_bstar(d, 5, i + 1, j);
+    d.lamE[o] = le;
+    d.lamU[4 * g.plane + o] = lam_bstar(d, 4, i, j);
+    d.lamU[5 * g.plane + o] = lam_bstar(d, 5, i, j);
+    for (int v = 0; v < 8; ++v)
+        if (non_b(v)) d.lamU[v * g.plane + o] += d.lamBar[v * g.plane + o];
+}
+
+// step 7: lam = lamU + shift^T lamC
+__global__ __launch_bounds__(256) void cell1_kernel(const AdjDesc *runs, const long long *first, int nruns)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    if (j >= d.g.ny) return;
+    const size_t o = (size_t)i * d.g.pitch + j;
+    for (int v = 0; v < 8; ++v) d.lam[v * d.g.plane + o] = d.lamU[v * d.g.plane + o] + shift_t(d, v, i, j);
+}
+
+// Sets every run's dt replay: dt_in = dts (or nullptr), dt_out = dts (or
+// nullptr), and seq0 so that the next step reads/writes entry idx0.
+__global__ void ctrl_seq_kernel(const AdjDesc *runs, Ctrl *ctrl, int nruns, int replay, int record, long long idx0)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    Ctrl &c = ctrl[r];
+    c.dt_in = replay ? runs[r].dts : nullptr;
+    c.dt_out = record ? runs[r].dts : nullptr;
+    c.seq0 = c.steps - idx0;
+}
+
+__global__ void ctrl_clear_kernel(Ctrl *ctrl, int nruns)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nruns) return;
+    ctrl[r].dt_in = nullptr;
+    ctrl[r].dt_out = nullptr;
+    ctrl[r].time = 0.0;
+}
+
+__global__ __launch_bounds__(256) void pack_plane_kernel(const double *__restrict__ src, double *__restrict__ dst,
+                                                         Grid g)
+{
+    const long long total = (long long)g.nx * g.ny;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / g.ny), j = (int)(k % g.ny);
+        dst[k] = src[(size_t)i * g.pitch + j];
+    }
+}
+
+
+// ------------------------------------------------- tangent-linear model
+// dF, dG of the faces the step reads (zero elsewhere), from the primal state
+// (U or U*) and its tangent, with the tangent of eta.
+template <int DIR>
+__global__ __launch_bounds__(256) void face_tlm_kernel(const AdjDesc *runs, const long long *first, int nruns,
+                                                       int stage)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    const Grid &g = d.g;
+    if (j >= g.ny) return;
+    const size_t o = (size_t)i * g.pitch + j;
+    double *out = DIR == 0 ? d.dF : d.dG;
+    const bool inside = DIR == 0 ? (i >= 2 && i <= g.nx - 4 && j >= 2 && j <= g.ny - 3)
+                                 : (i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 4);
+    if (!inside) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) out[v * g.plane + o] = 0.0;
+        return;
+    }
+    const Soa S{stage == 2 ? d.us : d.u, g.pitch, g.plane};
+    const Soa dS{stage == 2 ? d.dus : d.du, g.pitch, g.plane};
+    const EtaAcc E{d.eta, g.pitch, d.ph.eta};
+    const EtaAcc dE{d.deta, g.pitch, 0.0};
+    const double par[3] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps};
+    const double rpar[2] = {d.ph.dx, d.ph.dy};
+    double F[8], dF[8];
+    mhd_ad::flux_full_tlm(S, dS, E, dE, DIR, i, j, par, rpar, F, dF);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) out[v * g.plane + o] = dF[v];
+}
+
+// Tangent of the corner EMF E(i, j) = (G4(i,j) + G4(i+1,j) - F5(i,j) - F5(i,j+1)) / 4.
+__global__ __launch_bounds__(256) void emf_tlm_kernel(const AdjDesc *runs, const long long *first, int nruns)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    const Grid &g = d.g;
+    if (j >= g.ny) return;
+    const size_t o = (size_t)i * g.pitch + j;
+    double e = 0.0;
+    if (i < g.nx - 1 && j < g.ny - 1)
+        e = 0.25 * (d.dG[4 * g.plane + o] + d.dG[4 * g.plane + o + g.pitch] - d.dF[5 * g.plane + o]
+                    - d.dF[5 * g.plane + o + 1]);
+    d.dE[o] = e;
+}
+
+// Tangent of a stage update of the interior cells (ghosts are refilled by
+// the boundary): stage 1 writes dU* (B by CT), stage 2 writes dU (B = dB*).
+__global__ __launch_bounds__(256) void stage_tlm_kernel(const AdjDesc *runs, const long long *first, int nruns,
+                                                        int stage, int step)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    const Grid &g = d.g;
+    if (i < 3 || i > g.nx - 4 || j < 3 || j > g.ny - 4) return;
+    const size_t o = (size_t)i * g.pitch + j;
+    const double dt = d.dts[step];
+    const double dtdx = dt / d.ph.dx, dtdy = dt / d.ph.dy;
+    for (int v = 0; v < 8; ++v) {
+        const size_t q = v * g.plane + o;
+        if (v == 4 || v == 5) {
+            if (stage == 1) {
+                const double de = d.dE[o];
+                d.dus[q] = v == 4 ? d.du[q] - dtdy * (de - d.dE[o - 1]) : d.du[q] + dtdx * (de - d.dE[o - g.pitch]);
+            } else {
+                d.du[q] = d.dus[q];   // B^{n+1} = B*
+            }
+            continue;
+        }
+        const double div = dtdx * (d.dF[q] - d.dF[q - g.pitch]) + dtdy * (d.dG[q] - d.dG[q - 1]);
+        if (stage == 1) d.dus[q] = d.du[q] - div;
+        else d.du[q] = 0.5 * (d.du[q] + d.dus[q] - div);
+    }
+}
+
+// The boundary fill applied to a tangent state (it is linear): periodic x,
+// then periodic or symmetric y (the same copies and Bx-rebuild of By as the
+// forward fill).
+__global__ __launch_bounds__(256) void bc_tlm_x_kernel(const AdjDesc *runs, int stage)
+{
+    const AdjDesc &d = runs[blockIdx.y];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.g.ny) return;
+    double *T = stage == 1 ? d.dus : d.du;
+    const int nx = d.g.nx;
+    const size_t P = d.g.pitch;
+    const int dst[6] = {0, 1, 2, nx - 3, nx - 2, nx - 1};
+    const int src[6] = {nx - 6, nx - 5, nx - 4, 3, 4, 5};
+    for (int v = 0; v < 8; ++v)
+        for (int q = 0; q < 6; ++q) T[v * d.g.plane + dst[q] * P + j] = T[v * d.g.plane + src[q] * P + j];
+}
+
+__global__ __launch_bounds__(256) void bc_tlm_y_kernel(const AdjDesc *runs, int stage)
+{
+    const AdjDesc &d = runs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.g.nx) return;
+    double *T = stage == 1 ? d.dus : d.du;
+    const long long pl = d.g.plane;
+    const int ny = d.g.ny;
+    const size_t row = (size_t)i * d.g.pitch;
+    const int dst[6] = {0, 1, 2, ny - 3, ny - 2, ny - 1};
+    if (d.bc_y == MHD_BC_PERIODIC) {
+        const int src[6] = {ny - 6, ny - 5, ny - 4, 3, 4, 5};
+        for (int v = 0; v < 8; ++v)
+            for (int q = 0; q < 6; ++q) T[v * pl + row + dst[q]] = T[v * pl + row + src[q]];
+        return;
+    }
+    const int src[6] = {5, 4, 3, ny - 4, ny - 5, ny - 6};
+    for (int v = 0; v < 8; ++v) {
+        if (v == 5) continue;
+        for (int q = 0; q < 6; ++q) T[v * pl + row + dst[q]] = T[v * pl + row + src[q]];
+    }
+    double *by = T + 5 * pl + row;
+    const double *bx = T + 4 * pl;
+    if (i > 0) {
+        const size_t rm = (size_t)(i - 1) * d.g.pitch;
+        const double d3 = bx[row + 3] - bx[rm + 3], d4 = bx[row + 4] - bx[rm + 4];
+        const double dA = bx[row + ny - 4] - bx[rm + ny - 4], dB = bx[row + ny - 5] - bx[rm + ny - 5];
+        const double dC = bx[row + ny - 6] - bx[rm + ny - 6];
+        by[2] = by[3] + d3 / d.ph.dx * d.ph.dy;
+        by[1] = by[2] + d3 / d.ph.dx * d.ph.dy;
+        by[0] = by[1] + d4 / d.ph.dx * d.ph.dy;
+        by[ny - 3] = by[ny - 4] - dA / d.ph.dx * d.ph.dy;
+        by[ny - 2] = by[ny - 3] - dB / d.ph.dx * d.ph.dy;
+        by[ny - 1] = by[ny - 2] - dC / d.ph.dx * d.ph.dy;
+    } else {
+        by[2] = by[1] = by[0] = by[3];
+        by[ny - 3] = by[ny - 2] = by[ny - 1] = by[ny - 4];
+    }
+}
+
+__global__ __launch_bounds__(256) void tlm_init_kernel(const AdjDesc *runs, const long long *first, int nruns)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    if (j >= d.g.ny) return;
+    const size_t o = (size_t)i * d.g.pitch + j;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        d.du[v * d.g.plane + o] = d.du0[v * d.g.plane + o];
+        d.dus[v * d.g.plane + o] = 0.0;
+    }
+}
+
+thread_local std::string g_adj_error;
+
+}  // namespace
+
+struct mhd_adjoint {
+    mhd_batch *fwd = nullptr;
+    int nruns = 0;
+    std::vector<AdjDesc> desc;
+    AdjDesc *d_desc = nullptr;
+    long long *d_first = nullptr;
+    long long total_tiles = 0;
+    double *pool = nullptr;       // lam, lamBar, lamU, lamC, lamE, geta, u0 per run
+    double *hist = nullptr, *ckpt = nullptr, *dts = nullptr;
+    size_t hist_cap = 0, ckpt_cap = 0, dts_cap = 0;   // doubles
+    int nsteps = 0, stride = 0, nckpt = 0;
+    bool forward_done = false;
+    double *tpool = nullptr;      // tangent buffers (allocated on first use)
+    std::string err;
+};
+
+namespace {
+
+int afail(mhd_adjoint *a, int code, const std::string &msg)
+{
+    if (a) a->err = msg;
+    g_adj_error = msg;
+    return code;
+}
+
+#define AHIP(a, call)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (call);                                                                             \
+        if (e_ != hipSuccess) return afail(a, MHD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+hipStream_t stream_of(const mhd_adjoint *a) { return a->fwd->stream; }
+
+int upload_desc(mhd_adjoint *a)
+{
+    AHIP(a, hipMemcpyAsync(a->d_desc, a->desc.data(), sizeof(AdjDesc) * a->desc.size(), hipMemcpyHostToDevice,
+                           stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+dim3 tiles_grid(const mhd_adjoint *a) { return dim3((unsigned)((a->total_tiles + 3) / 4)); }
+
+int copy_all(mhd_adjoint *a, int src, int src_slot, int dst, int dst_slot)
+{
+    hipLaunchKernelGGL(copy_kernel, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->d_first, a->nruns, src,
+                       src_slot, dst, dst_slot);
+    AHIP(a, hipGetLastError());
+    return MHD_OK;
+}
+
+int bc_transpose(mhd_adjoint *a, int which)
+{
+    const mhd_batch *b = a->fwd;
+    hipLaunchKernelGGL(bct_y_kernel, dim3((b->max_nx + 255) / 256, a->nruns), dim3(256), 0, stream_of(a), a->d_desc,
+                       which);
+    hipLaunchKernelGGL(bct_x_kernel, dim3((b->max_ny + 255) / 256, a->nruns), dim3(256), 0, stream_of(a), a->d_desc,
+                       which);
+    AHIP(a, hipGetLastError());
+    return MHD_OK;
+}
+
+int set_replay(mhd_adjoint *a, bool replay, bool record, long long idx0)
+{
+    hipLaunchKernelGGL(ctrl_seq_kernel, dim3((a->nruns + 63) / 64), dim3(64), 0, stream_of(a), a->d_desc,
+                       a->fwd->ctrl, a->nruns, replay ? 1 : 0, record ? 1 : 0, idx0);
+    AHIP(a, hipGetLastError());
+    return MHD_OK;
+}
+
+// Grows a device buffer to hold at least n doubles (contents not kept).
+int ensure(mhd_adjoint *a, double **p, size_t *cap, size_t n)
+{
+    if (*cap >= n) return MHD_OK;
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    AHIP(a, hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(p, sizeof(double) * n);
+    if (e != hipSuccess)
+        return afail(a, e == hipErrorOutOfMemory ? MHD_ERR_NOMEM : MHD_ERR_HIP,
+                     std::string("adjoint storage: ") + hipGetErrorString(e));
+    *cap = n;
+    return MHD_OK;
+}
+
+int layout_storage(mhd_adjoint *a, int nsteps, int stride)
+{
+    const int K = stride > 0 ? stride : std::max(1, (int)std::ceil(std::sqrt((double)nsteps)));
+    const int nck = (nsteps + K - 1) / K;
+    size_t h = 0, c = 0;
+    for (const AdjDesc &d : a->desc) {
+        h += (size_t)K * 8 * d.g.plane;
+        c += (size_t)nck * 8 * d.g.plane;
+    }
+    int rc;
+    if ((rc = ensure(a, &a->hist, &a->hist_cap, h))) return rc;
+    if ((rc = ensure(a, &a->ckpt, &a->ckpt_cap, c))) return rc;
+    size_t ho = 0, co = 0;
+    for (AdjDesc &d : a->desc) {
+        d.hist = a->hist + ho;
+        d.ckpt = a->ckpt + co;
+        ho += (size_t)K * 8 * d.g.plane;
+        co += (size_t)nck * 8 * d.g.plane;
+    }
+    a->stride = K;
+    a->nckpt = nck;
+    return upload_desc(a);
+}
+
+int ensure_dts(mhd_adjoint *a, int nsteps)
+{
+    int rc;
+    if (a->dts_cap < (size_t)nsteps * a->nruns) {
+        // keep recorded values when growing
+        std::vector<double> old;
+        if (a->dts && a->dts_cap) {
+            old.resize(a->dts_cap);
+            AHIP(a, hipMemcpy(old.data(), a->dts, sizeof(double) * a->dts_cap, hipMemcpyDeviceToHost));
+        }
+        const size_t oldper = a->nruns ? a->dts_cap / a->nruns : 0;
+        size_t cap = a->dts_cap;
+        if ((rc = ensure(a, &a->dts, &cap, (size_t)nsteps * a->nruns))) return rc;
+        a->dts_cap = cap;
+        std::vector<double> fresh((size_t)nsteps * a->nruns, 0.0);
+        for (int r = 0; r < a->nruns && oldper; ++r)
+            std::copy(old.begin() + r * oldper, old.begin() + (r + 1) * oldper, fresh.begin() + (size_t)r * nsteps);
+        AHIP(a, hipMemcpy(a->dts, fresh.data(), sizeof(double) * fresh.size(), hipMemcpyHostToDevice));
+        for (int r = 0; r < a->nruns; ++r) a->desc[r].dts = a->dts + (size_t)r * nsteps;
+        return upload_desc(a);
+    }
+    return MHD_OK;
+}
+
+int check_run(mhd_adjoint *a, int run)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (run < 0 || run >= a->nruns) return afail(a, MHD_ERR_INVALID, "run index out of range");
+    return MHD_OK;
+}
+
+int per_run_steps(const mhd_adjoint *a) { return a->nruns ? (int)(a->dts_cap / a->nruns) : 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char *mhd_adjoint_last_error(const mhd_adjoint *a) { return a ? a->err.c_str() : g_adj_error.c_str(); }
+
+int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out)
+{
+    if (!out) return afail(nullptr, MHD_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!cfgs || nruns < 1) return afail(nullptr, MHD_ERR_INVALID, "need at least one run configuration");
+    for (int r = 0; r < nruns; ++r)
+        if (cfgs[r].bc_x != MHD_BC_PERIODIC)
+            return afail(nullptr, MHD_ERR_INVALID,
+                         "the adjoint needs periodic x (run " + std::to_string(r) + "): the symmetric-x fill reads a "
+                         "stale CT value and is not differentiated");
+    mhd_adjoint *a = new (std::nothrow) mhd_adjoint();
+    if (!a) return afail(nullptr, MHD_ERR_NOMEM, "host allocation failed");
+    int rc = mhd_batch_create(cfgs, nruns, &a->fwd);
+    if (rc) {
+        const std::string msg = mhd_batch_last_error(nullptr);
+        delete a;
+        return afail(nullptr, rc, msg);
+    }
+    a->nruns = nruns;
+    a->desc.resize(nruns);
+    std::vector<long long> first(nruns + 1, 0);
+    size_t total = 0;
+    for (int r = 0; r < nruns; ++r) {
+        const mhd::RunDesc &rd = a->fwd->desc[r];
+        AdjDesc &d = a->desc[r];
+        std::memset(&d, 0, sizeof d);
+        d.g = rd.g;
+        d.ph = rd.ph;
+        d.bc_y = rd.bc_y;
+        d.ntj = (rd.g.ny + 63) / 64;
+        d.u = rd.u;
+        d.us = rd.us;
+        first[r + 1] = first[r] + (long long)rd.g.nx * d.ntj;
+        total += (size_t)(4 * 8 + 2 + 8) * rd.g.plane;
+    }
+    a->total_tiles = first[nruns];
+    hipError_t e;
+    if ((e = hipMalloc(&a->pool, sizeof(double) * total)) != hipSuccess ||
+        (e = hipMalloc(&a->d_desc, sizeof(AdjDesc) * nruns)) != hipSuccess ||
+        (e = hipMalloc(&a->d_first, sizeof(long long) * (nruns + 1))) != hipSuccess ||
+        (e = hipMemsetAsync(a->pool, 0, sizeof(double) * total, a->fwd->stream)) != hipSuccess ||
+        (e = hipMemcpyAsync(a->d_first, first.data(), sizeof(long long) * (nruns + 1), hipMemcpyHostToDevice,
+                            a->fwd->stream)) != hipSuccess) {
+        const std::string msg = std::string("mhd_adjoint_create: ") + hipGetErrorString(e);
+        mhd_adjoint_destroy(a);
+        return afail(nullptr, e == hipErrorOutOfMemory ? MHD_ERR_NOMEM : MHD_ERR_HIP, msg);
+    }
+    size_t off = 0;
+    for (AdjDesc &d : a->desc) {
+        const size_t p8 = (size_t)8 * d.g.plane;
+        d.lam = a->pool + off; off += p8;
+        d.lamBar = a->pool + off; off += p8;
+        d.lamU = a->pool + off; off += p8;
+        d.lamC = a->pool + off; off += p8;
+        d.lamE = a->pool + off; off += d.g.plane;
+        d.geta = a->pool + off; off += d.g.plane;
+        d.u0 = a->pool + off; off += p8;
+    }
+    if ((rc = upload_desc(a))) {
+        const std::string msg = a->err;
+        mhd_adjoint_destroy(a);
+        return afail(nullptr, rc, msg);
+    }
+    *out = a;
+    return MHD_OK;
+}
+
+int mhd_adjoint_destroy(mhd_adjoint *a)
+{
+    if (!a) return MHD_OK;
+    if (a->fwd) (void)hipStreamSynchronize(a->fwd->stream);
+    (void)hipFree(a->pool);
+    (void)hipFree(a->d_desc);
+    (void)hipFree(a->d_first);
+    (void)hipFree(a->hist);
+    (void)hipFree(a->ckpt);
+    (void)hipFree(a->dts);
+    (void)hipFree(a->tpool);
+    mhd_batch_destroy(a->fwd);
+    delete a;
+    return MHD_OK;
+}
+
+int mhd_adjoint_set_eta_field(mhd_adjoint *a, int run, const double *host_eta)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if ((rc = mhd_batch_set_eta_field(a->fwd, run, host_eta))) return afail(a, rc, a->fwd->err);
+    a->desc[run].eta = a->fwd->desc[run].eta;
+    return upload_desc(a);
+}
+
+int mhd_adjoint_set_state(mhd_adjoint *a, int run, const double *host_aos)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if (!host_aos) return afail(a, MHD_ERR_INVALID, "NULL state");
+    const AdjDesc &d = a->desc[run];
+    const size_t bytes = sizeof(double) * 8 * (size_t)d.g.nx * d.g.ny;
+    AHIP(a, hipMemcpyAsync(a->fwd->aos, host_aos, bytes, hipMemcpyHostToDevice, stream_of(a)));
+    AHIP(a, mhd::launch_unpack(a->fwd->aos, d.u0, d.g, stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+int mhd_adjoint_set_dts(mhd_adjoint *a, int run, const double *host_dts, int nsteps)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if (!host_dts || nsteps < 1) return afail(a, MHD_ERR_INVALID, "need nsteps >= 1 dt values");
+    if ((rc = ensure_dts(a, nsteps))) return rc;
+    AHIP(a, hipMemcpy(a->desc[run].dts, host_dts, sizeof(double) * nsteps, hipMemcpyHostToDevice));
+    return MHD_OK;
+}
+
+int mhd_adjoint_get_dts(mhd_adjoint *a, int run, double *host_dts, int nsteps)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if (!host_dts || nsteps < 1 || nsteps > per_run_steps(a)) return afail(a, MHD_ERR_INVALID, "bad dt count");
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    AHIP(a, hipMemcpy(host_dts, a->desc[run].dts, sizeof(double) * nsteps, hipMemcpyDeviceToHost));
+    return MHD_OK;
+}
+
+int mhd_adjoint_forward(mhd_adjoint *a, int nsteps, int stride, int frozen_dt)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (nsteps < 1) return afail(a, MHD_ERR_INVALID, "nsteps must be >= 1");
+    if (frozen_dt && per_run_steps(a) < nsteps) return afail(a, MHD_ERR_STATE, "frozen dt: set nsteps dt values first");
+    int rc;
+    if ((rc = ensure_dts(a, nsteps))) return rc;
+    if ((rc = layout_storage(a, nsteps, stride))) return rc;
+    mhd_batch *b = a->fwd;
+    // U = BC(U0), time 0; dt recorded (or replayed) from entry 0
+    if ((rc = copy_all(a, kU0, 0, kU, 0))) return rc;
+    if ((rc = mhd_batch_apply_boundary(b))) return afail(a, rc, b->err);
+    hipLaunchKernelGGL(ctrl_clear_kernel, dim3((a->nruns + 63) / 64), dim3(64), 0, stream_of(a), b->ctrl, a->nruns);
+    if ((rc = set_replay(a, frozen_dt != 0, frozen_dt == 0, 0))) return rc;
+    if ((rc = copy_all(a, kU, 0, kCkpt, 0))) return rc;
+    for (int c = 0; c < a->nckpt; ++c) {
+        const int n = std::min(a->stride, nsteps - c * a->stride);
+        if ((rc = mhd_batch_step(b, n))) return afail(a, rc, b->err);
+        if (c + 1 < a->nckpt && (rc = copy_all(a, kU, 0, kCkpt, c + 1))) return rc;
+    }
+    if ((rc = set_replay(a, false, false, 0))) return rc;
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    a->nsteps = nsteps;
+    a->forward_done = true;
+    return MHD_OK;
+}
+
+int mhd_adjoint_get_state(mhd_adjoint *a, int run, double *host_aos)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if ((rc = mhd_batch_get_state(a->fwd, run, host_aos))) return afail(a, rc, a->fwd->err);
+    return MHD_OK;
+}
+
+int mhd_adjoint_set_seed(mhd_adjoint *a, int run, const double *host_aos)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if (!host_aos) return afail(a, MHD_ERR_INVALID, "NULL seed");
+    const AdjDesc &d = a->desc[run];
+    const size_t bytes = sizeof(double) * 8 * (size_t)d.g.nx * d.g.ny;
+    AHIP(a, hipMemcpyAsync(a->fwd->aos, host_aos, bytes, hipMemcpyHostToDevice, stream_of(a)));
+    AHIP(a, mhd::launch_unpack(a->fwd->aos, d.lam, d.g, stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+int mhd_adjoint_backward(mhd_adjoint *a)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (!a->forward_done) return afail(a, MHD_ERR_STATE, "run mhd_adjoint_forward first");
+    mhd_batch *b = a->fwd;
+    hipStream_t s = stream_of(a);
+    const dim3 tg = tiles_grid(a), blk(256);
+    int rc;
+    hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+    for (int c = a->nckpt - 1; c >= 0; --c) {
+        const int s0 = c * a->stride, s1 = std::min(a->nsteps, s0 + a->stride);
+        // recompute the segment's states U^{s0} .. U^{s1-1} with the recorded dt
+        if ((rc = copy_all(a, kCkpt, c, kU, 0))) return rc;
+        if ((rc = set_replay(a, true, false, s0))) return rc;
+        for (int st = s0; st < s1; ++st) {
+            if ((rc = copy_all(a, kU, 0, kHist, st - s0))) return rc;
+            if (st + 1 < s1 && (rc = mhd_batch_step(b, 1))) return afail(a, rc, b->err);
+        }
+        for (int st = s1 - 1; st >= s0; --st) {
+            // U = U^st, U* = BC(stage 1 of U^st)
+            if ((rc = copy_all(a, kHist, st - s0, kU, 0))) return rc;
+            if ((rc = set_replay(a, true, false, st))) return rc;
+            AHIP(a, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
+                                            b->items_per_xcd, b->cur, s));
+            AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
+            if ((rc = bc_transpose(a, kLam))) return rc;
+            hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+            hipLaunchKernelGGL((face_adj_kernel<2, 0>), tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL((face_adj_kernel<2, 1>), tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL(cell2_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
+            if ((rc = bc_transpose(a, kLamBar))) return rc;
+            hipLaunchKernelGGL(ct_adj_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+            hipLaunchKernelGGL((face_adj_kernel<1, 0>), tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL((face_adj_kernel<1, 1>), tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL(cell1_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
+            AHIP(a, hipGetLastError());
+        }
+    }
+    // the rollout started from BC(U0)
+    if ((rc = bc_transpose(a, kLam))) return rc;
+    if ((rc = set_replay(a, false, false, 0))) return rc;
+    AHIP(a, hipStreamSynchronize(s));
+    return MHD_OK;
+}
+
+int mhd_adjoint_get_grad_state(mhd_adjoint *a, int run, double *host_aos)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if (!host_aos) return afail(a, MHD_ERR_INVALID, "NULL output");
+    const AdjDesc &d = a->desc[run];
+    AHIP(a, mhd::launch_pack(d.lam, a->fwd->aos, d.g, stream_of(a)));
+    AHIP(a, hipMemcpyAsync(host_aos, a->fwd->aos, sizeof(double) * 8 * (size_t)d.g.nx * d.g.ny,
+                           hipMemcpyDeviceToHost, stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+int mhd_adjoint_get_grad_eta(mhd_adjoint *a, int run, double *host_eta)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if (!host_eta) return afail(a, MHD_ERR_INVALID, "NULL output");
+    const AdjDesc &d = a->desc[run];
+    hipLaunchKernelGGL(pack_plane_kernel, dim3(1024), dim3(256), 0, stream_of(a), d.geta, a->fwd->aos, d.g);
+    AHIP(a, hipGetLastError());
+    AHIP(a, hipMemcpyAsync(host_eta, a->fwd->aos, sizeof(double) * (size_t)d.g.nx * d.g.ny, hipMemcpyDeviceToHost,
+                           stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+
+static int ensure_tangent(mhd_adjoint *a)
+{
+    if (a->tpool) return MHD_OK;
+    size_t total = 0;
+    for (const AdjDesc &d : a->desc) total += (size_t)(5 * 8 + 2) * d.g.plane;
+    AHIP(a, hipMalloc(&a->tpool, sizeof(double) * total));
+    AHIP(a, hipMemsetAsync(a->tpool, 0, sizeof(double) * total, stream_of(a)));
+    size_t off = 0;
+    for (AdjDesc &d : a->desc) {
+        const size_t p8 = (size_t)8 * d.g.plane;
+        d.du0 = a->tpool + off; off += p8;
+        d.du = a->tpool + off; off += p8;
+        d.dus = a->tpool + off; off += p8;
+        d.dF = a->tpool + off; off += p8;
+        d.dG = a->tpool + off; off += p8;
+        d.dE = a->tpool + off; off += d.g.plane;
+        d.deta = a->tpool + off; off += d.g.plane;
+    }
+    return upload_desc(a);
+}
+
+int mhd_adjoint_set_direction(mhd_adjoint *a, int run, const double *host_dU0, const double *host_deta)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if ((rc = ensure_tangent(a))) return rc;
+    const AdjDesc &d = a->desc[run];
+    const size_t cells = (size_t)d.g.nx * d.g.ny;
+    if (host_dU0) {
+        AHIP(a, hipMemcpyAsync(a->fwd->aos, host_dU0, sizeof(double) * 8 * cells, hipMemcpyHostToDevice, stream_of(a)));
+        AHIP(a, mhd::launch_unpack(a->fwd->aos, d.du0, d.g, stream_of(a)));
+    } else {
+        AHIP(a, hipMemsetAsync(d.du0, 0, sizeof(double) * 8 * d.g.plane, stream_of(a)));
+    }
+    if (host_deta) {
+        AHIP(a, hipMemcpyAsync(a->fwd->aos, host_deta, sizeof(double) * cells, hipMemcpyHostToDevice, stream_of(a)));
+        AHIP(a, mhd::launch_scatter_plane(a->fwd->aos, d.deta, d.g, stream_of(a)));
+    } else {
+        AHIP(a, hipMemsetAsync(d.deta, 0, sizeof(double) * d.g.plane, stream_of(a)));
+    }
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (nsteps < 1 || nsteps > per_run_steps(a)) return afail(a, MHD_ERR_STATE, "needs a recorded or set dt per step");
+    int rc;
+    if ((rc = ensure_tangent(a))) return rc;
+    mhd_batch *b = a->fwd;
+    hipStream_t s = stream_of(a);
+    const dim3 tg = tiles_grid(a), blk(256);
+    const dim3 gx((b->max_ny + 255) / 256, a->nruns), gy((b->max_nx + 255) / 256, a->nruns);
+    if ((rc = copy_all(a, kU0, 0, kU, 0))) return rc;
+    if ((rc = mhd_batch_apply_boundary(b))) return afail(a, rc, b->err);
+    hipLaunchKernelGGL(tlm_init_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
+    hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 2);
+    hipLaunchKernelGGL(bc_tlm_y_kernel, gy, blk, 0, s, a->d_desc, 2);
+    if ((rc = set_replay(a, true, false, 0))) return rc;
+    for (int st = 0; st < nsteps; ++st) {
+        // tangents of stage 1 from (U, dU), primal U* = BC(stage 1 of U)
+        hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+        hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+        hipLaunchKernelGGL(emf_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
+        hipLaunchKernelGGL(stage_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1, st);
+        hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 1);
+        hipLaunchKernelGGL(bc_tlm_y_kernel, gy, blk, 0, s, a->d_desc, 1);
+        AHIP(a, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
+                                        b->items_per_xcd, b->cur, s));
+        AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
+        // tangents of stage 2 from (U*, dU*) (B keeps the stage-1 CT value)
+        hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+        hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+        hipLaunchKernelGGL(stage_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2, st);
+        hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 2);
+        hipLaunchKernelGGL(bc_tlm_y_kernel, gy, blk, 0, s, a->d_desc, 2);
+        AHIP(a, hipGetLastError());
+        // primal stage 2 and its boundary: the state the next step starts from
+        AHIP(a, mhd::launch_stage_batch(2, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
+                                        b->items_per_xcd, b->cur, s));
+        AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, false, true, b->cur ^ 1, s));
+        b->cur ^= 1;
+    }
+    if ((rc = set_replay(a, false, false, 0))) return rc;
+    AHIP(a, hipStreamSynchronize(s));
+    return MHD_OK;
+}
+
+int mhd_adjoint_get_tangent(mhd_adjoint *a, int run, double *host_dUN)
+{
+    int rc = check_run(a, run);
+    if (rc) return rc;
+    if (!host_dUN || !a->tpool) return afail(a, MHD_ERR_STATE, "run mhd_adjoint_tangent first");
+    const AdjDesc &d = a->desc[run];
+    AHIP(a, mhd::launch_pack(d.du, a->fwd->aos, d.g, stream_of(a)));
+    AHIP(a, hipMemcpyAsync(host_dUN, a->fwd->aos, sizeof(double) * 8 * (size_t)d.g.nx * d.g.ny,
+                           hipMemcpyDeviceToHost, stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+int mhd_adjoint_synchronize(mhd_adjoint *a)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+}  // extern "C"

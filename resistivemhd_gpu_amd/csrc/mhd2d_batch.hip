@@ -27,30 +27,7 @@ using mhd::Grid;
 using mhd::Phys;
 using mhd::RunDesc;
 
-struct mhd_batch {
-    int nruns = 0;
-    std::vector<mhd_config> cfg;
-    std::vector<RunDesc> desc;          // host copy of the descriptors
-    std::vector<double *> eta;          // per-run resistivity planes (nullptr: uniform)
-    RunDesc *d_desc = nullptr;
-    int *d_item_run = nullptr;
-    int item_cap = 0;
-    double *pool = nullptr;             // U, U* and the side row of every run
-    Ctrl *ctrl = nullptr;               // one control block per run
-    double *aos = nullptr;              // staging for host transfers (largest run)
-    hipStream_t stream = nullptr;
-    int device = 0, num_cu = 0;
-    bool exact = true;
-    int seg_request = 0, seg = 0;
-    int total_items = 0, xcds = 1, items_per_xcd = 0;
-    int max_nx = 0, max_ny = 0;
-    int cur = 0;
-    bool dt_valid = false;
-    bool use_graph = true;
-    hipGraphExec_t graph = nullptr;
-    int graph_cur = -1;                 // control slot the captured graph starts from
-    std::string err;
-};
+#include "mhd2d_batch_impl.h"
 
 namespace {
 
@@ -139,7 +116,9 @@ int default_seg(const mhd_batch *b)
     return best;
 }
 
-int refresh_dt(mhd_batch *b)
+}  // namespace
+
+int mhd_batch_refresh_dt(mhd_batch *b)
 {
     if (b->dt_valid) return MHD_OK;
     for (int r = 0; r < b->nruns; ++r)
@@ -148,6 +127,10 @@ int refresh_dt(mhd_batch *b)
     b->dt_valid = true;
     return MHD_OK;
 }
+
+namespace {
+
+int refresh_dt(mhd_batch *b) { return mhd_batch_refresh_dt(b); }
 
 int enqueue_step(mhd_batch *b, int cur)
 {

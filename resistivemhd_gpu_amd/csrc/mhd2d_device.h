@@ -29,6 +29,12 @@ struct Ctrl {
     // NaN, which then sticks (a NaN anywhere else is passed over)
     int first_nan[2];
     int pad;
+    // dt replay (frozen dt): when dt_in is set, stage 1 of step `steps`
+    // takes dt_in[steps - seq0] instead of the CFL dt; when dt_out is set,
+    // it records the dt it used there.
+    const double *dt_in;
+    double *dt_out;
+    long long seq0;
 };
 
 constexpr unsigned long long kInfBits = 0x7FF0000000000000ull;

@@ -117,7 +117,10 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             const double lim = fmin(ph.dx * ph.dx / ph.eta, ph.dy * ph.dy / ph.eta);
             dt = fmin(m, lim) * ph.cfl;
         }
+        const long long seq = ctrl->steps - ctrl->seq0;
+        if (ctrl->dt_in) dt = ctrl->dt_in[seq];
         if (item == 0 && lane == 0) {
+            if (ctrl->dt_out) ctrl->dt_out[seq] = dt;
             ctrl->dt = dt;
             ctrl->acc[cur ^ 1] = kInfBits;
             ctrl->first_nan[cur ^ 1] = 0;

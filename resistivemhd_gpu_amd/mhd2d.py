@@ -35,6 +35,11 @@ EXPORTS = (
     "mhd_batch_init_harris", "mhd_batch_step", "mhd_batch_synchronize", "mhd_batch_get_dt", "mhd_batch_get_time",
     "mhd_batch_check_crashed", "mhd_batch_set_numerics", "mhd_batch_set_sweep_length",
     "mhd_batch_get_sweep_length", "mhd_batch_set_graph", "mhd_batch_autotune",
+    "mhd_adjoint_create", "mhd_adjoint_destroy", "mhd_adjoint_last_error", "mhd_adjoint_set_eta_field",
+    "mhd_adjoint_set_state", "mhd_adjoint_set_dts", "mhd_adjoint_get_dts", "mhd_adjoint_forward",
+    "mhd_adjoint_get_state", "mhd_adjoint_set_seed", "mhd_adjoint_backward", "mhd_adjoint_get_grad_state",
+    "mhd_adjoint_get_grad_eta", "mhd_adjoint_synchronize", "mhd_adjoint_set_direction", "mhd_adjoint_tangent",
+    "mhd_adjoint_get_tangent",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -72,6 +77,13 @@ def load_library(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise MHDError(f"{path} is missing: build it with `python -m resistivemhd_gpu_amd.build`")
+    try:
+        # PyTorch ships its own HIP runtime; loading it first lets the library
+        # bind to the same HSA runtime, so torch tensors and this library can
+        # share the GPU in one process whichever initialises the device first.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     P = ctypes.c_void_p
     D = ctypes.POINTER(ctypes.c_double)
@@ -134,6 +146,23 @@ def load_library(path=LIB_PATH):
         "mhd_batch_get_sweep_length": (I, [P, ctypes.POINTER(I)]),
         "mhd_batch_set_graph": (I, [P, I]),
         "mhd_batch_autotune": (I, [P, I, ctypes.POINTER(I)]),
+        "mhd_adjoint_create": (I, [C, I, ctypes.POINTER(P)]),
+        "mhd_adjoint_destroy": (I, [P]),
+        "mhd_adjoint_last_error": (ctypes.c_char_p, [P]),
+        "mhd_adjoint_set_eta_field": (I, [P, I, D]),
+        "mhd_adjoint_set_state": (I, [P, I, D]),
+        "mhd_adjoint_set_dts": (I, [P, I, D, I]),
+        "mhd_adjoint_get_dts": (I, [P, I, D, I]),
+        "mhd_adjoint_forward": (I, [P, I, I, I]),
+        "mhd_adjoint_get_state": (I, [P, I, D]),
+        "mhd_adjoint_set_seed": (I, [P, I, D]),
+        "mhd_adjoint_backward": (I, [P]),
+        "mhd_adjoint_get_grad_state": (I, [P, I, D]),
+        "mhd_adjoint_get_grad_eta": (I, [P, I, D]),
+        "mhd_adjoint_synchronize": (I, [P]),
+        "mhd_adjoint_set_direction": (I, [P, I, D, D]),
+        "mhd_adjoint_tangent": (I, [P, I]),
+        "mhd_adjoint_get_tangent": (I, [P, I, D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -507,3 +536,113 @@ class ResistiveMHD2DBatch:
         v = ctypes.c_int()
         self._ok(self._lib.mhd_batch_autotune(self._h, int(steps), ctypes.byref(v)))
         return v.value
+
+
+class ResistiveMHD2DAdjoint:
+    """Forward rollout and discrete adjoint of the RK2 step for a batch of
+    runs (mhd_adjoint_* of the C ABI): dt of every step frozen at its forward
+    value, sqrt(N) checkpointing.  Gradients are of a loss of the final
+    states with respect to the initial states (before their boundary fill)
+    and the per-cell resistivity."""
+
+    def __init__(self, cfgs):
+        self._lib = load_library()
+        self.cfgs = [MhdConfig.from_buffer_copy(c) for c in cfgs]
+        arr = (MhdConfig * len(self.cfgs))(*self.cfgs)
+        h = ctypes.c_void_p()
+        rc = self._lib.mhd_adjoint_create(arr, len(self.cfgs), ctypes.byref(h))
+        if rc != 0:
+            msg = self._lib.mhd_adjoint_last_error(None)
+            raise MHDError(f"libmhd2d error {rc}: {msg.decode() if msg else ''}")
+        self._h = h
+        self.nsteps = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mhd_adjoint_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.cfgs)
+
+    def _ok(self, rc):
+        if rc != 0:
+            msg = self._lib.mhd_adjoint_last_error(self._h)
+            raise MHDError(f"libmhd2d error {rc}: {msg.decode() if msg else ''}")
+
+    def shape(self, run):
+        c = self.cfgs[run]
+        return (c.nx, c.ny, NVAR)
+
+    def setEtaField(self, run, eta):
+        if eta is None:
+            self._ok(self._lib.mhd_adjoint_set_eta_field(self._h, run, None))
+            return
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        self._ok(self._lib.mhd_adjoint_set_eta_field(self._h, run, _dptr(eta)))
+
+    def setU0(self, run, U0):
+        U0 = np.ascontiguousarray(U0, dtype=np.float64)
+        if U0.shape != self.shape(run):
+            raise ValueError(f"state shape {U0.shape} != {self.shape(run)}")
+        self._ok(self._lib.mhd_adjoint_set_state(self._h, run, _dptr(U0)))
+
+    def set_dts(self, run, dts):
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        self._ok(self._lib.mhd_adjoint_set_dts(self._h, run, _dptr(dts), len(dts)))
+
+    def dts(self, run, nsteps=None):
+        n = nsteps or self.nsteps
+        out = np.empty(n)
+        self._ok(self._lib.mhd_adjoint_get_dts(self._h, run, _dptr(out), n))
+        return out
+
+    def forward(self, nsteps, stride=0, frozen_dt=False):
+        self._ok(self._lib.mhd_adjoint_forward(self._h, int(nsteps), int(stride), 1 if frozen_dt else 0))
+        self.nsteps = int(nsteps)
+
+    def getU(self, run):
+        out = np.empty(self.shape(run))
+        self._ok(self._lib.mhd_adjoint_get_state(self._h, run, _dptr(out)))
+        return out
+
+    def set_seed(self, run, lam):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        if lam.shape != self.shape(run):
+            raise ValueError(f"seed shape {lam.shape} != {self.shape(run)}")
+        self._ok(self._lib.mhd_adjoint_set_seed(self._h, run, _dptr(lam)))
+
+    def backward(self):
+        self._ok(self._lib.mhd_adjoint_backward(self._h))
+
+    def grad_state(self, run):
+        out = np.empty(self.shape(run))
+        self._ok(self._lib.mhd_adjoint_get_grad_state(self._h, run, _dptr(out)))
+        return out
+
+    def grad_eta(self, run):
+        c = self.cfgs[run]
+        out = np.empty((c.nx, c.ny))
+        self._ok(self._lib.mhd_adjoint_get_grad_eta(self._h, run, _dptr(out)))
+        return out
+
+    def set_direction(self, run, dU0=None, deta=None):
+        """Direction of the tangent-linear model (None: zero)."""
+        a = None if dU0 is None else np.ascontiguousarray(dU0, dtype=np.float64)
+        b = None if deta is None else np.ascontiguousarray(deta, dtype=np.float64)
+        self._ok(self._lib.mhd_adjoint_set_direction(self._h, run, _dptr(a) if a is not None else None,
+                                                     _dptr(b) if b is not None else None))
+
+    def tangent(self, nsteps=None):
+        self._ok(self._lib.mhd_adjoint_tangent(self._h, int(nsteps or self.nsteps)))
+
+    def get_tangent(self, run):
+        out = np.empty(self.shape(run))
+        self._ok(self._lib.mhd_adjoint_get_tangent(self._h, run, _dptr(out)))
+        return out

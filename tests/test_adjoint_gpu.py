@@ -1,0 +1,223 @@
+"""GPU checks of the discrete adjoint (mhd_adjoint_*): the rollout it
+records equals the plain step bitwise, and the gradients of a linear
+functional of the final state, L = sum(W * U_N), with respect to the
+per-cell resistivity and the initial velocity agree with central finite
+differences of the same rollout with every dt frozen at its recorded value.
+"""
+import numpy as np
+import pytest
+
+import resistivemhd_gpu_amd as mhd
+from tests.helpers import boundary_applied
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _ot(n=24, eta=1e-2):
+    cfg = mhd.orszag_tang_config(n, eta=eta)
+    return cfg, mhd.fill_orszag_tang(cfg)
+
+
+def _harris(nx=40, ny=24):
+    """Harris sheet resolved by two cells per unit length (periodic x,
+    symmetric y)."""
+    cfg = mhd.harris_config(nx, ny, eta_max=2e-2)
+    cfg.dx = cfg.dy = 0.5
+    cfg.xmin, cfg.ymin = -0.25 * nx, -0.25 * ny
+    U = mhd.fill_harris(cfg, seed=3, noise_amp=5e-2)
+    return cfg, U
+
+
+def _eta_field(cfg, seed):
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0, 1, cfg.nx)[:, None]
+    y = np.linspace(0, 1, cfg.ny)[None, :]
+    base = 5e-3 + 3e-3 * np.sin(2 * np.pi * x + rng.uniform(0, 6)) * np.cos(2 * np.pi * y)
+    return np.ascontiguousarray(np.broadcast_to(base, (cfg.nx, cfg.ny)))
+
+
+def _weights(cfg, seed):
+    rng = np.random.default_rng(seed)
+    W = np.zeros((cfg.nx, cfg.ny, 8))
+    g = mhd.GHOST
+    W[g:-g, g:-g] = rng.normal(size=(cfg.nx - 2 * g, cfg.ny - 2 * g, 8))
+    return W
+
+
+def _loss(cfg, U0, eta, W, nsteps, dts):
+    a = mhd.ResistiveMHD2DAdjoint([cfg])
+    a.setU0(0, U0)
+    a.setEtaField(0, eta)
+    a.set_dts(0, dts)
+    a.forward(nsteps, frozen_dt=True)
+    return float((W * a.getU(0)).sum())
+
+
+def _velocity_direction(U0, seed):
+    """dU0 of a velocity perturbation (dvx, dvy) at fixed rho, p and B."""
+    rng = np.random.default_rng(seed)
+    rho = U0[..., 0]
+    u, v = U0[..., 1] / rho, U0[..., 2] / rho
+    dvx = rng.normal(size=rho.shape)
+    dvy = rng.normal(size=rho.shape)
+    dU = np.zeros_like(U0)
+    dU[..., 1] = rho * dvx
+    dU[..., 2] = rho * dvy
+    dU[..., 7] = rho * (u * dvx + v * dvy)
+    return dU
+
+
+def test_adjoint_rollout_equals_plain_step():
+    cfg, U0 = _ot(32)
+    eta = _eta_field(cfg, 1)
+    a = mhd.ResistiveMHD2DAdjoint([cfg])
+    a.setU0(0, U0)
+    a.setEtaField(0, eta)
+    a.forward(25, stride=4)
+    s = mhd.ResistiveMHD2D(cfg)
+    s.initializeU(U0)
+    s.setEtaField(eta)
+    dts = []
+    for _ in range(25):
+        s.oneStepRK2()
+        dts.append(s.dt)
+    assert np.array_equal(a.getU(0), s.getU())
+    assert np.array_equal(a.dts(0), np.array(dts))
+
+
+def _generic(U0, seed, s=0.05):
+    """The setups have exact symmetries (uniform density, By = Bz = 0, ...)
+    that put MUSCL's minmod at its kink, where the step has directional
+    derivatives but no gradient; a small random perturbation of every
+    component makes the map differentiable at the test point."""
+    rng = np.random.default_rng(seed)
+    U = U0.copy()
+    U[..., 0] *= 1 + 0.4 * s * rng.normal(size=U[..., 0].shape)
+    for v in (1, 2, 3, 4, 5, 6):
+        U[..., v] += s * rng.normal(size=U[..., v].shape)
+    U[..., 7] += s + 0.4 * s * rng.normal(size=U[..., 7].shape)
+    return U
+
+
+@pytest.mark.parametrize("case, tol", [("ot", 1e-12), ("harris", 1e-8)])
+def test_adjoint_matches_tangent_linear(case, tol):
+    """<dL/dU0, dU0> + <dL/deta, deta> against the tangent-linear model's
+    <W, dU_N> for random directions.  One step agrees to round-off in every
+    component; over 20 steps the noisy, under-resolved Harris sheet has
+    Jacobian entries that cancel by orders of magnitude (HLLD denominators
+    near degeneracy), so the two FP64 sweeps, which round differently,
+    drift apart at the 1e-10 level there."""
+    cfg, U0 = _ot() if case == "ot" else _harris()
+    U0 = _generic(U0, 9)
+    nsteps = 20
+    eta = _eta_field(cfg, 2)
+    W = _weights(cfg, 3)
+    print(f"case adjoint-tlm-{case}: nx={cfg.nx} ny={cfg.ny} steps={nsteps}", flush=True)
+    a = mhd.ResistiveMHD2DAdjoint([cfg])
+    a.setU0(0, U0)
+    a.setEtaField(0, eta)
+    a.forward(nsteps, stride=6)
+    a.set_seed(0, W)
+    a.backward()
+    gU, geta = a.grad_state(0), a.grad_eta(0)
+    assert np.all(np.isfinite(gU)) and np.all(np.isfinite(geta))
+    rng = np.random.default_rng(4)
+    for k in range(4):
+        dU0 = rng.normal(size=U0.shape) * (k != 1)
+        deta = rng.normal(size=eta.shape) * eta * (k != 0)
+        a.set_direction(0, dU0, deta)
+        a.tangent(nsteps)
+        tl = float((W * a.get_tangent(0)).sum())
+        ad = float((gU * dU0).sum() + (geta * deta).sum())
+        print(f"dir {k}: adjoint {ad:.15e} tangent {tl:.15e} rel {abs(ad - tl) / abs(tl):.2e}", flush=True)
+        assert abs(ad - tl) <= tol * abs(tl)
+
+
+def test_gradients_match_central_differences():
+    """Against central differences of the same rollout with every dt frozen
+    at its recorded value (Orszag-Tang, perturbed to a generic state)."""
+    cfg, U0 = _ot()
+    U0 = _generic(U0, 9)
+    nsteps = 20
+    eta = _eta_field(cfg, 2)
+    W = _weights(cfg, 3)
+    print(f"case adjoint-fd: nx={cfg.nx} ny={cfg.ny} steps={nsteps}", flush=True)
+    a = mhd.ResistiveMHD2DAdjoint([cfg])
+    a.setU0(0, U0)
+    a.setEtaField(0, eta)
+    a.forward(nsteps, stride=6)
+    dts = a.dts(0)
+    a.set_seed(0, W)
+    a.backward()
+    gU, geta = a.grad_state(0), a.grad_eta(0)
+    rng = np.random.default_rng(4)
+    for k in range(3):
+        de = rng.normal(size=eta.shape) * eta
+        h = 1e-5
+        fd = (_loss(cfg, U0, eta + h * de, W, nsteps, dts) - _loss(cfg, U0, eta - h * de, W, nsteps, dts)) / (2 * h)
+        ad = float((geta * de).sum())
+        print(f"eta dir {k}: adjoint {ad:.12e} fd {fd:.12e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+        assert abs(ad - fd) <= TOL * abs(fd)
+        dU = _velocity_direction(U0, 10 + k)
+        h = 1e-7
+        fd = (_loss(cfg, U0 + h * dU, eta, W, nsteps, dts) - _loss(cfg, U0 - h * dU, eta, W, nsteps, dts)) / (2 * h)
+        ad = float((gU * dU).sum())
+        print(f"v dir {k}: adjoint {ad:.12e} fd {fd:.12e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+        assert abs(ad - fd) <= TOL * abs(fd)
+
+
+def test_batched_adjoint_and_stride_agree():
+    """A batch of uneven runs gives each run the gradient it gets alone, for
+    any checkpoint stride (sums of atomics: equal to round-off)."""
+    cases = [_ot(24), _ot(40, eta=3e-3), _harris(32, 20)]
+    cases = [(c, _generic(U, 20 + k, 0.005 if k == 2 else 0.05)) for k, (c, U) in enumerate(cases)]
+    nsteps = 9
+    grads = []
+    a = mhd.ResistiveMHD2DAdjoint([c for c, _ in cases])
+    for r, (c, U0) in enumerate(cases):
+        a.setU0(r, U0)
+        a.setEtaField(r, _eta_field(c, r))
+    a.forward(nsteps, stride=2)
+    for r, (c, _) in enumerate(cases):
+        a.set_seed(r, _weights(c, 30 + r))
+    a.backward()
+    for r, (c, U0) in enumerate(cases):
+        grads.append((a.grad_state(r), a.grad_eta(r)))
+    for r, (c, U0) in enumerate(cases):
+        for stride in (1, 0, 9):
+            s = mhd.ResistiveMHD2DAdjoint([c])
+            s.setU0(0, U0)
+            s.setEtaField(0, _eta_field(c, r))
+            s.forward(nsteps, stride=stride)
+            s.set_seed(0, _weights(c, 30 + r))
+            s.backward()
+            gU, ge = s.grad_state(0), s.grad_eta(0)
+            # face contributions are summed with atomics: equal up to the
+            # rounding of their order, amplified where contributions cancel
+            np.testing.assert_allclose(gU, grads[r][0], rtol=1e-6, atol=1e-9 * np.abs(gU).max())
+            np.testing.assert_allclose(ge, grads[r][1], rtol=1e-6, atol=1e-9 * np.abs(ge).max())
+
+
+def test_torch_autograd_function():
+    import torch
+    from resistivemhd_gpu_amd.torch_adjoint import rollout
+    cfg, U0 = _ot(24)
+    eta = _eta_field(cfg, 5)
+    W = _weights(cfg, 6)
+    U0t = torch.tensor(U0[None], dtype=torch.float64, device="cuda", requires_grad=True)
+    etat = torch.tensor(eta[None], dtype=torch.float64, device="cuda", requires_grad=True)
+    UN = rollout(U0t, etat, [cfg], 12)
+    loss = (UN * torch.tensor(W[None], device="cuda")).sum()
+    loss.backward()
+    a = mhd.ResistiveMHD2DAdjoint([cfg])
+    a.setU0(0, U0)
+    a.setEtaField(0, eta)
+    a.forward(12)
+    a.set_seed(0, W)
+    a.backward()
+    np.testing.assert_allclose(U0t.grad[0].cpu().numpy(), a.grad_state(0), rtol=1e-8,
+                               atol=1e-11 * np.abs(a.grad_state(0)).max())
+    np.testing.assert_allclose(etat.grad[0].cpu().numpy(), a.grad_eta(0), rtol=1e-8,
+                               atol=1e-11 * np.abs(a.grad_eta(0)).max())
