@@ -177,10 +177,163 @@ def batch_bench(nruns, steps, plain_steps, run_steps, numerics):
     }
 
 
+def adjoint_bench(nruns, steps, run_steps):
+    """Runs/s of batched forward+adjoint against a loop of single-run
+    forward+adjoint objects, on the batched workload (states resident on the
+    GPU; dL/dU_N = U_N as the seed).  runs/s = runs * (steps / run_steps) /
+    seconds."""
+    import numpy as np
+    import resistivemhd_gpu_amd as mhd
+    runs = batch_configs(mhd, nruns)
+
+    def init(obj, r, c, kind, seed):
+        U0 = mhd.fill_harris(c, seed=seed, noise_amp=1e-3) if kind == "harris" else mhd.fill_orszag_tang(c)
+        obj.setU0(r, U0)
+
+    a = mhd.ResistiveMHD2DAdjoint([c for c, _, _ in runs])
+    for r, (c, kind, seed) in enumerate(runs):
+        init(a, r, c, kind, seed)
+    a.forward(2)
+    a.backward()
+    t0 = time.perf_counter()
+    a.forward(steps)
+    a.set_seed_all(a.getU_all())
+    a.backward()
+    t_batch = time.perf_counter() - t0
+    finite = all(np.isfinite(a.grad_state(r)).all() for r in range(0, nruns, 17))
+    del a
+
+    singles = []
+    for c, kind, seed in runs:
+        s = mhd.ResistiveMHD2DAdjoint([c])
+        init(s, 0, c, kind, seed)
+        s.forward(2)
+        s.backward()
+        singles.append(s)
+    t0 = time.perf_counter()
+    for s in singles:
+        s.forward(steps)
+        s.set_seed(0, s.getU(0))
+        s.backward()
+    t_plain = time.perf_counter() - t0
+    del singles
+    return {
+        "runs": nruns, "steps_per_run": run_steps, "timed_steps": steps,
+        "batched_fwd_adj_runs_per_s": nruns * steps / run_steps / t_batch,
+        "plain_fwd_adj_runs_per_s": nruns * steps / run_steps / t_plain,
+        "speedup_vs_plain": t_plain / t_batch,
+        "batched_s": t_batch, "plain_s": t_plain, "gradients_finite": bool(finite),
+        "note": "host transfers of U_N and the seeds included in both (whole-batch calls for the batch)",
+    }
+
+
+def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
+    """Harris sheet nx x ny, batch of nruns: 'observed' final states from a
+    seeded eta field per run (sum of 6 Gaussians, amplitudes log-uniform in
+    [1e-4, 5e-3]); loss = |(rho, Bx, By)(T) - observed|^2 from eta = 1e-3;
+    gradients with respect to eta(x, y), vx0 and vy0 from the adjoint,
+    checked on run 0 against central differences along ndirs random
+    directions with every dt frozen at its recorded value.  The initial vx
+    carries seeded noise of amplitude 1e-3 like vy, so that MUSCL's limiter
+    is not at its kink (vx = 0 exactly has directional derivatives but no
+    gradient)."""
+    import numpy as np
+    import resistivemhd_gpu_amd as mhd
+    g = mhd.GHOST
+    cfg = mhd.harris_config(nx, ny, eta_max=6e-3)
+    rng = np.random.default_rng(seed)
+    X = cfg.xmin + (np.arange(cfg.nx) - g + 0.5)[:, None] * cfg.dx
+    Y = cfg.ymin + (np.arange(cfg.ny) - g + 0.5)[None, :] * cfg.dy
+
+    def gaussians(r):
+        eta = np.zeros((cfg.nx, cfg.ny))
+        for _ in range(6):
+            amp = 10 ** r.uniform(-4, np.log10(5e-3))
+            x0, y0 = r.uniform(-80, 80), r.uniform(-5, 5)
+            w = r.uniform(3, 15)
+            eta += amp * np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / w ** 2)
+        return eta + 1e-4
+
+    U0 = mhd.fill_harris(cfg, seed=seed, noise_amp=1e-3)
+    vx = 1e-3 * (2 * rng.uniform(size=U0.shape[:2]) - 1)
+    U0[..., 7] += 0.5 * U0[..., 0] * vx * vx
+    U0[..., 1] = U0[..., 0] * vx
+    cfgs = [cfg] * nruns
+    t0 = time.perf_counter()
+    obs = mhd.ResistiveMHD2DAdjoint(cfgs)
+    for r in range(nruns):
+        obs.setU0(r, U0)
+        obs.setEtaField(r, gaussians(np.random.default_rng(seed + 100 + r)))
+    obs.forward(steps)
+    target = [obs.getU(r) for r in range(nruns)]
+    del obs
+    eta0 = np.full((cfg.nx, cfg.ny), 1e-3)
+    mask = np.zeros((cfg.nx, cfg.ny, 8))
+    mask[g:-g, g:-g, [0, 4, 5]] = 1.0
+
+    a = mhd.ResistiveMHD2DAdjoint(cfgs)
+    for r in range(nruns):
+        a.setU0(r, U0)
+        a.setEtaField(r, eta0)
+    t1 = time.perf_counter()
+    a.forward(steps)
+    UN = [a.getU(r) for r in range(nruns)]
+    for r in range(nruns):
+        a.set_seed(r, 2.0 * mask * (UN[r] - target[r]))
+    a.backward()
+    t_grad = time.perf_counter() - t1
+    dts = a.dts(0, steps)
+    gU, geta = a.grad_state(0), a.grad_eta(0)
+    rho = U0[..., 0]
+    gvx = gU[..., 1] * rho + gU[..., 7] * rho * vx
+    gvy = gU[..., 2] * rho + gU[..., 7] * rho * (U0[..., 2] / rho)
+    del a
+
+    one = mhd.ResistiveMHD2DAdjoint([cfg])
+
+    def loss(U, eta):
+        one.setU0(0, U)
+        one.setEtaField(0, eta)
+        one.set_dts(0, dts)
+        one.forward(steps, frozen_dt=True)
+        return float((mask * (one.getU(0) - target[0]) ** 2).sum())
+
+    def with_velocity(dvx, dvy, h):
+        U = U0.copy()
+        u = vx + h * dvx
+        v = U0[..., 2] / rho + h * dvy
+        U[..., 1], U[..., 2] = rho * u, rho * v
+        U[..., 7] = U0[..., 7] - 0.5 * rho * (vx * vx + (U0[..., 2] / rho) ** 2) + 0.5 * rho * (u * u + v * v)
+        return U
+
+    errs = []
+    drng = np.random.default_rng(seed + 7)
+    for k in range(ndirs):
+        de = drng.normal(size=eta0.shape) * 1e-3
+        dvx = drng.normal(size=rho.shape) * 1e-3
+        dvy = drng.normal(size=rho.shape) * 1e-3
+        h = 1e-4
+        fd = (loss(with_velocity(dvx, dvy, h), eta0 + h * de) - loss(with_velocity(dvx, dvy, -h), eta0 - h * de)) / (2 * h)
+        ad = float((geta * de).sum() + (gvx * dvx)[g:-g, g:-g].sum() + (gvy * dvy)[g:-g, g:-g].sum())
+        errs.append(abs(ad - fd) / abs(fd))
+    return {
+        "workload": f"harris {nx}x{ny}, batch {nruns}, {steps} steps", "directions": ndirs,
+        "rel_err_max": max(errs), "rel_err_median": float(np.median(errs)), "rel_errs": errs,
+        "fwd_adj_seconds_batch": t_grad, "setup_seconds": t1 - t0,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["harris", "turbulence", "batch"], default="harris")
+    ap.add_argument("--workload", choices=["harris", "turbulence", "batch", "adjoint", "gradcheck"],
+                    default="harris")
+    ap.add_argument("--adj-steps", type=int, default=40, help="timed forward+adjoint steps per run")
+    ap.add_argument("--check-nx", type=int, default=1024)
+    ap.add_argument("--check-ny", type=int, default=512)
+    ap.add_argument("--check-runs", type=int, default=32)
+    ap.add_argument("--check-steps", type=int, default=2000)
+    ap.add_argument("--check-dirs", type=int, default=16)
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=40, help="timed batched steps")
     ap.add_argument("--plain-steps", type=int, default=10, help="timed steps per run of the plain loop")
@@ -211,6 +364,20 @@ def main():
     import torch
     import resistivemhd_gpu_amd as mhd
 
+    if args.workload == "adjoint":
+        res = adjoint_bench(args.batch_runs, args.adj_steps, 500)
+        print(json.dumps({"metric": "batched forward+adjoint runs/s vs plain per-instance loop",
+                          "value": res["batched_fwd_adj_runs_per_s"], "unit": "runs/s", "n_gpus": 1,
+                          "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": batch_configs.__doc__.split(":")[0].strip()}, "adjoint": res}),
+              flush=True)
+        return
+    if args.workload == "gradcheck":
+        res = gradient_check(args.check_runs, args.check_nx, args.check_ny, args.check_steps, args.check_dirs)
+        print(json.dumps({"metric": "adjoint gradient vs central differences (relative error)",
+                          "value": res["rel_err_max"], "unit": "relative", "higher_is_better": False,
+                          "dtype": "f64", "data": "synthetic", "gradcheck": res}), flush=True)
+        return
     if args.workload == "batch":
         res = batch_bench(args.batch_runs, args.batch_steps, args.plain_steps, 500, args.numerics)
         print(json.dumps({"metric": "batched runs/s vs plain per-instance loop", "value": res["batched_runs_per_s"],

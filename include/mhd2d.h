@@ -283,6 +283,12 @@ int mhd_adjoint_get_grad_eta(mhd_adjoint *a, int run, double *host_eta);     /* 
 int mhd_adjoint_set_direction(mhd_adjoint *a, int run, const double *host_dU0, const double *host_deta);
 int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps);
 int mhd_adjoint_get_tangent(mhd_adjoint *a, int run, double *host_dUN);
+/* Whole-batch transfers (one copy and one synchronisation): every run's AoS
+ * state concatenated in run order. */
+int mhd_adjoint_set_states(mhd_adjoint *a, const double *host_all);        /* all U0 */
+int mhd_adjoint_get_states(mhd_adjoint *a, double *host_all);              /* all U_N */
+int mhd_adjoint_set_seeds(mhd_adjoint *a, const double *host_all);         /* all dL/dU_N */
+int mhd_adjoint_get_grad_states(mhd_adjoint *a, double *host_all);         /* all dL/dU0 */
 int mhd_adjoint_synchronize(mhd_adjoint *a);
 
 #ifdef __cplusplus

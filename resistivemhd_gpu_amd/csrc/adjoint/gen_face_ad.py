@@ -91,7 +91,10 @@ class Lowering:
 
 
 def parse(path):
+    """Returns params, inputs, outputs, ops and the segment index of each op
+    (`segment:` lines split a listing into segments for the adjoint)."""
     params, inputs, outputs, stmts = [], [], [], []
+    seg = 0
     for line in open(path):
         line = line.split("#", 1)[0].strip()
         if not line:
@@ -102,12 +105,17 @@ def parse(path):
             inputs = line.split(":", 1)[1].split()
         elif line.startswith("outputs:"):
             outputs = line.split(":", 1)[1].split()
+        elif line.startswith("segment:"):
+            seg += 1 if stmts else 0
         else:
-            stmts.append(ast.parse(line).body[0])
+            stmts.append((seg, ast.parse(line).body[0]))
     low = Lowering(params)
-    for st in stmts:
+    segs = []
+    for sg, st in stmts:
+        n0 = len(low.ops)
         low.expr(st.value, dst=st.targets[0].id)
-    return params, inputs, outputs, low.ops
+        segs += [sg] * (len(low.ops) - n0)
+    return params, inputs, outputs, low.ops, segs
 
 
 C_OP = {"add": "{0} + {1}", "sub": "{0} - {1}", "mul": "{0} * {1}", "div": "{0} / {1}", "neg": "-{0}",
@@ -144,13 +152,13 @@ def tangent_lines(params, inputs, ops):
         elif op == "mul":
             e = f"{d(a)} * {b} + {a} * {d(b)}"
         elif op == "div":
-            e = f"({d(a)} - {dst} * {d(b)}) / {b}"
+            e = f"({d(a)} - {dst} * {d(b)}) * mhd_ad_rcp({b})"
         elif op == "neg":
             e = f"-{d(a)}"
         elif op == "copy":
             e = d(a)
         elif op == "sqrt":
-            e = f"({dst} > 0.0 ? 0.5 * {d(a)} / {dst} : 0.0)"
+            e = f"({dst} > 0.0 ? 0.5 * {d(a)} * mhd_ad_rcp({dst}) : 0.0)"
         elif op == "fabs":
             e = f"({a} > 0.0 ? {d(a)} : ({a} < 0.0 ? -{d(a)} : 0.0))"
         elif op == "tmin":
@@ -163,6 +171,105 @@ def tangent_lines(params, inputs, ops):
             raise ValueError(op)
         out.append(f"    const double {dst}_d = {e};")
     return out
+
+
+def reverse_op(dst, op, args, active, ren):
+    """Adjoint statements of one three-address op; ren maps a primal name to
+    the local (recomputed) copy its value is read from."""
+    out = []
+    a = args[0] if args else None
+    b = args[1] if len(args) > 1 else None
+    t = f"{dst}_b"
+    V = lambda x: ren.get(x, x)
+    acc = []
+    if op == "add":
+        acc = [(a, t), (b, t)]
+    elif op == "sub":
+        acc = [(a, t), (b, f"-{t}")]
+    elif op == "mul":
+        acc = [(a, f"{t} * {V(b)}"), (b, f"{t} * {V(a)}")]
+    elif op == "div":
+        out.append(f"    const double {dst}_r = mhd_ad_rcp({V(b)});")
+        acc = [(a, f"{t} * {dst}_r"), (b, f"-{t} * {V(dst)} * {dst}_r")]
+    elif op == "neg":
+        acc = [(a, f"-{t}")]
+    elif op == "copy":
+        acc = [(a, t)]
+    elif op == "sqrt":
+        acc = [(a, f"({V(dst)} > 0.0 ? 0.5 * {t} * mhd_ad_rcp({V(dst)}) : 0.0)")]
+    elif op == "fabs":
+        acc = [(a, f"({V(a)} > 0.0 ? {t} : ({V(a)} < 0.0 ? -{t} : 0.0))")]
+    elif op in ("tmin", "tmax"):
+        sel = f"({V(b)}) < ({V(a)})" if op == "tmin" else f"({V(a)}) < ({V(b)})"
+        if active(a) and active(b):
+            out.append(f"    if ({sel}) {b}_b += {t}; else {a}_b += {t};")
+        elif active(a):
+            out.append(f"    if (!({sel})) {a}_b += {t};")
+        elif active(b):
+            out.append(f"    if ({sel}) {b}_b += {t};")
+        return out
+    elif op in ("sgn", "ind"):
+        return out
+    else:
+        raise ValueError(op)
+    for v, e in acc:
+        if active(v):
+            out.append(f"    {v}_b += {e};")
+    return out
+
+
+def segmented_adjoint_lines(params, inputs, outputs, ops, segs):
+    """Checkpointed reverse mode: the primal runs once for the values that
+    cross segment boundaries; each segment is then recomputed from its
+    inputs (behind a register barrier, so the recomputation is not merged
+    with the first pass) right before its reverse, which bounds the live set
+    to one segment's intermediates."""
+    cst = set(params)
+    active = lambda a: not (is_lit(a) or a in cst)
+    out = []
+    names = list(inputs) + [dst for dst, _, _ in ops]
+    for n in names:
+        out.append(f"    double {n}_b = 0.0;")
+    for i, v in enumerate(outputs):
+        out.append(f"    {v}_b += out_b[{i}];")
+    for k in sorted(set(segs), reverse=True):
+        idx = [n for n, sg in enumerate(segs) if sg == k]
+        defined = {ops[n][0] for n in idx}
+        used = []
+        for n in idx:
+            for a in ops[n][2]:
+                if active(a) and a not in defined and a not in used and not a.startswith("("):
+                    used.append(a)
+        ren = {}
+        out.append(f"    {{  // segment {k}: recompute, then reverse")
+        for a in used:
+            ren[a] = f"{a}_s{k}"
+            out.append(f"    double {a}_s{k} = {a};")
+            out.append(f"    MHD_AD_BARRIER({a}_s{k});")
+        for n in idx:
+            dst, op, args = ops[n]
+            ren[dst] = f"{dst}_s{k}"
+        for n in idx:
+            dst, op, args = ops[n]
+            if op == "ind":
+                cond = args[0]
+                for nm in sorted(ren, key=len, reverse=True):
+                    cond = re_sub_name(cond, nm, ren[nm])
+                out.append(f"    const double {dst}_s{k} = (double){cond};")
+            else:
+                out.append(f"    const double {dst}_s{k} = {C_OP[op].format(*[ren.get(x, x) for x in args])};")
+        for n in reversed(idx):
+            dst, op, args = ops[n]
+            out += reverse_op(dst, op, args, active, ren)
+        out.append("    }")
+    for i, v in enumerate(inputs):
+        out.append(f"    in_b[{i}] = {v}_b;")
+    return out
+
+
+def re_sub_name(text, name, repl):
+    import re
+    return re.sub(r"\b" + re.escape(name) + r"\b", repl, text)
 
 
 def adjoint_lines(params, inputs, outputs, ops):
@@ -186,13 +293,14 @@ def adjoint_lines(params, inputs, outputs, ops):
         elif op == "mul":
             acc = [(a, f"{t} * {b}"), (b, f"{t} * {a}")]
         elif op == "div":
-            acc = [(a, f"{t} / {b}"), (b, f"-{t} * {dst} / {b}")]
+            out.append(f"    const double {dst}_r = mhd_ad_rcp({b});")
+            acc = [(a, f"{t} * {dst}_r"), (b, f"-{t} * {dst} * {dst}_r")]
         elif op == "neg":
             acc = [(a, f"-{t}")]
         elif op == "copy":
             acc = [(a, t)]
         elif op == "sqrt":
-            acc = [(a, f"({dst} > 0.0 ? 0.5 * {t} / {dst} : 0.0)")]
+            acc = [(a, f"({dst} > 0.0 ? 0.5 * {t} * mhd_ad_rcp({dst}) : 0.0)")]
         elif op == "fabs":
             acc = [(a, f"({a} > 0.0 ? {t} : ({a} < 0.0 ? -{t} : 0.0))")]
         elif op == "tmin":
@@ -224,7 +332,7 @@ def adjoint_lines(params, inputs, outputs, ops):
 
 
 def emit(name, path):
-    params, inputs, outputs, ops = parse(path)
+    params, inputs, outputs, ops, segs = parse(path)
     src = os.path.basename(path)
     n_in, n_out, n_par = len(inputs), len(outputs), len(params)
     lines = [f"// ---- {name}: from {src}",
@@ -247,7 +355,10 @@ def emit(name, path):
               f"MHD_HD inline void {name}_adj(const double *in, const double *par, const double *out_b,"
               f" double *in_b)", "{"]
     lines += primal_lines(params, inputs, ops)
-    lines += adjoint_lines(params, inputs, outputs, ops)
+    if len(set(segs)) > 1:
+        lines += segmented_adjoint_lines(params, inputs, outputs, ops, segs)
+    else:
+        lines += adjoint_lines(params, inputs, outputs, ops)
     lines += ["}", ""]
     return lines
 
@@ -277,6 +388,31 @@ def main(argv):
             "#endif",
             "",
             "namespace mhd_ad {",
+            "",
+            "// Reciprocal for the derivative factors of the tangent and adjoint",
+            "// passes (the primal keeps IEEE division, so branch decisions replay",
+            "// the forward step exactly): on the GPU the hardware estimate refined",
+            "// by two Newton steps (within an ulp or two); on the host 1/x.",
+            "// Register barrier: the compiler must treat x as a new value, so a",
+            "// segment recomputed for the reverse pass is not merged with (and does",
+            "// not keep alive) the first primal pass.",
+            "#if defined(__HIP_DEVICE_COMPILE__)",
+            "#define MHD_AD_BARRIER(x) asm volatile(\"\" : \"+v\"(x))",
+            "#else",
+            "#define MHD_AD_BARRIER(x) ((void)0)",
+            "#endif",
+            "",
+            "MHD_HD inline double mhd_ad_rcp(double x)",
+            "{",
+            "#if defined(__HIP_DEVICE_COMPILE__)",
+            "    double r = __builtin_amdgcn_rcp(x);",
+            "    r = fma(fma(-x, r, 1.0), r, r);",
+            "    r = fma(fma(-x, r, 1.0), r, r);",
+            "    return r;",
+            "#else",
+            "    return 1.0 / x;",
+            "#endif",
+            "}",
             ""]
     for spec in argv[2:]:
         path, _, name = spec.partition(":")

@@ -551,6 +551,8 @@ struct mhd_adjoint {
     int nsteps = 0, stride = 0, nckpt = 0;
     bool forward_done = false;
     double *tpool = nullptr;      // tangent buffers (allocated on first use)
+    double *stage_all = nullptr;  // AoS staging of every run at once (allocated on first use)
+    std::vector<size_t> aos_off;  // offset of each run's AoS state in stage_all
     std::string err;
 };
 
@@ -765,6 +767,7 @@ int mhd_adjoint_destroy(mhd_adjoint *a)
     (void)hipFree(a->ckpt);
     (void)hipFree(a->dts);
     (void)hipFree(a->tpool);
+    (void)hipFree(a->stage_all);
     mhd_batch_destroy(a->fwd);
     delete a;
     return MHD_OK;
@@ -1035,6 +1038,56 @@ int mhd_adjoint_get_tangent(mhd_adjoint *a, int run, double *host_dUN)
     AHIP(a, hipStreamSynchronize(stream_of(a)));
     return MHD_OK;
 }
+
+
+// ---- whole-batch transfers: one host copy and one synchronisation for
+// every run (states concatenated in run order, AoS as mhd_set_state)
+static int ensure_stage_all(mhd_adjoint *a)
+{
+    if (a->stage_all) return MHD_OK;
+    a->aos_off.assign(a->nruns + 1, 0);
+    for (int r = 0; r < a->nruns; ++r)
+        a->aos_off[r + 1] = a->aos_off[r] + (size_t)8 * a->desc[r].g.nx * a->desc[r].g.ny;
+    AHIP(a, hipMalloc(&a->stage_all, sizeof(double) * a->aos_off[a->nruns]));
+    return MHD_OK;
+}
+
+static int upload_all(mhd_adjoint *a, const double *host, int which)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (!host) return afail(a, MHD_ERR_INVALID, "NULL host buffer");
+    int rc;
+    if ((rc = ensure_stage_all(a))) return rc;
+    AHIP(a, hipMemcpyAsync(a->stage_all, host, sizeof(double) * a->aos_off[a->nruns], hipMemcpyHostToDevice,
+                           stream_of(a)));
+    for (int r = 0; r < a->nruns; ++r) {
+        const AdjDesc &d = a->desc[r];
+        AHIP(a, mhd::launch_unpack(a->stage_all + a->aos_off[r], which == kU0 ? d.u0 : d.lam, d.g, stream_of(a)));
+    }
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+static int download_all(mhd_adjoint *a, double *host, int which)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (!host) return afail(a, MHD_ERR_INVALID, "NULL host buffer");
+    int rc;
+    if ((rc = ensure_stage_all(a))) return rc;
+    for (int r = 0; r < a->nruns; ++r) {
+        const AdjDesc &d = a->desc[r];
+        AHIP(a, mhd::launch_pack(which == kU ? d.u : d.lam, a->stage_all + a->aos_off[r], d.g, stream_of(a)));
+    }
+    AHIP(a, hipMemcpyAsync(host, a->stage_all, sizeof(double) * a->aos_off[a->nruns], hipMemcpyDeviceToHost,
+                           stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+int mhd_adjoint_set_states(mhd_adjoint *a, const double *host_all) { return upload_all(a, host_all, kU0); }
+int mhd_adjoint_set_seeds(mhd_adjoint *a, const double *host_all) { return upload_all(a, host_all, kLam); }
+int mhd_adjoint_get_states(mhd_adjoint *a, double *host_all) { return download_all(a, host_all, kU); }
+int mhd_adjoint_get_grad_states(mhd_adjoint *a, double *host_all) { return download_all(a, host_all, kLam); }
 
 int mhd_adjoint_synchronize(mhd_adjoint *a)
 {

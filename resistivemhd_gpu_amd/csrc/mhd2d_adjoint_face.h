@@ -145,6 +145,17 @@ MHD_HD inline void flux_full_adj(const S &U, const E &eta, int dir, int i, int j
         resist_y_adj(rin, rpar, d_b, rin_b);
     }
     face_flux_adj(in, par, ff_b, in_b);
+    // the resistive terms' Bz and By (x faces) / Bz and Bx (y faces) inputs
+    // belong to the same four cells: fold them in before scattering
+    for (int c = 0; c < 4; ++c) {
+        if (dir == 0) {
+            in_b[8 * c + 6] += rin_b[c];
+            in_b[8 * c + 5] += rin_b[4 + c];
+        } else {
+            in_b[8 * c + 5] += rin_b[c];       // rotated slot of Bz
+            in_b[8 * c + 6] += rin_b[4 + c];   // rotated slot of Bx
+        }
+    }
     for (int c = 0; c < 4; ++c) {
         int ci, cj;
         stencil_cell(dir, i, j, c, ci, cj);
@@ -153,7 +164,7 @@ MHD_HD inline void flux_full_adj(const S &U, const E &eta, int dir, int i, int j
             if (b != 0.0) add(ci, cj, dir == 0 ? v : rot_y(v), b);
         }
     }
-    for (int s = 0; s < 14; ++s) {
+    for (int s = 8; s < 14; ++s) {
         if (rin_b[s] == 0.0) continue;
         int ci, cj, v;
         resist_slot(dir, i, j, s, ci, cj, v);

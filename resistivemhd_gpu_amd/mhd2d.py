@@ -39,7 +39,8 @@ EXPORTS = (
     "mhd_adjoint_set_state", "mhd_adjoint_set_dts", "mhd_adjoint_get_dts", "mhd_adjoint_forward",
     "mhd_adjoint_get_state", "mhd_adjoint_set_seed", "mhd_adjoint_backward", "mhd_adjoint_get_grad_state",
     "mhd_adjoint_get_grad_eta", "mhd_adjoint_synchronize", "mhd_adjoint_set_direction", "mhd_adjoint_tangent",
-    "mhd_adjoint_get_tangent",
+    "mhd_adjoint_get_tangent", "mhd_adjoint_set_states", "mhd_adjoint_get_states", "mhd_adjoint_set_seeds",
+    "mhd_adjoint_get_grad_states",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -163,6 +164,10 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_set_direction": (I, [P, I, D, D]),
         "mhd_adjoint_tangent": (I, [P, I]),
         "mhd_adjoint_get_tangent": (I, [P, I, D]),
+        "mhd_adjoint_set_states": (I, [P, D]),
+        "mhd_adjoint_get_states": (I, [P, D]),
+        "mhd_adjoint_set_seeds": (I, [P, D]),
+        "mhd_adjoint_get_grad_states": (I, [P, D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -646,3 +651,41 @@ class ResistiveMHD2DAdjoint:
         out = np.empty(self.shape(run))
         self._ok(self._lib.mhd_adjoint_get_tangent(self._h, run, _dptr(out)))
         return out
+
+    # -- whole-batch transfers (lists of per-run arrays) --------------------
+    def _concat(self, arrays):
+        if len(arrays) != len(self.cfgs):
+            raise ValueError("one array per run")
+        for r, x in enumerate(arrays):
+            if np.shape(x) != self.shape(r):
+                raise ValueError(f"run {r}: shape {np.shape(x)} != {self.shape(r)}")
+        return np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).ravel() for x in arrays]))
+
+    def _split(self, flat):
+        out, o = [], 0
+        for r in range(len(self.cfgs)):
+            n = int(np.prod(self.shape(r)))
+            out.append(flat[o:o + n].reshape(self.shape(r)))
+            o += n
+        return out
+
+    def _total(self):
+        return sum(int(np.prod(self.shape(r))) for r in range(len(self.cfgs)))
+
+    def setU0_all(self, arrays):
+        flat = self._concat(arrays)
+        self._ok(self._lib.mhd_adjoint_set_states(self._h, _dptr(flat)))
+
+    def getU_all(self):
+        flat = np.empty(self._total())
+        self._ok(self._lib.mhd_adjoint_get_states(self._h, _dptr(flat)))
+        return self._split(flat)
+
+    def set_seed_all(self, arrays):
+        flat = self._concat(arrays)
+        self._ok(self._lib.mhd_adjoint_set_seeds(self._h, _dptr(flat)))
+
+    def grad_state_all(self):
+        flat = np.empty(self._total())
+        self._ok(self._lib.mhd_adjoint_get_grad_states(self._h, _dptr(flat)))
+        return self._split(flat)
