@@ -121,6 +121,14 @@ int mhd_check_crashed(mhd_handle *h, int *crashed);
 int mhd_save(mhd_handle *h, const char *dir, const char *name, int step);
 int mhd_load(mhd_handle *h, const char *path);
 
+/* Per-rank files of the MPI library's ResistiveMHD2D::save
+ * (ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU/resistiveMHD2D.cu:457-486):
+ * {dir}/{name}_{step}_{rank}.bin with the interior cells only (ghosts left
+ * out), i-major, 8 doubles per cell.  mhd_load_subdomain reads such a file
+ * into the interior and keeps the ghost cells; apply the boundary after. */
+int mhd_save_subdomain(mhd_handle *h, const char *dir, const char *name, int step, int rank);
+int mhd_load_subdomain(mhd_handle *h, const char *path);
+
 int mhd_synchronize(mhd_handle *h);
 
 /* Initial-state setups (host only, no GPU needed): fill an AoS buffer of

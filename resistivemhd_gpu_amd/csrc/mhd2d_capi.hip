@@ -19,6 +19,7 @@
 // The self-exchange halo (M/mpi.cu:104-188 with every neighbour = self) is
 // the periodic fill in x then y.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -555,6 +556,61 @@ int mhd_load(mhd_handle *h, const char *path)
     std::fclose(f);
     if (n != buf.size() || extra)
         return fail(h, MHD_ERR_IO, std::string(path) + ": size does not match nx*ny*8 doubles");
+    return mhd_set_state(h, buf.data());
+}
+
+// Per-rank file of the MPI library's ResistiveMHD2D::save
+// (ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU/resistiveMHD2D.cu:457-486):
+// {dir}/{name}_{step}_{rank}.bin holding the interior cells only, i-major,
+// 8 doubles per cell.
+int mhd_save_subdomain(mhd_handle *h, const char *dir, const char *name, int step, int rank)
+{
+    if (!h || !dir || !name) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    const int nx = h->cfg.nx, ny = h->cfg.ny, g = MHD_GHOST;
+    std::vector<double> buf, out;
+    try {
+        buf.resize(8 * (size_t)nx * ny);
+        out.reserve(8 * (size_t)(nx - 2 * g) * (ny - 2 * g));
+    } catch (...) {
+        return fail(h, MHD_ERR_NOMEM, "host allocation failed");
+    }
+    int rc = mhd_get_state(h, buf.data());
+    if (rc) return rc;
+    for (int i = g; i < nx - g; ++i)
+        out.insert(out.end(), buf.begin() + 8 * ((size_t)i * ny + g), buf.begin() + 8 * ((size_t)i * ny + ny - g));
+    const std::string path = std::string(dir) + "/" + name + "_" + std::to_string(step) + "_" + std::to_string(rank) +
+                             ".bin";
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return fail(h, MHD_ERR_IO, "cannot open " + path + ": " + std::strerror(errno));
+    const size_t n = std::fwrite(out.data(), sizeof(double), out.size(), f);
+    const int closed = std::fclose(f);
+    if (n != out.size() || closed != 0) return fail(h, MHD_ERR_IO, "short write to " + path);
+    return MHD_OK;
+}
+
+int mhd_load_subdomain(mhd_handle *h, const char *path)
+{
+    if (!h || !path) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    const int nx = h->cfg.nx, ny = h->cfg.ny, g = MHD_GHOST;
+    std::vector<double> buf, in;
+    try {
+        buf.resize(8 * (size_t)nx * ny);
+        in.resize(8 * (size_t)(nx - 2 * g) * (ny - 2 * g));
+    } catch (...) {
+        return fail(h, MHD_ERR_NOMEM, "host allocation failed");
+    }
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(h, MHD_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+    const size_t n = std::fread(in.data(), sizeof(double), in.size(), f);
+    const bool extra = std::fgetc(f) != EOF;
+    std::fclose(f);
+    if (n != in.size() || extra)
+        return fail(h, MHD_ERR_IO, std::string(path) + ": size does not match the interior's cells * 8 doubles");
+    int rc = mhd_get_state(h, buf.data());
+    if (rc) return rc;
+    const size_t row = 8 * (size_t)(ny - 2 * g);
+    for (int i = g; i < nx - g; ++i)
+        std::copy(in.begin() + (i - g) * row, in.begin() + (i - g + 1) * row, buf.begin() + 8 * ((size_t)i * ny + g));
     return mhd_set_state(h, buf.data());
 }
 

@@ -40,7 +40,7 @@ EXPORTS = (
     "mhd_adjoint_get_state", "mhd_adjoint_set_seed", "mhd_adjoint_backward", "mhd_adjoint_get_grad_state",
     "mhd_adjoint_get_grad_eta", "mhd_adjoint_synchronize", "mhd_adjoint_set_direction", "mhd_adjoint_tangent",
     "mhd_adjoint_get_tangent", "mhd_adjoint_set_states", "mhd_adjoint_get_states", "mhd_adjoint_set_seeds",
-    "mhd_adjoint_get_grad_states",
+    "mhd_adjoint_get_grad_states", "mhd_save_subdomain", "mhd_load_subdomain",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -168,6 +168,8 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_get_states": (I, [P, D]),
         "mhd_adjoint_set_seeds": (I, [P, D]),
         "mhd_adjoint_get_grad_states": (I, [P, D]),
+        "mhd_save_subdomain": (I, [P, ctypes.c_char_p, ctypes.c_char_p, I, I]),
+        "mhd_load_subdomain": (I, [P, ctypes.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -320,6 +322,15 @@ class ResistiveMHD2D:
     def save(self, directoryname, filenameWithoutStep, step):
         self._ok(self._lib.mhd_save(self._h, os.fsencode(directoryname), os.fsencode(filenameWithoutStep),
                                     int(step)))
+
+    def saveSubdomain(self, directoryname, filenameWithoutStep, step, rank=0):
+        """The MPI library's per-rank save: interior cells only,
+        {dir}/{name}_{step}_{rank}.bin."""
+        self._ok(self._lib.mhd_save_subdomain(self._h, directoryname.encode(), filenameWithoutStep.encode(),
+                                              int(step), int(rank)))
+
+    def loadSubdomain(self, path):
+        self._ok(self._lib.mhd_load_subdomain(self._h, str(path).encode()))
 
     def load(self, path):
         self._ok(self._lib.mhd_load(self._h, os.fsencode(path)))
