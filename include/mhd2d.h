@@ -292,11 +292,19 @@ int mhd_adjoint_set_direction(mhd_adjoint *a, int run, const double *host_dU0, c
 int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps);
 int mhd_adjoint_get_tangent(mhd_adjoint *a, int run, double *host_dUN);
 /* Whole-batch transfers (one copy and one synchronisation): every run's AoS
- * state concatenated in run order. */
+ * state (or eta plane, nx*ny) concatenated in run order.  The _device forms
+ * take a device pointer on the handle's GPU (e.g. a torch tensor's storage)
+ * and copy nothing across PCIe; the caller orders its own stream first. */
 int mhd_adjoint_set_states(mhd_adjoint *a, const double *host_all);        /* all U0 */
 int mhd_adjoint_get_states(mhd_adjoint *a, double *host_all);              /* all U_N */
 int mhd_adjoint_set_seeds(mhd_adjoint *a, const double *host_all);         /* all dL/dU_N */
 int mhd_adjoint_get_grad_states(mhd_adjoint *a, double *host_all);         /* all dL/dU0 */
+int mhd_adjoint_get_grad_etas(mhd_adjoint *a, double *host_all);           /* all dL/deta */
+int mhd_adjoint_set_states_device(mhd_adjoint *a, const double *dev_all);
+int mhd_adjoint_get_states_device(mhd_adjoint *a, double *dev_all);
+int mhd_adjoint_set_seeds_device(mhd_adjoint *a, const double *dev_all);
+int mhd_adjoint_get_grad_states_device(mhd_adjoint *a, double *dev_all);
+int mhd_adjoint_get_grad_etas_device(mhd_adjoint *a, double *dev_all);
 int mhd_adjoint_synchronize(mhd_adjoint *a);
 
 #ifdef __cplusplus

@@ -95,6 +95,7 @@ def parse(path):
     (`segment:` lines split a listing into segments for the adjoint)."""
     params, inputs, outputs, stmts = [], [], [], []
     seg = 0
+    guard = None
     for line in open(path):
         line = line.split("#", 1)[0].strip()
         if not line:
@@ -107,14 +108,19 @@ def parse(path):
             outputs = line.split(":", 1)[1].split()
         elif line.startswith("segment:"):
             seg += 1 if stmts else 0
+        elif line.startswith("guard:"):
+            g = line.split(":", 1)[1].strip()
+            guard = None if g == "none" else g
         else:
-            stmts.append((seg, ast.parse(line).body[0]))
+            stmts.append((seg, guard, ast.parse(line).body[0]))
     low = Lowering(params)
-    segs = []
-    for sg, st in stmts:
+    segs, guards = [], []
+    for sg, gd, st in stmts:
         n0 = len(low.ops)
         low.expr(st.value, dst=st.targets[0].id)
         segs += [sg] * (len(low.ops) - n0)
+        guards += [gd] * (len(low.ops) - n0)
+    parse.guards = guards
     return params, inputs, outputs, low.ops, segs
 
 
@@ -128,20 +134,62 @@ def is_lit(a):
     return a[0].isdigit() or a[0] == "-"
 
 
-def primal_lines(params, inputs, ops):
-    out = [f"    const double {p} = par[{i}];" for i, p in enumerate(params)]
-    out += [f"    const double {v} = in[{i}];" for i, v in enumerate(inputs)]
-    for dst, op, args in ops:
-        out.append(f"    const double {dst} = {C_OP[op].format(*args)};")
+def groups(guards):
+    """Runs of consecutive ops with the same guard: (guard, [op indices])."""
+    out = []
+    for n, g in enumerate(guards):
+        if out and out[-1][0] == g:
+            out[-1][1].append(n)
+        else:
+            out.append((g, [n]))
     return out
 
 
-def tangent_lines(params, inputs, ops):
+def primal_lines(params, inputs, ops, guards=None):
+    """Primal statements.  With guards (tangent and adjoint only), an op
+    under a guard is evaluated only where the guard holds (its result feeds
+    nothing else there); the plain primal evaluates every op."""
+    out = [f"    const double {p} = par[{i}];" for i, p in enumerate(params)]
+    out += [f"    const double {v} = in[{i}];" for i, v in enumerate(inputs)]
+    if guards is None:
+        guards = [None] * len(ops)
+    for g, idx in groups(guards):
+        if g is None:
+            for n in idx:
+                dst, op, args = ops[n]
+                out.append(f"    const double {dst} = {C_OP[op].format(*args)};")
+        else:
+            out += [f"    double {ops[n][0]} = 0.0;" for n in idx]
+            out.append(f"    if ({g}) {{")
+            for n in idx:
+                dst, op, args = ops[n]
+                out.append(f"    {dst} = {C_OP[op].format(*args)};")
+            out.append("    }")
+    return out
+
+
+def tangent_lines(params, inputs, ops, guards=None):
     cst = set(params)
     d = lambda a: "0.0" if (is_lit(a) or a in cst) else f"{a}_d"
     out = []
     for i, v in enumerate(inputs):
         out.append(f"    const double {v}_d = in_d[{i}];")
+    if guards is None:
+        guards = [None] * len(ops)
+    for g, idx in groups(guards):
+        body = tangent_ops([ops[n] for n in idx], d)
+        if g is None:
+            out += [f"    const double {x}" for x in body]
+        else:
+            out += [f"    double {ops[n][0]}_d = 0.0;" for n in idx]
+            out.append(f"    if ({g}) {{")
+            out += [f"    {x}" for x in body]
+            out.append("    }")
+    return out
+
+
+def tangent_ops(ops, d):
+    out = []
     for dst, op, args in ops:
         a = args[0] if args else None
         b = args[1] if len(args) > 1 else None
@@ -169,7 +217,7 @@ def tangent_lines(params, inputs, ops):
             e = "0.0"
         else:
             raise ValueError(op)
-        out.append(f"    const double {dst}_d = {e};")
+        out.append(f"{dst}_d = {e};")
     return out
 
 
@@ -272,7 +320,7 @@ def re_sub_name(text, name, repl):
     return re.sub(r"\b" + re.escape(name) + r"\b", repl, text)
 
 
-def adjoint_lines(params, inputs, outputs, ops):
+def adjoint_lines(params, inputs, outputs, ops, guards=None):
     cst = set(params)
     active = lambda a: not (is_lit(a) or a in cst)
     out = []
@@ -281,51 +329,19 @@ def adjoint_lines(params, inputs, outputs, ops):
         out.append(f"    double {n}_b = 0.0;")
     for i, v in enumerate(outputs):
         out.append(f"    {v}_b += out_b[{i}];")
-    for dst, op, args in reversed(ops):
-        a = args[0] if args else None
-        b = args[1] if len(args) > 1 else None
-        t = f"{dst}_b"
-        acc = []
-        if op == "add":
-            acc = [(a, t), (b, t)]
-        elif op == "sub":
-            acc = [(a, t), (b, f"-{t}")]
-        elif op == "mul":
-            acc = [(a, f"{t} * {b}"), (b, f"{t} * {a}")]
-        elif op == "div":
-            out.append(f"    const double {dst}_r = mhd_ad_rcp({b});")
-            acc = [(a, f"{t} * {dst}_r"), (b, f"-{t} * {dst} * {dst}_r")]
-        elif op == "neg":
-            acc = [(a, f"-{t}")]
-        elif op == "copy":
-            acc = [(a, t)]
-        elif op == "sqrt":
-            acc = [(a, f"({dst} > 0.0 ? 0.5 * {t} * mhd_ad_rcp({dst}) : 0.0)")]
-        elif op == "fabs":
-            acc = [(a, f"({a} > 0.0 ? {t} : ({a} < 0.0 ? -{t} : 0.0))")]
-        elif op == "tmin":
-            if active(a) and active(b):
-                out.append(f"    if (({b}) < ({a})) {b}_b += {t}; else {a}_b += {t};")
-            elif active(a):
-                out.append(f"    if (!(({b}) < ({a}))) {a}_b += {t};")
-            elif active(b):
-                out.append(f"    if (({b}) < ({a})) {b}_b += {t};")
-            continue
-        elif op == "tmax":
-            if active(a) and active(b):
-                out.append(f"    if (({a}) < ({b})) {b}_b += {t}; else {a}_b += {t};")
-            elif active(a):
-                out.append(f"    if (!(({a}) < ({b}))) {a}_b += {t};")
-            elif active(b):
-                out.append(f"    if (({a}) < ({b})) {b}_b += {t};")
-            continue
-        elif op in ("sgn", "ind"):
-            continue
+    if guards is None:
+        guards = [None] * len(ops)
+    for g, idx in reversed(groups(guards)):
+        body = []
+        for n in reversed(idx):
+            dst, op, args = ops[n]
+            body += reverse_op(dst, op, args, active, {})
+        if g is None:
+            out += body
         else:
-            raise ValueError(op)
-        for v, e in acc:
-            if active(v):
-                out.append(f"    {v}_b += {e};")
+            out.append(f"    if ({g}) {{")
+            out += body
+            out.append("    }")
     for i, v in enumerate(inputs):
         out.append(f"    in_b[{i}] = {v}_b;")
     return out
@@ -333,6 +349,7 @@ def adjoint_lines(params, inputs, outputs, ops):
 
 def emit(name, path):
     params, inputs, outputs, ops, segs = parse(path)
+    guards = parse.guards
     src = os.path.basename(path)
     n_in, n_out, n_par = len(inputs), len(outputs), len(params)
     lines = [f"// ---- {name}: from {src}",
@@ -347,6 +364,8 @@ def emit(name, path):
     lines += ["}", "",
               f"MHD_HD inline void {name}_tlm(const double *in, const double *in_d, const double *par, double *out,"
               f" double *out_d)", "{"]
+    # (the tangent keeps every candidate: guarding it raised its register
+    # pressure on gfx950 without saving time)
     lines += primal_lines(params, inputs, ops)
     lines += tangent_lines(params, inputs, ops)
     lines += [f"    out[{i}] = {v};" for i, v in enumerate(outputs)]
@@ -354,11 +373,11 @@ def emit(name, path):
     lines += ["}", "",
               f"MHD_HD inline void {name}_adj(const double *in, const double *par, const double *out_b,"
               f" double *in_b)", "{"]
-    lines += primal_lines(params, inputs, ops)
+    lines += primal_lines(params, inputs, ops, guards)
     if len(set(segs)) > 1:
         lines += segmented_adjoint_lines(params, inputs, outputs, ops, segs)
     else:
-        lines += adjoint_lines(params, inputs, outputs, ops)
+        lines += adjoint_lines(params, inputs, outputs, ops, guards)
     lines += ["}", ""]
     return lines
 

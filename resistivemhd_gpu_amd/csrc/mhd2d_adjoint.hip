@@ -1052,42 +1052,82 @@ static int ensure_stage_all(mhd_adjoint *a)
     return MHD_OK;
 }
 
-static int upload_all(mhd_adjoint *a, const double *host, int which)
+// device: the buffer is device memory (for instance a torch tensor on the
+// same GPU): no host copy; the library's stream waits for nothing else, so
+// the caller synchronises its own stream before the call.
+static int upload_all(mhd_adjoint *a, const double *src, int which, bool device)
 {
     if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
-    if (!host) return afail(a, MHD_ERR_INVALID, "NULL host buffer");
+    if (!src) return afail(a, MHD_ERR_INVALID, "NULL buffer");
     int rc;
     if ((rc = ensure_stage_all(a))) return rc;
-    AHIP(a, hipMemcpyAsync(a->stage_all, host, sizeof(double) * a->aos_off[a->nruns], hipMemcpyHostToDevice,
-                           stream_of(a)));
+    const double *stage = src;
+    if (!device) {
+        AHIP(a, hipMemcpyAsync(a->stage_all, src, sizeof(double) * a->aos_off[a->nruns], hipMemcpyHostToDevice,
+                               stream_of(a)));
+        stage = a->stage_all;
+    }
     for (int r = 0; r < a->nruns; ++r) {
         const AdjDesc &d = a->desc[r];
-        AHIP(a, mhd::launch_unpack(a->stage_all + a->aos_off[r], which == kU0 ? d.u0 : d.lam, d.g, stream_of(a)));
+        AHIP(a, mhd::launch_unpack(stage + a->aos_off[r], which == kU0 ? d.u0 : d.lam, d.g, stream_of(a)));
     }
     AHIP(a, hipStreamSynchronize(stream_of(a)));
     return MHD_OK;
 }
 
-static int download_all(mhd_adjoint *a, double *host, int which)
+static int download_all(mhd_adjoint *a, double *dst, int which, bool device)
 {
     if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
-    if (!host) return afail(a, MHD_ERR_INVALID, "NULL host buffer");
+    if (!dst) return afail(a, MHD_ERR_INVALID, "NULL buffer");
     int rc;
     if ((rc = ensure_stage_all(a))) return rc;
+    double *stage = device ? dst : a->stage_all;
     for (int r = 0; r < a->nruns; ++r) {
         const AdjDesc &d = a->desc[r];
-        AHIP(a, mhd::launch_pack(which == kU ? d.u : d.lam, a->stage_all + a->aos_off[r], d.g, stream_of(a)));
+        AHIP(a, mhd::launch_pack(which == kU ? d.u : d.lam, stage + a->aos_off[r], d.g, stream_of(a)));
     }
-    AHIP(a, hipMemcpyAsync(host, a->stage_all, sizeof(double) * a->aos_off[a->nruns], hipMemcpyDeviceToHost,
-                           stream_of(a)));
+    if (!device)
+        AHIP(a, hipMemcpyAsync(dst, a->stage_all, sizeof(double) * a->aos_off[a->nruns], hipMemcpyDeviceToHost,
+                               stream_of(a)));
     AHIP(a, hipStreamSynchronize(stream_of(a)));
     return MHD_OK;
 }
 
-int mhd_adjoint_set_states(mhd_adjoint *a, const double *host_all) { return upload_all(a, host_all, kU0); }
-int mhd_adjoint_set_seeds(mhd_adjoint *a, const double *host_all) { return upload_all(a, host_all, kLam); }
-int mhd_adjoint_get_states(mhd_adjoint *a, double *host_all) { return download_all(a, host_all, kU); }
-int mhd_adjoint_get_grad_states(mhd_adjoint *a, double *host_all) { return download_all(a, host_all, kLam); }
+int mhd_adjoint_set_states(mhd_adjoint *a, const double *host_all) { return upload_all(a, host_all, kU0, false); }
+int mhd_adjoint_set_seeds(mhd_adjoint *a, const double *host_all) { return upload_all(a, host_all, kLam, false); }
+int mhd_adjoint_get_states(mhd_adjoint *a, double *host_all) { return download_all(a, host_all, kU, false); }
+int mhd_adjoint_get_grad_states(mhd_adjoint *a, double *host_all) { return download_all(a, host_all, kLam, false); }
+int mhd_adjoint_set_states_device(mhd_adjoint *a, const double *dev_all) { return upload_all(a, dev_all, kU0, true); }
+int mhd_adjoint_set_seeds_device(mhd_adjoint *a, const double *dev_all) { return upload_all(a, dev_all, kLam, true); }
+int mhd_adjoint_get_states_device(mhd_adjoint *a, double *dev_all) { return download_all(a, dev_all, kU, true); }
+int mhd_adjoint_get_grad_states_device(mhd_adjoint *a, double *dev_all)
+{
+    return download_all(a, dev_all, kLam, true);
+}
+
+// every run's dL/deta (nx*ny per run, concatenated), host or device
+static int grad_eta_all(mhd_adjoint *a, double *dst, bool device)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (!dst) return afail(a, MHD_ERR_INVALID, "NULL buffer");
+    int rc;
+    if ((rc = ensure_stage_all(a))) return rc;
+    double *stage = device ? dst : a->stage_all;
+    size_t off = 0;
+    for (int r = 0; r < a->nruns; ++r) {
+        const AdjDesc &d = a->desc[r];
+        hipLaunchKernelGGL(pack_plane_kernel, dim3(1024), dim3(256), 0, stream_of(a), d.geta, stage + off, d.g);
+        off += (size_t)d.g.nx * d.g.ny;
+    }
+    AHIP(a, hipGetLastError());
+    if (!device)
+        AHIP(a, hipMemcpyAsync(dst, a->stage_all, sizeof(double) * off, hipMemcpyDeviceToHost, stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    return MHD_OK;
+}
+
+int mhd_adjoint_get_grad_etas(mhd_adjoint *a, double *host_all) { return grad_eta_all(a, host_all, false); }
+int mhd_adjoint_get_grad_etas_device(mhd_adjoint *a, double *dev_all) { return grad_eta_all(a, dev_all, true); }
 
 int mhd_adjoint_synchronize(mhd_adjoint *a)
 {

@@ -40,7 +40,9 @@ EXPORTS = (
     "mhd_adjoint_get_state", "mhd_adjoint_set_seed", "mhd_adjoint_backward", "mhd_adjoint_get_grad_state",
     "mhd_adjoint_get_grad_eta", "mhd_adjoint_synchronize", "mhd_adjoint_set_direction", "mhd_adjoint_tangent",
     "mhd_adjoint_get_tangent", "mhd_adjoint_set_states", "mhd_adjoint_get_states", "mhd_adjoint_set_seeds",
-    "mhd_adjoint_get_grad_states", "mhd_save_subdomain", "mhd_load_subdomain",
+    "mhd_adjoint_get_grad_states", "mhd_save_subdomain", "mhd_load_subdomain", "mhd_adjoint_get_grad_etas",
+    "mhd_adjoint_set_states_device", "mhd_adjoint_get_states_device", "mhd_adjoint_set_seeds_device",
+    "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -170,6 +172,12 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_get_grad_states": (I, [P, D]),
         "mhd_save_subdomain": (I, [P, ctypes.c_char_p, ctypes.c_char_p, I, I]),
         "mhd_load_subdomain": (I, [P, ctypes.c_char_p]),
+        "mhd_adjoint_get_grad_etas": (I, [P, D]),
+        "mhd_adjoint_set_states_device": (I, [P, P]),
+        "mhd_adjoint_get_states_device": (I, [P, P]),
+        "mhd_adjoint_set_seeds_device": (I, [P, P]),
+        "mhd_adjoint_get_grad_states_device": (I, [P, P]),
+        "mhd_adjoint_get_grad_etas_device": (I, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -700,3 +708,39 @@ class ResistiveMHD2DAdjoint:
         flat = np.empty(self._total())
         self._ok(self._lib.mhd_adjoint_get_grad_states(self._h, _dptr(flat)))
         return self._split(flat)
+
+    def grad_eta_all(self):
+        n = [self.cfgs[r].nx * self.cfgs[r].ny for r in range(len(self.cfgs))]
+        flat = np.empty(sum(n))
+        self._ok(self._lib.mhd_adjoint_get_grad_etas(self._h, _dptr(flat)))
+        out, o = [], 0
+        for r, k in enumerate(n):
+            out.append(flat[o:o + k].reshape(self.cfgs[r].nx, self.cfgs[r].ny))
+            o += k
+        return out
+
+    # -- device-side transfers: flat float64 torch tensors on the GPU -------
+    def _dev(self, t, numel):
+        import torch
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+            raise TypeError("expects a contiguous float64 tensor on the GPU")
+        if t.numel() != numel:
+            raise ValueError(f"{t.numel()} elements, expected {numel}")
+        torch.cuda.current_stream().synchronize()
+        return ctypes.c_void_p(t.data_ptr())
+
+    def setU0_all_device(self, t):
+        self._ok(self._lib.mhd_adjoint_set_states_device(self._h, self._dev(t, self._total())))
+
+    def getU_all_device(self, t):
+        self._ok(self._lib.mhd_adjoint_get_states_device(self._h, self._dev(t, self._total())))
+
+    def set_seed_all_device(self, t):
+        self._ok(self._lib.mhd_adjoint_set_seeds_device(self._h, self._dev(t, self._total())))
+
+    def grad_state_all_device(self, t):
+        self._ok(self._lib.mhd_adjoint_get_grad_states_device(self._h, self._dev(t, self._total())))
+
+    def grad_eta_all_device(self, t):
+        n = sum(c.nx * c.ny for c in self.cfgs)
+        self._ok(self._lib.mhd_adjoint_get_grad_etas_device(self._h, self._dev(t, n)))

@@ -26,27 +26,41 @@ class MHDRollout(torch.autograd.Function):
         if len(cfgs) != B or eta.shape[0] != B:
             raise ValueError("one configuration and one eta field per run")
         adj = ResistiveMHD2DAdjoint(cfgs)
-        U0n = U0.detach().to("cpu").numpy()
         etan = eta.detach().to("cpu").numpy()
         for r in range(B):
-            adj.setU0(r, U0n[r])
             adj.setEtaField(r, etan[r])
-        adj.forward(int(nsteps), int(stride))
-        UN = np.stack([adj.getU(r) for r in range(B)])
+        if U0.is_cuda:
+            # states move between torch's memory and the library's on the GPU
+            adj.setU0_all_device(U0.detach().contiguous().reshape(-1))
+            adj.forward(int(nsteps), int(stride))
+            UN = torch.empty_like(U0)
+            adj.getU_all_device(UN.view(-1))
+        else:
+            U0n = U0.detach().contiguous().numpy()
+            adj.setU0_all([U0n[r] for r in range(B)])
+            adj.forward(int(nsteps), int(stride))
+            UN = torch.from_numpy(np.stack(adj.getU_all()))
         ctx.adj = adj
-        ctx.device = U0.device
-        return torch.from_numpy(UN).to(U0.device)
+        return UN
 
     @staticmethod
     def backward(ctx, gUN):
         adj = ctx.adj
-        g = gUN.detach().to("cpu").contiguous().numpy()
-        for r in range(len(adj)):
-            adj.set_seed(r, g[r])
-        adj.backward()
-        gU0 = np.stack([adj.grad_state(r) for r in range(len(adj))])
-        geta = np.stack([adj.grad_eta(r) for r in range(len(adj))])
-        return torch.from_numpy(gU0).to(ctx.device), torch.from_numpy(geta).to(ctx.device), None, None, None
+        g = gUN.detach().contiguous()
+        if g.is_cuda:
+            adj.set_seed_all_device(g.reshape(-1))
+            adj.backward()
+            gU0 = torch.empty_like(g)
+            adj.grad_state_all_device(gU0.view(-1))
+            geta = torch.empty(g.shape[:-1], dtype=g.dtype, device=g.device)
+            adj.grad_eta_all_device(geta.view(-1))
+        else:
+            gn = g.numpy()
+            adj.set_seed_all([gn[r] for r in range(len(adj))])
+            adj.backward()
+            gU0 = torch.from_numpy(np.stack(adj.grad_state_all()))
+            geta = torch.from_numpy(np.stack(adj.grad_eta_all()))
+        return gU0, geta, None, None, None
 
 
 def rollout(U0, eta, cfgs, nsteps, stride=0):

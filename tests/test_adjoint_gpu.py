@@ -221,3 +221,53 @@ def test_torch_autograd_function():
                                atol=1e-11 * np.abs(a.grad_state(0)).max())
     np.testing.assert_allclose(etat.grad[0].cpu().numpy(), a.grad_eta(0), rtol=1e-8,
                                atol=1e-11 * np.abs(a.grad_eta(0)).max())
+
+
+def test_whole_batch_transfers_host_and_device():
+    """The whole-batch forms (host and device pointers) move the same data
+    as the per-run calls."""
+    import torch
+    cases = [_ot(24), _harris(32, 20)]
+    cfgs = [c for c, _ in cases]
+    U0s = [_generic(U, 40 + k, 0.005 if k else 0.05) for k, (_, U) in enumerate(cases)]
+    Ws = [_weights(c, 50 + r) for r, c in enumerate(cfgs)]
+    a = mhd.ResistiveMHD2DAdjoint(cfgs)
+    for r, c in enumerate(cfgs):
+        a.setEtaField(r, _eta_field(c, r))
+    a.setU0_all_device(torch.tensor(np.concatenate([u.ravel() for u in U0s]), device="cuda"))
+    a.forward(6, stride=2)
+    a.set_seed_all_device(torch.tensor(np.concatenate([w.ravel() for w in Ws]), device="cuda"))
+    a.backward()
+    b = mhd.ResistiveMHD2DAdjoint(cfgs)
+    for r, c in enumerate(cfgs):
+        b.setEtaField(r, _eta_field(c, r))
+        b.setU0(r, U0s[r])
+    b.forward(6, stride=2)
+    for r in range(len(cfgs)):
+        b.set_seed(r, Ws[r])
+    b.backward()
+    tot = sum(u.size for u in U0s)
+    neta = sum(c.nx * c.ny for c in cfgs)
+    UN = torch.empty(tot, dtype=torch.float64, device="cuda")
+    gU = torch.empty(tot, dtype=torch.float64, device="cuda")
+    ge = torch.empty(neta, dtype=torch.float64, device="cuda")
+    a.getU_all_device(UN)
+    a.grad_state_all_device(gU)
+    a.grad_eta_all_device(ge)
+    UN, gU, ge = UN.cpu().numpy(), gU.cpu().numpy(), ge.cpu().numpy()
+    host_U, host_gU, host_ge = a.getU_all(), a.grad_state_all(), a.grad_eta_all()
+    o = oe = 0
+    for r, c in enumerate(cfgs):
+        n = U0s[r].size
+        assert np.array_equal(UN[o:o + n].reshape(U0s[r].shape), b.getU(r))
+        assert np.array_equal(host_U[r], b.getU(r))
+        assert np.array_equal(gU[o:o + n].reshape(U0s[r].shape), host_gU[r])
+        assert np.array_equal(ge[oe:oe + c.nx * c.ny].reshape(c.nx, c.ny), host_ge[r])
+        np.testing.assert_allclose(host_gU[r], b.grad_state(r), rtol=1e-6, atol=1e-9 * np.abs(host_gU[r]).max())
+        np.testing.assert_allclose(host_ge[r], b.grad_eta(r), rtol=1e-6, atol=1e-9 * np.abs(host_ge[r]).max())
+        o += n
+        oe += c.nx * c.ny
+    with pytest.raises(ValueError):
+        a.getU_all_device(torch.empty(tot - 1, dtype=torch.float64, device="cuda"))
+    with pytest.raises(TypeError):
+        a.getU_all_device(torch.empty(tot, dtype=torch.float32, device="cuda"))
