@@ -68,11 +68,12 @@ struct AdjDesc {
 
 // ---------------------------------------------------------------- kernels
 // Work list: run r owns tiles [first[r], first[r+1]); a tile is 64
-// consecutive rows of one column, one wavefront.
+// consecutive rows of one column, one wavefront (blocks hold blockDim.x / 64
+// tiles).
 __device__ __forceinline__ bool tile_cell(const long long *first, int nruns, int &r, int &i, int &j,
                                           const AdjDesc *runs)
 {
-    const long long t = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (t >= first[nruns]) return false;
     int lo = 0, hi = nruns - 1;
     while (lo < hi) {
@@ -245,8 +246,10 @@ __device__ __forceinline__ bool non_b(int v) { return v != 4 && v != 5; }
 
 // Face adjoints of one stage and direction: face (i, j) sits between cell
 // (i, j) and (i+1, j) (DIR 0) or (i, j+1) (DIR 1).
+// One wavefront per block: at ~500 registers a SIMD holds one such wave, and
+// single-wave blocks let it take the next tile as soon as its own ends.
 template <int STAGE, int DIR>
-__global__ __launch_bounds__(256) void face_adj_kernel(const AdjDesc *runs, const long long *first, int nruns,
+__global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const long long *first, int nruns,
                                                        int step)
 {
     int r, i, j;
@@ -582,6 +585,7 @@ int upload_desc(mhd_adjoint *a)
 }
 
 dim3 tiles_grid(const mhd_adjoint *a) { return dim3((unsigned)((a->total_tiles + 3) / 4)); }
+dim3 tiles_grid1(const mhd_adjoint *a) { return dim3((unsigned)a->total_tiles); }
 
 int copy_all(mhd_adjoint *a, int src, int src_slot, int dst, int dst_slot)
 {
@@ -869,7 +873,7 @@ int mhd_adjoint_backward(mhd_adjoint *a)
     if (!a->forward_done) return afail(a, MHD_ERR_STATE, "run mhd_adjoint_forward first");
     mhd_batch *b = a->fwd;
     hipStream_t s = stream_of(a);
-    const dim3 tg = tiles_grid(a), blk(256);
+    const dim3 tg = tiles_grid(a), tg1 = tiles_grid1(a), blk(256);
     int rc;
     hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
     for (int c = a->nckpt - 1; c >= 0; --c) {
@@ -890,14 +894,14 @@ int mhd_adjoint_backward(mhd_adjoint *a)
             AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
             if ((rc = bc_transpose(a, kLam))) return rc;
             hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            hipLaunchKernelGGL((face_adj_kernel<2, 0>), tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
-            hipLaunchKernelGGL((face_adj_kernel<2, 1>), tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL((face_adj_kernel<2, 0>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL((face_adj_kernel<2, 1>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st);
             hipLaunchKernelGGL(cell2_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
             if ((rc = bc_transpose(a, kLamBar))) return rc;
             hipLaunchKernelGGL(ct_adj_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
             hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            hipLaunchKernelGGL((face_adj_kernel<1, 0>), tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
-            hipLaunchKernelGGL((face_adj_kernel<1, 1>), tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL((face_adj_kernel<1, 0>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL((face_adj_kernel<1, 1>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st);
             hipLaunchKernelGGL(cell1_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
             AHIP(a, hipGetLastError());
         }

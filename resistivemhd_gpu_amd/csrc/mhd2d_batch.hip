@@ -54,9 +54,10 @@ long long plane_size(int nx, int ny) { return ((long long)nx * plane_pitch(ny) +
 
 void drop_graph(mhd_batch *b)
 {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-    b->graph_cur = -1;
+    for (hipGraphExec_t &g : b->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 // Work list for sweep length L: items of run r are [first_item, first_item +
@@ -146,9 +147,13 @@ int enqueue_step(mhd_batch *b, int cur)
 
 constexpr int kGraphSteps = 2;   // even: the captured graph ends on the slot it started from
 
+// Captures kGraphSteps steps starting from control slot b->cur (one graph per
+// starting slot, so an odd number of steps between calls costs no recapture).
 int capture_graph(mhd_batch *b)
 {
-    drop_graph(b);
+    hipGraphExec_t &exec = b->graph[b->cur];
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
     hipGraph_t graph = nullptr;
     BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = MHD_OK;
@@ -159,13 +164,12 @@ int capture_graph(mhd_batch *b)
         return rc;
     }
     if (e != hipSuccess) return bfail(b, MHD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
-        b->graph = nullptr;
+        exec = nullptr;
         return bfail(b, MHD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
-    b->graph_cur = b->cur;
     return MHD_OK;
 }
 
@@ -387,10 +391,8 @@ int mhd_batch_step(mhd_batch *b, int nsteps)
     if (rc) return rc;
     int n = 0;
     if (b->use_graph && nsteps >= kGraphSteps) {
-        if (!b->graph || b->graph_cur != b->cur) {
-            if ((rc = capture_graph(b))) return rc;
-        }
-        for (; n + kGraphSteps <= nsteps; n += kGraphSteps) BHIP(b, hipGraphLaunch(b->graph, b->stream));
+        if (!b->graph[b->cur] && (rc = capture_graph(b))) return rc;
+        for (; n + kGraphSteps <= nsteps; n += kGraphSteps) BHIP(b, hipGraphLaunch(b->graph[b->cur], b->stream));
     }
     for (; n < nsteps; ++n) {
         if ((rc = enqueue_step(b, b->cur))) return rc;

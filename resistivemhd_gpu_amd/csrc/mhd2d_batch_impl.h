@@ -28,8 +28,7 @@ struct mhd_batch {
     int cur = 0;
     bool dt_valid = false;
     bool use_graph = true;
-    hipGraphExec_t graph = nullptr;
-    int graph_cur = -1;                 // control slot the captured graph starts from
+    hipGraphExec_t graph[2] = {nullptr, nullptr};   // captured steps, by the control slot they start from
     std::string err;
 };
 
