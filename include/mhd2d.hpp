@@ -90,6 +90,18 @@ public:
     }
     void load(const std::string &path) { check(mhd_load(h_, path.c_str()), h_); }
 
+    // The MPI library's per-rank save (ResistiveMHD_multiGPU_cuda/
+    // ResistiveMHD2D_GPU/resistiveMHD2D.cu:457-486): interior cells only.
+    void saveSubdomain(const std::string &directoryname, const std::string &filenameWithoutStep, int step, int rank)
+    {
+        check(mhd_save_subdomain(h_, directoryname.c_str(), filenameWithoutStep.c_str(), step, rank), h_);
+    }
+    void loadSubdomain(const std::string &path) { check(mhd_load_subdomain(h_, path.c_str()), h_); }
+
+    // MHD_SCHEME_SINGLE (default) or MHD_SCHEME_MULTI_SHARE: the MPI
+    // library's step on one rank, periodic halo filled on the device.
+    void setScheme(int scheme) { check(mhd_set_scheme(h_, scheme), h_); }
+
     // Boundary::periodicBoundaryX2nd, ... applied to the current state
     void periodicBoundaryX2nd() { check(mhd_apply_boundary(h_, 0, MHD_BC_PERIODIC), h_); }
     void periodicBoundaryY2nd() { check(mhd_apply_boundary(h_, 1, MHD_BC_PERIODIC), h_); }

@@ -132,6 +132,14 @@ CPP_MAIN = r"""
 #include <cstdio>
 #include <vector>
 #include "mhd2d.hpp"
+// one rank of the MPI library's example loop: its step on one subdomain and
+// the per-rank interior-only save
+void multi_rank(mhd2d::ResistiveMHD2D &sim, int rank) {
+    sim.setScheme(MHD_SCHEME_MULTI_SHARE);
+    mhd_init_turbulence(sim.handle(), 1, 8, 0.5, 0.5);
+    sim.oneStepRK2();
+    sim.saveSubdomain(".", "turb", 0, rank);
+}
 // the reference example's time loop (EX/bhattacharjee/main_bhattacharjee.cu:264-296)
 int main() {
     mhd_config cfg;
