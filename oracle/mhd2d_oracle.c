@@ -46,6 +46,7 @@ struct oracle_sim {
     double dx, dy, gamma, cfl, eps, eta;
     int bc_x, bc_y;
     double dt, time;
+    double dt_fixed;   /* > 0: every step uses this dt (tests of the flux and CT parts alone) */
     long steps;
     /* ResistiveMHD2D members, P/resistiveMHD_2D.hpp:11-23 */
     Cons *U, *UBar;
@@ -647,6 +648,7 @@ static double calculate_dt(struct oracle_sim *s)
         if (s->dtVector[k] < m) m = s->dtVector[k];
     double lim = fmin(s->dx * s->dx / s->eta, s->dy * s->dy / s->eta);
     s->dt = fmin(m, lim) * s->cfl;
+    if (s->dt_fixed > 0.0) s->dt = s->dt_fixed;
     return s->dt;
 }
 
@@ -954,6 +956,7 @@ static double m_calculate_dt(struct oracle_sim *s)
             if (d < m) m = d;
         }
     s->dt = m * s->cfl;
+    if (s->dt_fixed > 0.0) s->dt = s->dt_fixed;
     return s->dt;
 }
 
@@ -1100,6 +1103,7 @@ int oracle_check_crashed_multi(const struct oracle_sim *s)
 }
 
 double oracle_get_dt(const struct oracle_sim *s) { return s->dt; }
+void oracle_set_fixed_dt(struct oracle_sim *s, double dt) { s->dt_fixed = dt; }
 double oracle_get_time(const struct oracle_sim *s) { return s->time; }
 
 /* checkCalculationIsCrashed: P/resistiveMHD_2D.cu:294-309 */

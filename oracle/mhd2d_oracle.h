@@ -22,6 +22,8 @@ void oracle_apply_boundary_axis(struct oracle_sim *s, int axis, int kind);
 double oracle_calculate_dt(struct oracle_sim *s);
 void oracle_step(struct oracle_sim *s, int nsteps);
 double oracle_get_dt(const struct oracle_sim *s);
+/* dt > 0: every later step uses dt instead of the CFL value (0: back to CFL). */
+void oracle_set_fixed_dt(struct oracle_sim *s, double dt);
 double oracle_get_time(const struct oracle_sim *s);
 int oracle_check_crashed(const struct oracle_sim *s);
 void oracle_get_stage1_fluxes(const struct oracle_sim *s, double *F, double *G);

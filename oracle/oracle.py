@@ -43,6 +43,7 @@ def load():
         "oracle_calculate_dt": (c_d, [P]),
         "oracle_step": (None, [P, c_i]),
         "oracle_get_dt": (c_d, [P]),
+        "oracle_set_fixed_dt": (None, [P, c_d]),
         "oracle_get_time": (c_d, [P]),
         "oracle_check_crashed": (c_i, [P]),
         "oracle_get_stage1_fluxes": (None, [P, D, D]),
@@ -113,6 +114,10 @@ class OracleSim:
 
     def apply_boundary_axis(self, axis, kind):
         self._lib.oracle_apply_boundary_axis(self._s, axis, kind)
+
+    def set_fixed_dt(self, dt):
+        """dt > 0: every later step uses dt instead of the CFL value."""
+        self._lib.oracle_set_fixed_dt(self._s, dt)
 
     def calculate_dt(self):
         return self._lib.oracle_calculate_dt(self._s)

@@ -335,3 +335,130 @@ def test_multi_scheme_rejects_symmetric_and_eta_field():
     o = OracleSim(20, 20, 0.1, 0.1, bc_y=BC_SYMMETRIC)
     with pytest.raises(ValueError):
         o.set_scheme(SCHEME_MULTI_SHARE)
+
+
+# ------------------------------------- rotation by 90 degrees (both schemes)
+def _rotate(U, g=3):
+    """The state rotated by +90 degrees about the grid centre, (x, y) ->
+    (-y, x), on the interior n x n cells of a periodic grid: cell (i', j')
+    holds old cell (j', n-1-i'); velocities and B turn with the frame,
+    (u, v) -> (-v, u); face fields move to the faces they land on (the new
+    x-face Bx' is minus the old y-face By one row down, the new y-face By'
+    is the old x-face Bx).  Ghosts are left to the boundary fill."""
+    I = U[g:-g, g:-g]
+
+    def rot(a):
+        return a[:, ::-1].T
+
+    out = np.zeros_like(U)
+    R = out[g:-g, g:-g]
+    R[..., 0] = rot(I[..., 0])
+    R[..., 1] = -rot(I[..., 2])
+    R[..., 2] = rot(I[..., 1])
+    R[..., 3] = rot(I[..., 3])
+    R[..., 4] = -rot(np.roll(I[..., 5], 1, axis=1))
+    R[..., 5] = rot(I[..., 4])
+    R[..., 6] = rot(I[..., 6])
+    R[..., 7] = rot(I[..., 7])
+    return out
+
+
+def _generic_periodic_state(n, seed):
+    import resistivemhd_gpu_amd as mhd
+    cfg = mhd.turbulence_config(n, eta=2e-3)
+    U = mhd.fill_turbulence(cfg, seed=seed, kmax=4, rms_v=0.6, rms_b=0.6)
+    rng = np.random.default_rng(seed)
+    U[..., 0] *= 1.0 + 0.2 * rng.uniform(-1, 1, U.shape[:2])
+    U[..., 3] += 0.2 * rng.normal(size=U.shape[:2])
+    U[..., 6] += 0.2 * rng.normal(size=U.shape[:2])
+    U[..., 7] += 1.0 + 0.3 * rng.uniform(size=U.shape[:2])
+    return cfg, U
+
+
+def test_rotate_helper_is_a_quarter_turn():
+    """Four quarter turns give the identity, and a quarter turn keeps the
+    discrete divergence of B a rotated scalar field."""
+    cfg, U = _generic_periodic_state(24, 3)
+    o = OracleSim.from_config(cfg)
+    o.set_state(U)
+    o.apply_boundary()
+    U = o.get_state()
+    R = U
+    for _ in range(4):
+        o.set_state(_rotate(R))
+        o.apply_boundary()
+        R = o.get_state()
+    assert np.array_equal(R[3:-3, 3:-3], U[3:-3, 3:-3])
+    o.set_state(_rotate(U))
+    o.apply_boundary()
+    R1 = o.get_state()
+    d0, d1 = _divb(U, cfg.dx, cfg.dy), _divb(R1, cfg.dx, cfg.dy)
+    np.testing.assert_allclose(d1, d0[:, ::-1].T, atol=1e-12)
+
+
+def _run_fixed_dt(cfg, state, scheme, nsteps, dt=0.02):
+    o = OracleSim.from_config(cfg)
+    if scheme:
+        o.set_scheme(scheme)
+    o.set_fixed_dt(dt)
+    o.set_state(state)
+    o.apply_boundary()
+    o.step(nsteps)
+    assert not o.crashed()
+    return o.get_state()
+
+
+def _assert_commutes(A, B, tol=1e-11):
+    for v in range(8):
+        a, b = A[3:-3, 3:-3, v], B[3:-3, 3:-3, v]
+        scale = np.abs(a).max()
+        assert np.abs(a - b).max() <= tol * scale, (v, np.abs(a - b).max(), scale)
+
+
+def _transpose(U):
+    """The state mirrored across the diagonal x = y: cell (i, j) -> (j, i),
+    (u, v) -> (v, u), face Bx -> face By and back (with B -> -B, which the
+    equations allow, the axial field keeps the same components)."""
+    T = np.transpose(U, (1, 0, 2)).copy()
+    T[..., 1], T[..., 2] = U[..., 2].T.copy(), U[..., 1].T.copy()
+    T[..., 4], T[..., 5] = U[..., 5].T.copy(), U[..., 4].T.copy()
+    return T
+
+
+@pytest.mark.parametrize("scheme", [0, SCHEME_MULTI_SHARE])
+def test_step_commutes_with_transpose(scheme):
+    """step(transpose(U)) = transpose(step(U)) to round-off on a generic
+    periodic state, dt held fixed (the reference's cell dt reads the stored
+    face B as if it were centred, which no symmetry maps onto itself).
+    Pins the y sweep against the x sweep (rotation into the x frame and
+    back, the resistive stencils of both directions) and, for the MPI
+    library's step, the CT-contact EMF: its S/N terms map onto its W/E terms,
+    so an index, upwind-flag or sign slip in any one of them breaks the
+    symmetry (checked by planting each in a copy of the oracle: 4e-5 to 3e-2
+    against 1e-15)."""
+    cfg, U = _generic_periodic_state(32, 5)
+    o = OracleSim.from_config(cfg)
+    o.set_state(U)
+    o.apply_boundary()
+    U = o.get_state()
+    _assert_commutes(_transpose(_run_fixed_dt(cfg, U, scheme, 3)), _run_fixed_dt(cfg, _transpose(U), scheme, 3))
+
+
+def test_step_commutes_with_quarter_turn():
+    """The single-GPU step also commutes with a quarter turn.  The MPI
+    library's step does not, by construction: its stages advance the face
+    fields by flux differences taken half a cell to the left (Bx) and below
+    (By) the face (M/resistiveMHD2D.cu:64-100), offsets that a
+    transpose keeps and a quarter turn does not."""
+    cfg, U = _generic_periodic_state(32, 5)
+    o = OracleSim.from_config(cfg)
+    o.set_state(U)
+    o.apply_boundary()
+    U = o.get_state()
+    A = _rotate(_run_fixed_dt(cfg, U, 0, 3))
+    _assert_commutes(A, _run_fixed_dt(cfg, _rotate(U), 0, 3))
+    # a wrong turn (velocity not turned) does not commute
+    bad = _rotate(U)
+    bad[..., 1], bad[..., 2] = bad[..., 2].copy(), bad[..., 1].copy()
+    C = _run_fixed_dt(cfg, bad, 0, 3)
+    assert np.abs(C[3:-3, 3:-3, 1] - A[3:-3, 3:-3, 1]).max() > 1e-3 * np.abs(A[3:-3, 3:-3, 1]).max()
