@@ -59,7 +59,8 @@ struct AdjDesc {
     double *u, *us;               // U^n and U* (the forward batch's buffers)
     const double *eta;            // per-cell resistivity or nullptr (uniform ph.eta)
     double *lam, *lamBar, *lamU, *lamC, *lamE, *geta;
-    double *u0, *hist, *ckpt;     // initial state, segment states, checkpoints
+    double *u0, *hist, *ckpt;     // initial state, segment states U^n, checkpoints
+    double *hists;                // segment states U* (stage-1 results with their boundary)
     double *dts;                  // dt of every step of the rollout
     // tangent-linear model: direction (dU0, deta), tangent state dU, dU*,
     // tangent face fluxes dF, dG and corner EMF dE
@@ -114,7 +115,7 @@ struct AtomicAdd {
     }
 };
 
-enum Buf { kU0 = 0, kU = 1, kHist = 2, kCkpt = 3, kLam = 4, kLamBar = 5, kLamU = 6, kLamC = 7 };
+enum Buf { kU0 = 0, kU = 1, kHist = 2, kCkpt = 3, kLam = 4, kLamBar = 5, kLamU = 6, kLamC = 7, kUs = 8, kHistS = 9 };
 
 __device__ __forceinline__ double *buf_ptr(const AdjDesc &d, int which, int slot)
 {
@@ -127,6 +128,8 @@ __device__ __forceinline__ double *buf_ptr(const AdjDesc &d, int which, int slot
     case kLam: return d.lam;
     case kLamBar: return d.lamBar;
     case kLamU: return d.lamU;
+    case kUs: return d.us;
+    case kHistS: return d.hists + slot * s8;
     default: return d.lamC;
     }
 }
@@ -248,9 +251,11 @@ __device__ __forceinline__ bool non_b(int v) { return v != 4 && v != 5; }
 // (i, j) and (i+1, j) (DIR 0) or (i, j+1) (DIR 1).
 // One wavefront per block: at ~500 registers a SIMD holds one such wave, and
 // single-wave blocks let it take the next tile as soon as its own ends.
+// slot >= 0: the primal state of this stage is history slot `slot` (U^n for
+// stage 1, U* for stage 2); slot < 0: the live buffers u / us.
 template <int STAGE, int DIR>
 __global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const long long *first, int nruns,
-                                                       int step)
+                                                       int step, int slot)
 {
     int r, i, j;
     if (!tile_cell(first, nruns, r, i, j, runs)) return;
@@ -277,7 +282,9 @@ __global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const
         any |= lamF[4] != 0.0 || lamF[5] != 0.0;
     }
     if (!any) return;
-    const Soa S{STAGE == 2 ? d.us : d.u, g.pitch, g.plane};
+    const double *prim = slot < 0 ? (STAGE == 2 ? d.us : d.u)
+                                   : (STAGE == 2 ? d.hists : d.hist) + (size_t)slot * 8 * g.plane;
+    const Soa S{prim, g.pitch, g.plane};
     const EtaAcc E{d.eta, g.pitch, d.ph.eta};
     AtomicAdd add{d.lamC, d.geta, g.pitch, g.plane};
     const double par[3] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps};
@@ -549,8 +556,8 @@ struct mhd_adjoint {
     long long *d_first = nullptr;
     long long total_tiles = 0;
     double *pool = nullptr;       // lam, lamBar, lamU, lamC, lamE, geta, u0 per run
-    double *hist = nullptr, *ckpt = nullptr, *dts = nullptr;
-    size_t hist_cap = 0, ckpt_cap = 0, dts_cap = 0;   // doubles
+    double *hist = nullptr, *hists = nullptr, *ckpt = nullptr, *dts = nullptr;
+    size_t hist_cap = 0, hists_cap = 0, ckpt_cap = 0, dts_cap = 0;   // doubles
     int nsteps = 0, stride = 0, nckpt = 0;
     bool forward_done = false;
     double *tpool = nullptr;      // tangent buffers (allocated on first use)
@@ -641,10 +648,12 @@ int layout_storage(mhd_adjoint *a, int nsteps, int stride)
     }
     int rc;
     if ((rc = ensure(a, &a->hist, &a->hist_cap, h))) return rc;
+    if ((rc = ensure(a, &a->hists, &a->hists_cap, h))) return rc;
     if ((rc = ensure(a, &a->ckpt, &a->ckpt_cap, c))) return rc;
     size_t ho = 0, co = 0;
     for (AdjDesc &d : a->desc) {
         d.hist = a->hist + ho;
+        d.hists = a->hists + ho;
         d.ckpt = a->ckpt + co;
         ho += (size_t)K * 8 * d.g.plane;
         co += (size_t)nck * 8 * d.g.plane;
@@ -768,6 +777,7 @@ int mhd_adjoint_destroy(mhd_adjoint *a)
     (void)hipFree(a->d_desc);
     (void)hipFree(a->d_first);
     (void)hipFree(a->hist);
+    (void)hipFree(a->hists);
     (void)hipFree(a->ckpt);
     (void)hipFree(a->dts);
     (void)hipFree(a->tpool);
@@ -878,30 +888,35 @@ int mhd_adjoint_backward(mhd_adjoint *a)
     hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
     for (int c = a->nckpt - 1; c >= 0; --c) {
         const int s0 = c * a->stride, s1 = std::min(a->nsteps, s0 + a->stride);
-        // recompute the segment's states U^{s0} .. U^{s1-1} with the recorded dt
+        // recompute the segment with the recorded dt: history slot st - s0
+        // gets U^st and U* = BC(stage 1 of U^st); the segment's last step
+        // keeps both in the live buffers
         if ((rc = copy_all(a, kCkpt, c, kU, 0))) return rc;
         if ((rc = set_replay(a, true, false, s0))) return rc;
-        for (int st = s0; st < s1; ++st) {
+        for (int st = s0; st + 1 < s1; ++st) {
             if ((rc = copy_all(a, kU, 0, kHist, st - s0))) return rc;
-            if (st + 1 < s1 && (rc = mhd_batch_step(b, 1))) return afail(a, rc, b->err);
+            if ((rc = mhd_batch_step(b, 1))) return afail(a, rc, b->err);
+            if ((rc = copy_all(a, kUs, 0, kHistS, st - s0))) return rc;
         }
+        AHIP(a, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
+                                        b->items_per_xcd, b->cur, s));
+        AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
         for (int st = s1 - 1; st >= s0; --st) {
-            // U = U^st, U* = BC(stage 1 of U^st)
-            if ((rc = copy_all(a, kHist, st - s0, kU, 0))) return rc;
-            if ((rc = set_replay(a, true, false, st))) return rc;
-            AHIP(a, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
-                                            b->items_per_xcd, b->cur, s));
-            AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
+            const int slot = st + 1 < s1 ? st - s0 : -1;
             if ((rc = bc_transpose(a, kLam))) return rc;
             hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            hipLaunchKernelGGL((face_adj_kernel<2, 0>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st);
-            hipLaunchKernelGGL((face_adj_kernel<2, 1>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL((face_adj_kernel<2, 0>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st,
+                               slot);
+            hipLaunchKernelGGL((face_adj_kernel<2, 1>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st,
+                               slot);
             hipLaunchKernelGGL(cell2_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
             if ((rc = bc_transpose(a, kLamBar))) return rc;
             hipLaunchKernelGGL(ct_adj_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
             hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            hipLaunchKernelGGL((face_adj_kernel<1, 0>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st);
-            hipLaunchKernelGGL((face_adj_kernel<1, 1>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st);
+            hipLaunchKernelGGL((face_adj_kernel<1, 0>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st,
+                               slot);
+            hipLaunchKernelGGL((face_adj_kernel<1, 1>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st,
+                               slot);
             hipLaunchKernelGGL(cell1_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
             AHIP(a, hipGetLastError());
         }
