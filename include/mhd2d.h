@@ -306,6 +306,13 @@ int mhd_adjoint_set_seeds_device(mhd_adjoint *a, const double *dev_all);
 int mhd_adjoint_get_grad_states_device(mhd_adjoint *a, double *dev_all);
 int mhd_adjoint_get_grad_etas_device(mhd_adjoint *a, double *dev_all);
 int mhd_adjoint_synchronize(mhd_adjoint *a);
+/* Kernel form of the face adjoints (no counterpart in the reference): one
+ * face per lane with atomics, or a sweep along x with LDS rings that
+ * accumulates cell adjoints before they leave for HBM.  AUTO picks the sweep
+ * for batches large enough to fill the GPU with it.  Results agree to the
+ * rounding of the atomic sums. */
+enum mhd_adj_face_kernel { MHD_ADJ_FACE_AUTO = 0, MHD_ADJ_FACE_PER_FACE = 1, MHD_ADJ_FACE_SWEEP = 2 };
+int mhd_adjoint_set_face_kernel(mhd_adjoint *a, int mode);
 
 #ifdef __cplusplus
 }

@@ -292,6 +292,269 @@ __global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const
     mhd_ad::flux_full_adj(S, E, DIR, i, j, par, rpar, lamF, add);
 }
 
+// ------------------------------------------------ face adjoints, sweep form
+// A workgroup of two wavefronts takes one item: the 64 rows j0 .. j0+63 of
+// one run and the face columns ia .. ib-1.  Wave 0 computes the x faces and
+// wave 1 the y faces of each column, in step.  Everything a column's faces
+// read sits in LDS rings (rows j0-2 .. j0+65; state columns i-2 .. i+2,
+// adjoint and resistivity columns as the stencils need).  The columns the
+// next step adds arrive by direct-to-LDS loads into a separate staging
+// array while this step computes (a separate object, so the compiler need
+// not wait for them before reading the rings), and are copied into the
+// rings at the next step.  Cell and resistivity adjoints accumulate in an
+// LDS ring of five columns and leave for HBM once per column when
+// complete, instead of with 38 atomics per face.
+constexpr int kSwRows = 68;   // LDS rows: global rows j0-2 .. j0+65
+constexpr int kSwSt = 5;      // state columns i-2 .. i+2
+constexpr int kSwRing = 2;    // adjoint / resistivity columns
+constexpr int kSwAcc = 5;     // accumulator columns i-2 .. i+2
+constexpr int kSwStage = 16;  // staged planes: 8 state, 6 adjoint, corner EMF, resistivity
+
+struct __attribute__((aligned(16))) SweepLds {   // every row 16-byte aligned
+    double st[8][kSwSt][kSwRows];
+    double lm[6][kSwRing][kSwRows];   // adjoint components 0, 1, 2, 3, 6, 7
+    double le[kSwRing][kSwRows];      // corner-EMF adjoint (stage 1)
+    double et[kSwRing][kSwRows];      // resistivity plane
+    double acc[9][kSwAcc][kSwRows];   // cell adjoints 0..7, resistivity adjoint 8
+};
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+// s_waitcnt vmcnt(0) as the compiler's own instruction (so it knows the
+// direct-to-LDS loads have landed): vmcnt 0, expcnt and lgkmcnt at maximum.
+__device__ __forceinline__ void wait_vm_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// Loads rows j0-2 .. j0+65 of one plane column (col_base = &plane[i * pitch],
+// 64-byte aligned) into dst[kSwRows]: lane l < 34 moves rows 2l, 2l+1 with
+// one 16-byte direct-to-LDS load (LDS address dst + 16 l).  Row pairs start
+// at even rows, so every load is 16-byte aligned; a pair past the grid is
+// clamped to the last even-started in-grid pair, which only changes rows
+// >= ny - 1 (no valid face reads row ny - 1 or beyond; rows below 0 are
+// likewise never read).
+__device__ __forceinline__ void sweep_dma(const double *col_base, int j0, int ny, double *dst, int lane)
+{
+    if (lane < kSwRows / 2) {
+        const int r0 = min(max(j0 - 2 + 2 * lane, 0), (ny - 2) & ~1);
+        __builtin_amdgcn_global_load_lds((glb_void_t *)(col_base + r0), (lds_void_t *)dst, 16, 0, 0);
+    }
+}
+
+// Accessors over the LDS rings.  Column indices are wave-uniform, so the
+// ring slot (column mod ring length) is scalar arithmetic.
+struct SweepState {
+    const SweepLds *l;
+    int j0;
+    __device__ double operator()(int v, int ci, int cj) const { return l->st[v][ci % kSwSt][cj - j0 + 2]; }
+};
+
+struct SweepEta {
+    const SweepLds *l;
+    int j0;
+    bool field;
+    double uniform;
+    __device__ double operator()(int ci, int cj) const { return field ? l->et[ci % kSwRing][cj - j0 + 2] : uniform; }
+};
+
+struct SweepAdd {
+    SweepLds *l;
+    int j0;
+    __device__ void operator()(int ci, int cj, int v, double b) const
+    {
+        __hip_atomic_fetch_add(&l->acc[v][ci % kSwAcc][cj - j0 + 2], b, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+};
+
+// Item -> run, first row, face columns [ia, ib).
+__device__ __forceinline__ bool sweep_item(const AdjDesc *runs, const long long *sfirst, int nruns, int cols,
+                                           int &r, int &j0, int &ia, int &ib)
+{
+    const long long t = blockIdx.x;
+    if (t >= sfirst[nruns]) return false;
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sfirst[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    r = lo;
+    const long long lt = t - sfirst[lo];
+    const int ntj = runs[lo].ntj;
+    j0 = (int)(lt % ntj) * 64;
+    ia = 2 + (int)(lt / ntj) * cols;
+    ib = min(ia + cols, runs[lo].g.nx - 2);
+    return true;
+}
+
+__device__ __forceinline__ int lm_index(int v) { return v < 4 ? v : v - 2; }
+
+// The fields of a run the sweep reads, loaded once per workgroup (uniform).
+struct SweepRun {
+    int nx, ny, pitch;
+    long long plane;
+    const double *prim, *L, *lamE, *eta;
+    double *lamC, *geta;
+};
+
+// Adds accumulator column m to HBM (global atomics: neighbouring items share
+// the edge columns and rows) and clears it.
+__device__ __forceinline__ void sweep_flush(SweepLds &l, const SweepRun &g, int m, int j0, int tid)
+{
+    const int sa = m % kSwAcc;
+    // components 0..7: 16 threads each; the resistivity adjoint: 68 threads
+    const int v = tid >> 4;
+    double *dst = g.lamC + v * g.plane + (size_t)m * g.pitch + (j0 - 2);
+    for (int r = tid & 15; r < kSwRows; r += 16) {
+        const double b = l.acc[v][sa][r];
+        l.acc[v][sa][r] = 0.0;
+        const int gj = j0 - 2 + r;
+        if (b != 0.0 && gj >= 0 && gj < g.ny) unsafeAtomicAdd(dst + r, b);
+    }
+    if (tid < kSwRows) {
+        const double b = l.acc[8][sa][tid];
+        l.acc[8][sa][tid] = 0.0;
+        const int gj = j0 - 2 + tid;
+        if (b != 0.0 && gj >= 0 && gj < g.ny) unsafeAtomicAdd(g.geta + (size_t)m * g.pitch + gj, b);
+    }
+}
+
+// Plane p of the 16 a sweep stages (0..7 state, 8..13 adjoint 0,1,2,3,6,7,
+// 14 corner-EMF adjoint, 15 resistivity) at column col, or nullptr where the
+// run has none.
+template <int STAGE>
+__device__ __forceinline__ const double *sweep_plane(const SweepRun &g, int p, int col)
+{
+    const size_t o = (size_t)col * g.pitch;
+    if (p < 8) return g.prim + p * g.plane + o;
+    if (p < 14) {
+        const int q = p - 8;
+        return g.L + (q < 4 ? q : q + 2) * g.plane + o;
+    }
+    if (p == 14) return STAGE == 1 ? g.lamE + o : nullptr;
+    return g.eta ? g.eta + o : nullptr;
+}
+
+// Ring slot that plane p of column col occupies.
+__device__ __forceinline__ double *sweep_slot(SweepLds &l, int p, int col)
+{
+    if (p < 8) return l.st[p][col % kSwSt];
+    if (p < 14) return l.lm[p - 8][col % kSwRing];
+    if (p == 14) return l.le[col % kSwRing];
+    return l.et[col % kSwRing];
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs, const long long *sfirst, int nruns,
+                                                             int cols, int step, int slot)
+{
+    __shared__ SweepLds l;
+    __shared__ __attribute__((aligned(16))) double stg[kSwStage][kSwRows];
+    int r, j0, ia, ib;
+    if (!sweep_item(runs, sfirst, nruns, cols, r, j0, ia, ib)) return;
+    r = __builtin_amdgcn_readfirstlane(r);
+    j0 = __builtin_amdgcn_readfirstlane(j0);
+    ia = __builtin_amdgcn_readfirstlane(ia);
+    ib = __builtin_amdgcn_readfirstlane(ib);
+    const AdjDesc &d = runs[r];
+    SweepRun g;
+    g.nx = d.g.nx;
+    g.ny = d.g.ny;
+    g.pitch = d.g.pitch;
+    g.plane = d.g.plane;
+    g.prim = slot < 0 ? (STAGE == 2 ? d.us : d.u) : (STAGE == 2 ? d.hists : d.hist) + (size_t)slot * 8 * d.g.plane;
+    g.L = STAGE == 2 ? d.lam : d.lamBar;
+    g.lamE = d.lamE;
+    g.eta = d.eta;
+    g.lamC = d.lamC;
+    g.geta = d.geta;
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // 0: x faces, 1: y faces
+    const int lane = tid & 63;
+    const int j = j0 + lane;
+    const double dt = d.dts[step];
+    const double coef = (STAGE == 2 ? 0.5 : 1.0) * dt / (wave == 0 ? d.ph.dx : d.ph.dy);
+    const double par[3] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps};
+    const double rpar[2] = {d.ph.dx, d.ph.dy};
+    const double eta_uniform = d.ph.eta;
+
+    // every column step ia reads, straight into the rings: state ia-2 ..
+    // ia+2, adjoint and resistivity ia, ia+1, corner-EMF adjoint ia-1, ia;
+    // the two waves take alternate planes
+    for (int k = tid; k < 9 * kSwAcc * kSwRows; k += 128) (&l.acc[0][0][0])[k] = 0.0;
+    for (int p = wave; p < kSwStage; p += 2) {
+        const int c0 = p < 8 ? ia - 2 : (p == 14 ? ia - 1 : ia);
+        const int c1 = p < 8 ? ia + 2 : (p == 14 ? ia : ia + 1);
+        for (int c = c0; c <= c1; ++c) {
+            const double *src = sweep_plane<STAGE>(g, p, c);
+            if (src) sweep_dma(src, j0, g.ny, sweep_slot(l, p, c), lane);
+        }
+    }
+
+    for (int i = ia; i < ib; ++i) {
+        wait_vm_all();       // this wave's loads for step i landed
+        __syncthreads();     // and the other wave's; step i-1 done
+        if (i > ia) {
+            // the staged columns into their ring slots, whose previous
+            // columns no wave reads from step i on
+            {
+                // 8 threads per plane, 16-byte moves: 34 row pairs each
+                const int p = tid >> 3;
+                const int c = p < 8 ? i + 2 : (p == 14 ? i : i + 1);
+                double2 *to = reinterpret_cast<double2 *>(sweep_slot(l, p, c));
+                const double2 *from = reinterpret_cast<const double2 *>(stg[p]);
+                for (int q = tid & 7; q < kSwRows / 2; q += 8) to[q] = from[q];
+            }
+            __syncthreads();
+        }
+        if (i - 2 >= ia - 1) sweep_flush(l, g, i - 2, j0, tid);
+        if (i + 1 < ib) {
+            // the columns step i+1 adds: state i+3, adjoint and resistivity
+            // i+2, corner-EMF adjoint i+1
+            for (int p = wave; p < kSwStage; p += 2) {
+                const int c = p < 8 ? i + 3 : (p == 14 ? i + 1 : i + 2);
+                const double *src = sweep_plane<STAGE>(g, p, c);
+                if (src) sweep_dma(src, j0, g.ny, stg[p], lane);
+            }
+        }
+
+        const bool valid = wave == 0 ? (i <= g.nx - 4 && j >= 2 && j <= g.ny - 3)
+                                     : (i <= g.nx - 3 && j >= 2 && j <= g.ny - 4);
+        if (valid) {
+            const int ri = i % kSwRing, ri1 = (i + 1) % kSwRing;
+            const int row = lane + 2;
+            double lamF[8];
+            bool any = false;
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                lamF[v] = 0.0;
+                if (non_b(v)) {
+                    const double *Lq = l.lm[lm_index(v)][0];
+                    const double here = Lq[ri * kSwRows + row];
+                    const double next = wave == 0 ? Lq[ri1 * kSwRows + row] : Lq[ri * kSwRows + row + 1];
+                    lamF[v] = coef * (next - here);
+                }
+                any |= lamF[v] != 0.0;
+            }
+            if (STAGE == 1) {
+                // E(i, j) = (G4(i, j) + G4(i+1, j) - F5(i, j) - F5(i, j+1)) / 4
+                const double e0 = l.le[ri][row];
+                if (wave == 0) lamF[5] = -0.25 * (e0 + l.le[ri][row - 1]);
+                else lamF[4] = 0.25 * (e0 + l.le[(i - 1) % kSwRing][row]);
+                any |= lamF[4] != 0.0 || lamF[5] != 0.0;
+            }
+            if (any) {
+                SweepState S{&l, j0};
+                SweepEta E{&l, j0, g.eta != nullptr, eta_uniform};
+                SweepAdd add{&l, j0};
+                mhd_ad::flux_full_adj(S, E, wave, i, j, par, rpar, lamF, add);
+            }
+        }
+    }
+    __syncthreads();
+    for (int m = max(ia - 1, ib - 2); m <= ib + 1; ++m) sweep_flush(l, g, m, j0, tid);
+}
+
 // Adjoint of the face-to-centre averaging (shiftUToCenterForCT) at cell
 // (i, j): component v of the stored state.
 __device__ __forceinline__ double shift_t(const AdjDesc &d, int v, int i, int j)
@@ -555,6 +818,12 @@ struct mhd_adjoint {
     AdjDesc *d_desc = nullptr;
     long long *d_first = nullptr;
     long long total_tiles = 0;
+    // sweep form of the face adjoints: item prefix per run, columns per item
+    long long *d_sfirst = nullptr;
+    long long sweep_items = 0;
+    int sweep_cols = 0;
+    int face_kernel = MHD_ADJ_FACE_AUTO;
+    bool face_kernel_auto_sweep = false;
     double *pool = nullptr;       // lam, lamBar, lamU, lamC, lamE, geta, u0 per run
     double *hist = nullptr, *hists = nullptr, *ckpt = nullptr, *dts = nullptr;
     size_t hist_cap = 0, hists_cap = 0, ckpt_cap = 0, dts_cap = 0;   // doubles
@@ -593,6 +862,61 @@ int upload_desc(mhd_adjoint *a)
 
 dim3 tiles_grid(const mhd_adjoint *a) { return dim3((unsigned)((a->total_tiles + 3) / 4)); }
 dim3 tiles_grid1(const mhd_adjoint *a) { return dim3((unsigned)a->total_tiles); }
+
+// Sweep items: a 64-row strip of a run and `cols` face columns; enough items
+// for about 16 rounds of resident workgroups (two per CU), 8 to 64 columns.
+int layout_sweep(mhd_adjoint *a)
+{
+    long long strips_cols = 0;
+    for (const AdjDesc &d : a->desc) strips_cols += (long long)d.ntj * (d.g.nx - 4);
+    const long long target = 16LL * 2 * std::max(1, a->fwd->num_cu);
+    a->sweep_cols = (int)std::min<long long>(64, std::max<long long>(8, strips_cols / target));
+    std::vector<long long> first(a->nruns + 1, 0);
+    for (int r = 0; r < a->nruns; ++r) {
+        const AdjDesc &d = a->desc[r];
+        first[r + 1] = first[r] + (long long)d.ntj * ((d.g.nx - 4 + a->sweep_cols - 1) / a->sweep_cols);
+    }
+    a->sweep_items = first[a->nruns];
+    AHIP(a, hipMalloc(&a->d_sfirst, sizeof(long long) * (a->nruns + 1)));
+    AHIP(a, hipMemcpyAsync(a->d_sfirst, first.data(), sizeof(long long) * (a->nruns + 1), hipMemcpyHostToDevice,
+                           stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    // the per-face form wins where the sweep would leave the machine idle:
+    // fewer strip-columns than ~16 rounds of 8-column items
+    a->face_kernel_auto_sweep = strips_cols >= target * 8;
+    return MHD_OK;
+}
+
+bool use_sweep(const mhd_adjoint *a)
+{
+    return a->face_kernel == MHD_ADJ_FACE_SWEEP || (a->face_kernel == MHD_ADJ_FACE_AUTO && a->face_kernel_auto_sweep);
+}
+
+// The four face-adjoint launches of one step (stage s, both directions).
+int launch_face_adj(mhd_adjoint *a, int stage, int step, int slot)
+{
+    hipStream_t s = stream_of(a);
+    if (use_sweep(a)) {
+        const dim3 grid((unsigned)a->sweep_items), blk(128);
+        if (stage == 2)
+            hipLaunchKernelGGL(face_adj_sweep_kernel<2>, grid, blk, 0, s, a->d_desc, a->d_sfirst, a->nruns,
+                               a->sweep_cols, step, slot);
+        else
+            hipLaunchKernelGGL(face_adj_sweep_kernel<1>, grid, blk, 0, s, a->d_desc, a->d_sfirst, a->nruns,
+                               a->sweep_cols, step, slot);
+    } else {
+        const dim3 tg1 = tiles_grid1(a), blk(64);
+        if (stage == 2) {
+            hipLaunchKernelGGL((face_adj_kernel<2, 0>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step, slot);
+            hipLaunchKernelGGL((face_adj_kernel<2, 1>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step, slot);
+        } else {
+            hipLaunchKernelGGL((face_adj_kernel<1, 0>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step, slot);
+            hipLaunchKernelGGL((face_adj_kernel<1, 1>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step, slot);
+        }
+    }
+    AHIP(a, hipGetLastError());
+    return MHD_OK;
+}
 
 int copy_all(mhd_adjoint *a, int src, int src_slot, int dst, int dst_slot)
 {
@@ -760,7 +1084,7 @@ int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out)
         d.geta = a->pool + off; off += d.g.plane;
         d.u0 = a->pool + off; off += p8;
     }
-    if ((rc = upload_desc(a))) {
+    if ((rc = layout_sweep(a)) || (rc = upload_desc(a))) {
         const std::string msg = a->err;
         mhd_adjoint_destroy(a);
         return afail(nullptr, rc, msg);
@@ -776,6 +1100,7 @@ int mhd_adjoint_destroy(mhd_adjoint *a)
     (void)hipFree(a->pool);
     (void)hipFree(a->d_desc);
     (void)hipFree(a->d_first);
+    (void)hipFree(a->d_sfirst);
     (void)hipFree(a->hist);
     (void)hipFree(a->hists);
     (void)hipFree(a->ckpt);
@@ -784,6 +1109,15 @@ int mhd_adjoint_destroy(mhd_adjoint *a)
     (void)hipFree(a->stage_all);
     mhd_batch_destroy(a->fwd);
     delete a;
+    return MHD_OK;
+}
+
+int mhd_adjoint_set_face_kernel(mhd_adjoint *a, int mode)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (mode != MHD_ADJ_FACE_AUTO && mode != MHD_ADJ_FACE_PER_FACE && mode != MHD_ADJ_FACE_SWEEP)
+        return afail(a, MHD_ERR_INVALID, "mode must be MHD_ADJ_FACE_AUTO, _PER_FACE or _SWEEP");
+    a->face_kernel = mode;
     return MHD_OK;
 }
 
@@ -883,7 +1217,7 @@ int mhd_adjoint_backward(mhd_adjoint *a)
     if (!a->forward_done) return afail(a, MHD_ERR_STATE, "run mhd_adjoint_forward first");
     mhd_batch *b = a->fwd;
     hipStream_t s = stream_of(a);
-    const dim3 tg = tiles_grid(a), tg1 = tiles_grid1(a), blk(256);
+    const dim3 tg = tiles_grid(a), blk(256);
     int rc;
     hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
     for (int c = a->nckpt - 1; c >= 0; --c) {
@@ -905,18 +1239,12 @@ int mhd_adjoint_backward(mhd_adjoint *a)
             const int slot = st + 1 < s1 ? st - s0 : -1;
             if ((rc = bc_transpose(a, kLam))) return rc;
             hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            hipLaunchKernelGGL((face_adj_kernel<2, 0>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st,
-                               slot);
-            hipLaunchKernelGGL((face_adj_kernel<2, 1>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st,
-                               slot);
+            if ((rc = launch_face_adj(a, 2, st, slot))) return rc;
             hipLaunchKernelGGL(cell2_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
             if ((rc = bc_transpose(a, kLamBar))) return rc;
             hipLaunchKernelGGL(ct_adj_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
             hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            hipLaunchKernelGGL((face_adj_kernel<1, 0>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st,
-                               slot);
-            hipLaunchKernelGGL((face_adj_kernel<1, 1>), tg1, dim3(64), 0, s, a->d_desc, a->d_first, a->nruns, st,
-                               slot);
+            if ((rc = launch_face_adj(a, 1, st, slot))) return rc;
             hipLaunchKernelGGL(cell1_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
             AHIP(a, hipGetLastError());
         }

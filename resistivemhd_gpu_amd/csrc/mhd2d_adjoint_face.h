@@ -14,6 +14,16 @@
 #pragma once
 #include "mhd2d_face_ad.h"
 
+// Loops over stencil slots are unrolled so that accessors see constant
+// offsets from the face (a ring-buffer accessor indexes small arrays by them).
+#if defined(__clang__)
+#define MHD_UNROLL _Pragma("unroll")
+#elif defined(__GNUC__)
+#define MHD_UNROLL _Pragma("GCC unroll 16")
+#else
+#define MHD_UNROLL
+#endif
+
 namespace mhd_ad {
 
 // Centred component (shiftUToCenterForCT, resistiveMHD_2D.cu:313-368):
@@ -45,9 +55,11 @@ MHD_HD inline void stencil_cell(int dir, int i, int j, int c, int &ci, int &cj)
 template <class S>
 MHD_HD inline void gather_face(const S &U, int dir, int i, int j, double *in)
 {
+    MHD_UNROLL
     for (int c = 0; c < 4; ++c) {
         int ci, cj;
         stencil_cell(dir, i, j, c, ci, cj);
+        MHD_UNROLL
         for (int v = 0; v < 8; ++v) in[8 * c + v] = centred(U, dir == 0 ? v : rot_y(v), ci, cj);
     }
 }
@@ -57,6 +69,7 @@ template <class S, class E>
 MHD_HD inline void gather_resist(const S &U, const E &eta, int dir, int i, int j, double *in)
 {
     if (dir == 0) {
+        MHD_UNROLL
         for (int c = 0; c < 4; ++c) {
             in[c] = centred(U, 6, i - 1 + c, j);
             in[4 + c] = centred(U, 5, i - 1 + c, j);
@@ -68,6 +81,7 @@ MHD_HD inline void gather_resist(const S &U, const E &eta, int dir, int i, int j
         in[12] = eta(i, j);
         in[13] = eta(i + 1, j);
     } else {
+        MHD_UNROLL
         for (int c = 0; c < 4; ++c) {
             in[c] = centred(U, 6, i, j - 1 + c);
             in[4 + c] = centred(U, 4, i, j - 1 + c);
@@ -92,6 +106,30 @@ MHD_HD inline void resist_slot(int dir, int i, int j, int s, int &ci, int &cj, i
         if (s < 8) { ci = i; cj = j - 1 + (s & 3); v = s < 4 ? 6 : 4; return; }
         if (s < 12) { ci = (s & 1) ? i + 1 : i - 1; cj = s < 10 ? j : j + 1; v = 5; return; }
         ci = i; cj = s == 12 ? j : j + 1; v = 8;
+    }
+}
+
+// Scatter of one face's input adjoints: in_b (4 cells x 8 slots, rotated
+// for y faces) and the resistive stencil's remaining slots 8..13.
+template <class Add>
+MHD_HD inline void scatter_face(int dir, int i, int j, const double *in_b, const double *rin_b, Add &add)
+{
+    MHD_UNROLL
+    for (int c = 0; c < 4; ++c) {
+        int ci, cj;
+        stencil_cell(dir, i, j, c, ci, cj);
+        MHD_UNROLL
+        for (int v = 0; v < 8; ++v) {
+            const double b = in_b[8 * c + v];
+            if (b != 0.0) add(ci, cj, dir == 0 ? v : rot_y(v), b);
+        }
+    }
+    MHD_UNROLL
+    for (int s = 8; s < 14; ++s) {
+        if (rin_b[s] == 0.0) continue;
+        int ci, cj, v;
+        resist_slot(dir, i, j, s, ci, cj, v);
+        add(ci, cj, v, rin_b[s]);
     }
 }
 
@@ -125,20 +163,25 @@ MHD_HD inline void flux_full(const S &U, const E &eta, int dir, int i, int j, co
 
 // Adjoint of flux_full: for the flux adjoint lamF (conserved order), calls
 // add(ci, cj, v, value) for every centred-state input (v = 0..7) and
-// add(ci, cj, 8, value) for every eta input.
+// add(ci, cj, 8, value) for every eta input.  The direction-dependent
+// gathers and scatters sit in branches on dir around one copy of the
+// generated face adjoint, so a caller with a run-time (wave-uniform) dir
+// compiles each with constant stencil offsets.
 template <class S, class E, class Add>
 MHD_HD inline void flux_full_adj(const S &U, const E &eta, int dir, int i, int j, const double *par,
                                  const double *rpar, const double *lamF, Add &add)
 {
     double in[32], in_b[32], ff_b[7], rin[14], rin_b[14], d_b[3];
-    gather_face(U, dir, i, j, in);
-    gather_resist(U, eta, dir, i, j, rin);
     if (dir == 0) {
+        gather_face(U, 0, i, j, in);
+        gather_resist(U, eta, 0, i, j, rin);
         ff_b[0] = lamF[0]; ff_b[1] = lamF[1]; ff_b[2] = lamF[2]; ff_b[3] = lamF[3];
         ff_b[4] = lamF[5]; ff_b[5] = lamF[6]; ff_b[6] = lamF[7];
         d_b[0] = -lamF[5]; d_b[1] = lamF[6]; d_b[2] = lamF[7];
         resist_x_adj(rin, rpar, d_b, rin_b);
     } else {
+        gather_face(U, 1, i, j, in);
+        gather_resist(U, eta, 1, i, j, rin);
         ff_b[0] = lamF[0]; ff_b[1] = lamF[2]; ff_b[2] = lamF[3]; ff_b[3] = lamF[1];
         ff_b[4] = lamF[6]; ff_b[5] = lamF[4]; ff_b[6] = lamF[7];
         d_b[0] = lamF[4]; d_b[1] = -lamF[6]; d_b[2] = lamF[7];
@@ -147,28 +190,20 @@ MHD_HD inline void flux_full_adj(const S &U, const E &eta, int dir, int i, int j
     face_flux_adj(in, par, ff_b, in_b);
     // the resistive terms' Bz and By (x faces) / Bz and Bx (y faces) inputs
     // belong to the same four cells: fold them in before scattering
-    for (int c = 0; c < 4; ++c) {
-        if (dir == 0) {
+    if (dir == 0) {
+        MHD_UNROLL
+        for (int c = 0; c < 4; ++c) {
             in_b[8 * c + 6] += rin_b[c];
             in_b[8 * c + 5] += rin_b[4 + c];
-        } else {
+        }
+        scatter_face(0, i, j, in_b, rin_b, add);
+    } else {
+        MHD_UNROLL
+        for (int c = 0; c < 4; ++c) {
             in_b[8 * c + 5] += rin_b[c];       // rotated slot of Bz
             in_b[8 * c + 6] += rin_b[4 + c];   // rotated slot of Bx
         }
-    }
-    for (int c = 0; c < 4; ++c) {
-        int ci, cj;
-        stencil_cell(dir, i, j, c, ci, cj);
-        for (int v = 0; v < 8; ++v) {
-            const double b = in_b[8 * c + v];
-            if (b != 0.0) add(ci, cj, dir == 0 ? v : rot_y(v), b);
-        }
-    }
-    for (int s = 8; s < 14; ++s) {
-        if (rin_b[s] == 0.0) continue;
-        int ci, cj, v;
-        resist_slot(dir, i, j, s, ci, cj, v);
-        add(ci, cj, v, rin_b[s]);
+        scatter_face(1, i, j, in_b, rin_b, add);
     }
 }
 

@@ -42,7 +42,7 @@ EXPORTS = (
     "mhd_adjoint_get_tangent", "mhd_adjoint_set_states", "mhd_adjoint_get_states", "mhd_adjoint_set_seeds",
     "mhd_adjoint_get_grad_states", "mhd_save_subdomain", "mhd_load_subdomain", "mhd_adjoint_get_grad_etas",
     "mhd_adjoint_set_states_device", "mhd_adjoint_get_states_device", "mhd_adjoint_set_seeds_device",
-    "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device",
+    "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device", "mhd_adjoint_set_face_kernel",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -178,6 +178,7 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_set_seeds_device": (I, [P, P]),
         "mhd_adjoint_get_grad_states_device": (I, [P, P]),
         "mhd_adjoint_get_grad_etas_device": (I, [P, P]),
+        "mhd_adjoint_set_face_kernel": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -708,6 +709,13 @@ class ResistiveMHD2DAdjoint:
         flat = np.empty(self._total())
         self._ok(self._lib.mhd_adjoint_get_grad_states(self._h, _dptr(flat)))
         return self._split(flat)
+
+    FACE_AUTO, FACE_PER_FACE, FACE_SWEEP = 0, 1, 2
+
+    def set_face_kernel(self, mode):
+        """Kernel form of the face adjoints: FACE_AUTO, FACE_PER_FACE or
+        FACE_SWEEP (results agree to the rounding of atomic sums)."""
+        self._ok(self._lib.mhd_adjoint_set_face_kernel(self._h, int(mode)))
 
     def grad_eta_all(self):
         n = [self.cfgs[r].nx * self.cfgs[r].ny for r in range(len(self.cfgs))]

@@ -101,8 +101,9 @@ def _generic(U0, seed, s=0.05):
     return U
 
 
+@pytest.mark.parametrize("face", ["per_face", "sweep"])
 @pytest.mark.parametrize("case, tol", [("ot", 1e-12), ("harris", 1e-8)])
-def test_adjoint_matches_tangent_linear(case, tol):
+def test_adjoint_matches_tangent_linear(case, tol, face):
     """<dL/dU0, dU0> + <dL/deta, deta> against the tangent-linear model's
     <W, dU_N> for random directions.  One step agrees to round-off in every
     component; over 20 steps the noisy, under-resolved Harris sheet has
@@ -114,8 +115,9 @@ def test_adjoint_matches_tangent_linear(case, tol):
     nsteps = 20
     eta = _eta_field(cfg, 2)
     W = _weights(cfg, 3)
-    print(f"case adjoint-tlm-{case}: nx={cfg.nx} ny={cfg.ny} steps={nsteps}", flush=True)
+    print(f"case adjoint-tlm-{case}-{face}: nx={cfg.nx} ny={cfg.ny} steps={nsteps}", flush=True)
     a = mhd.ResistiveMHD2DAdjoint([cfg])
+    a.set_face_kernel(a.FACE_SWEEP if face == "sweep" else a.FACE_PER_FACE)
     a.setU0(0, U0)
     a.setEtaField(0, eta)
     a.forward(nsteps, stride=6)
@@ -271,3 +273,33 @@ def test_whole_batch_transfers_host_and_device():
         a.getU_all_device(torch.empty(tot - 1, dtype=torch.float64, device="cuda"))
     with pytest.raises(TypeError):
         a.getU_all_device(torch.empty(tot, dtype=torch.float32, device="cuda"))
+
+
+def test_face_kernel_forms_agree():
+    """The sweep form of the face adjoints (LDS rings, accumulation before
+    HBM) gives the per-face form's gradients on a batch of uneven runs, with
+    and without a resistivity plane (sums of the same terms in other orders)."""
+    cases = [_ot(24), _ot(70, eta=3e-3), _harris(80, 50)]
+    cases = [(c, _generic(U, 60 + k, 0.005 if k == 2 else 0.05)) for k, (c, U) in enumerate(cases)]
+    print("case face-kernel-forms: runs", [(c.nx, c.ny) for c, _ in cases], flush=True)
+    out = {}
+    for face in ("per_face", "sweep"):
+        a = mhd.ResistiveMHD2DAdjoint([c for c, _ in cases])
+        a.set_face_kernel(a.FACE_SWEEP if face == "sweep" else a.FACE_PER_FACE)
+        for r, (c, U0) in enumerate(cases):
+            a.setU0(r, U0)
+            if r != 1:
+                a.setEtaField(r, _eta_field(c, r))
+        a.forward(7, stride=3)
+        for r, (c, _) in enumerate(cases):
+            a.set_seed(r, _weights(c, 70 + r))
+        a.backward()
+        out[face] = [(a.grad_state(r), a.grad_eta(r)) for r in range(len(cases))]
+    for r in range(len(cases)):
+        # the noisy Harris sheet amplifies summation-order differences (see
+        # test_adjoint_matches_tangent_linear); Orszag-Tang does not
+        rtol, atol = (1e-9, 1e-12) if r < 2 else (1e-6, 1e-9)
+        for k in range(2):
+            ref, got = out["per_face"][r][k], out["sweep"][r][k]
+            assert np.all(np.isfinite(got))
+            np.testing.assert_allclose(got, ref, rtol=rtol, atol=atol * np.abs(ref).max())
