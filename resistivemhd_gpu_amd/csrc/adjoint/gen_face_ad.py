@@ -367,17 +367,22 @@ def emit(name, path):
     # (the tangent keeps every candidate: guarding it raised its register
     # pressure on gfx950 without saving time)
     lines += primal_lines(params, inputs, ops)
-    lines += tangent_lines(params, inputs, ops)
     lines += [f"    out[{i}] = {v};" for i, v in enumerate(outputs)]
+    # derivative arithmetic may contract to FMA (only the primal must round
+    # as the reference does, for its branch decisions)
+    lines += ["    {", "    MHD_AD_CONTRACT"]
+    lines += tangent_lines(params, inputs, ops)
     lines += [f"    out_d[{i}] = {v}_d;" for i, v in enumerate(outputs)]
-    lines += ["}", "",
+    lines += ["    }", "}", "",
               f"MHD_HD inline void {name}_adj(const double *in, const double *par, const double *out_b,"
               f" double *in_b)", "{"]
     lines += primal_lines(params, inputs, ops, guards)
     if len(set(segs)) > 1:
         lines += segmented_adjoint_lines(params, inputs, outputs, ops, segs)
     else:
+        lines += ["    {", "    MHD_AD_CONTRACT"]
         lines += adjoint_lines(params, inputs, outputs, ops, guards)
+        lines += ["    }"]
     lines += ["}", ""]
     return lines
 
@@ -415,6 +420,13 @@ def main(argv):
             "// Register barrier: the compiler must treat x as a new value, so a",
             "// segment recomputed for the reverse pass is not merged with (and does",
             "// not keep alive) the first primal pass.",
+            "// The tangent and adjoint passes may contract a*b+c to one FMA (the",
+            "// build keeps contraction off elsewhere, for bitwise parity).",
+            "#if defined(__clang__)",
+            "#define MHD_AD_CONTRACT _Pragma(\"clang fp contract(fast)\")",
+            "#else",
+            "#define MHD_AD_CONTRACT",
+            "#endif",
             "#if defined(__HIP_DEVICE_COMPILE__)",
             "#define MHD_AD_BARRIER(x) asm volatile(\"\" : \"+v\"(x))",
             "#else",
