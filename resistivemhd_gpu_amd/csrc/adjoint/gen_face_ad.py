@@ -27,6 +27,7 @@ adjoint differentiates exactly the branch the forward step took.
 Usage: gen_face_ad.py OUT.h LISTING.ssa[:NAME] ...
 """
 import ast
+import re
 import os
 import sys
 
@@ -145,10 +146,39 @@ def groups(guards):
     return out
 
 
-def primal_lines(params, inputs, ops, guards=None):
+# The adjoint's own primal pass: values that no branch depends on only set
+# the point the derivatives are taken at, so there division and square root
+# use the refined hardware estimates.  Everything a comparison or a guard
+# reads, transitively, keeps IEEE arithmetic, so every branch replays the
+# forward step's decision exactly.
+C_OP_FAST = dict(C_OP, div="{0} * mhd_ad_rcp({1})", sqrt="mhd_ad_sqrt({0})")
+BRANCH_OPS = ("tmin", "tmax", "sgn", "ind")
+
+
+def branch_inputs(ops, guards):
+    """Names some branch (comparison op or guard) depends on, transitively."""
+    need = set()
+    for dst, op, args in ops:
+        if op in BRANCH_OPS:
+            need.update(a for a in args if not is_lit(a))
+    for g in guards or []:
+        if g:
+            need.update(re.findall(r"[A-Za-z_][A-Za-z_0-9]*", g))
+    for dst, op, args in reversed(ops):
+        if dst in need:
+            need.update(a for a in args if not is_lit(a))
+    return need
+
+
+def primal_lines(params, inputs, ops, guards=None, fast=False):
     """Primal statements.  With guards (tangent and adjoint only), an op
     under a guard is evaluated only where the guard holds (its result feeds
-    nothing else there); the plain primal evaluates every op."""
+    nothing else there); the plain primal evaluates every op.  fast: the
+    adjoint's primal arithmetic (C_OP_FAST off every branch's inputs)."""
+    exact = branch_inputs(ops, guards) if fast else None
+
+    def C(dst, op):
+        return C_OP_FAST[op] if fast and dst not in exact else C_OP[op]
     out = [f"    const double {p} = par[{i}];" for i, p in enumerate(params)]
     out += [f"    const double {v} = in[{i}];" for i, v in enumerate(inputs)]
     if guards is None:
@@ -157,13 +187,13 @@ def primal_lines(params, inputs, ops, guards=None):
         if g is None:
             for n in idx:
                 dst, op, args = ops[n]
-                out.append(f"    const double {dst} = {C_OP[op].format(*args)};")
+                out.append(f"    const double {dst} = {C(dst, op).format(*args)};")
         else:
             out += [f"    double {ops[n][0]} = 0.0;" for n in idx]
             out.append(f"    if ({g}) {{")
             for n in idx:
                 dst, op, args = ops[n]
-                out.append(f"    {dst} = {C_OP[op].format(*args)};")
+                out.append(f"    {dst} = {C(dst, op).format(*args)};")
             out.append("    }")
     return out
 
@@ -376,7 +406,7 @@ def emit(name, path):
     lines += ["    }", "}", "",
               f"MHD_HD inline void {name}_adj(const double *in, const double *par, const double *out_b,"
               f" double *in_b)", "{"]
-    lines += primal_lines(params, inputs, ops, guards)
+    lines += primal_lines(params, inputs, ops, guards, fast=True)
     if len(set(segs)) > 1:
         lines += segmented_adjoint_lines(params, inputs, outputs, ops, segs)
     else:
@@ -414,7 +444,8 @@ def main(argv):
             "namespace mhd_ad {",
             "",
             "// Reciprocal for the derivative factors of the tangent and adjoint",
-            "// passes (the primal keeps IEEE division, so branch decisions replay",
+            "// passes and for the adjoint's primal values that no branch depends on",
+            "// (every branch input keeps IEEE division, so branch decisions replay",
             "// the forward step exactly): on the GPU the hardware estimate refined",
             "// by two Newton steps (within an ulp or two); on the host 1/x.",
             "// Register barrier: the compiler must treat x as a new value, so a",
@@ -442,6 +473,20 @@ def main(argv):
             "    return r;",
             "#else",
             "    return 1.0 / x;",
+            "#endif",
+            "}",
+            "",
+            "// Square root for the adjoint's primal pass: x * rsqrt(x) refined by one",
+            "// Newton step on the GPU (0 at 0, NaN below); sqrt on the host.",
+            "MHD_HD inline double mhd_ad_sqrt(double x)",
+            "{",
+            "#if defined(__HIP_DEVICE_COMPILE__)",
+            "    const double r = __builtin_amdgcn_rsq(x);",
+            "    double s = x * r;",
+            "    s = fma(fma(-s, s, x), 0.5 * r, s);",
+            "    return x > 0.0 ? s : (x == 0.0 ? 0.0 : __builtin_nan(\"\"));",
+            "#else",
+            "    return sqrt(x);",
             "#endif",
             "}",
             ""]
