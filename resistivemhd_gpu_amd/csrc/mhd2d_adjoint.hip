@@ -419,22 +419,6 @@ __device__ __forceinline__ void sweep_flush(SweepLds &l, const SweepRun &g, int 
     }
 }
 
-// Plane p of the 16 a sweep stages (0..7 state, 8..13 adjoint 0,1,2,3,6,7,
-// 14 corner-EMF adjoint, 15 resistivity) at column col, or nullptr where the
-// run has none.
-template <int STAGE>
-__device__ __forceinline__ const double *sweep_plane(const SweepRun &g, int p, int col)
-{
-    const size_t o = (size_t)col * g.pitch;
-    if (p < 8) return g.prim + p * g.plane + o;
-    if (p < 14) {
-        const int q = p - 8;
-        return g.L + (q < 4 ? q : q + 2) * g.plane + o;
-    }
-    if (p == 14) return STAGE == 1 ? g.lamE + o : nullptr;
-    return g.eta ? g.eta + o : nullptr;
-}
-
 // Ring slot that plane p of column col occupies.
 __device__ __forceinline__ double *sweep_slot(SweepLds &l, int p, int col)
 {
@@ -442,6 +426,27 @@ __device__ __forceinline__ double *sweep_slot(SweepLds &l, int p, int col)
     if (p < 14) return l.lm[p - 8][col % kSwRing];
     if (p == 14) return l.le[col % kSwRing];
     return l.et[col % kSwRing];
+}
+
+// One wave's share of the loads of a step: wave 0 the 8 state planes of
+// column cs, wave 1 the six adjoint planes and the resistivity of column cl
+// and the corner-EMF adjoint of column ce.  dst_* pick the LDS target
+// (staging or ring slot) per plane.
+template <int STAGE, class Dst>
+__device__ __forceinline__ void sweep_issue(const SweepRun &g, int wave, int cs, int cl, int ce, int j0, int lane,
+                                            Dst dst)
+{
+    if (wave == 0) {
+        const double *base = g.prim + (size_t)cs * g.pitch;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) sweep_dma(base + v * g.plane, j0, g.ny, dst(v, cs), lane);
+    } else {
+        const double *base = g.L + (size_t)cl * g.pitch;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sweep_dma(base + (q < 4 ? q : q + 2) * g.plane, j0, g.ny, dst(8 + q, cl), lane);
+        if (STAGE == 1) sweep_dma(g.lamE + (size_t)ce * g.pitch, j0, g.ny, dst(14, ce), lane);
+        if (g.eta) sweep_dma(g.eta + (size_t)cl * g.pitch, j0, g.ny, dst(15, cl), lane);
+    }
 }
 
 template <int STAGE>
@@ -482,12 +487,17 @@ __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs
     // ia+2, adjoint and resistivity ia, ia+1, corner-EMF adjoint ia-1, ia;
     // the two waves take alternate planes
     for (int k = tid; k < 9 * kSwAcc * kSwRows; k += 128) (&l.acc[0][0][0])[k] = 0.0;
-    for (int p = wave; p < kSwStage; p += 2) {
-        const int c0 = p < 8 ? ia - 2 : (p == 14 ? ia - 1 : ia);
-        const int c1 = p < 8 ? ia + 2 : (p == 14 ? ia : ia + 1);
-        for (int c = c0; c <= c1; ++c) {
-            const double *src = sweep_plane<STAGE>(g, p, c);
-            if (src) sweep_dma(src, j0, g.ny, sweep_slot(l, p, c), lane);
+    if (wave == 0) {
+        for (int c = ia - 2; c <= ia + 2; ++c)
+            for (int v = 0; v < 8; ++v)
+                sweep_dma(g.prim + v * g.plane + (size_t)c * g.pitch, j0, g.ny, l.st[v][c % kSwSt], lane);
+    } else {
+        for (int c = ia; c <= ia + 1; ++c) {
+            for (int q = 0; q < 6; ++q)
+                sweep_dma(g.L + (q < 4 ? q : q + 2) * g.plane + (size_t)c * g.pitch, j0, g.ny, l.lm[q][c % kSwRing],
+                          lane);
+            if (g.eta) sweep_dma(g.eta + (size_t)c * g.pitch, j0, g.ny, l.et[c % kSwRing], lane);
+            if (STAGE == 1) sweep_dma(g.lamE + (size_t)(c - 1) * g.pitch, j0, g.ny, l.le[(c - 1) % kSwRing], lane);
         }
     }
 
@@ -511,11 +521,7 @@ __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs
         if (i + 1 < ib) {
             // the columns step i+1 adds: state i+3, adjoint and resistivity
             // i+2, corner-EMF adjoint i+1
-            for (int p = wave; p < kSwStage; p += 2) {
-                const int c = p < 8 ? i + 3 : (p == 14 ? i + 1 : i + 2);
-                const double *src = sweep_plane<STAGE>(g, p, c);
-                if (src) sweep_dma(src, j0, g.ny, stg[p], lane);
-            }
+            sweep_issue<STAGE>(g, wave, i + 3, i + 2, i + 1, j0, lane, [&](int p, int) { return stg[p]; });
         }
 
         const bool valid = wave == 0 ? (i <= g.nx - 4 && j >= 2 && j <= g.ny - 3)
