@@ -832,6 +832,11 @@ struct mhd_adjoint {
     bool face_kernel_auto_sweep = false;
     double *pool = nullptr;       // lam, lamBar, lamU, lamC, lamE, geta, u0 per run
     double *hist = nullptr, *hists = nullptr, *ckpt = nullptr, *dts = nullptr;
+    // per history slot k, two copies of the batch's run descriptors: A_k
+    // (U = hist[k], U* = hists[k]) and B_k (U = hist[k+1], U* = hists[k]), so
+    // a recomputed step writes its states straight into the history
+    mhd::RunDesc *d_slot_desc = nullptr;
+    size_t slot_desc_cap = 0;   // descriptors
     size_t hist_cap = 0, hists_cap = 0, ckpt_cap = 0, dts_cap = 0;   // doubles
     int nsteps = 0, stride = 0, nckpt = 0;
     bool forward_done = false;
@@ -990,7 +995,36 @@ int layout_storage(mhd_adjoint *a, int nsteps, int stride)
     }
     a->stride = K;
     a->nckpt = nck;
-    return upload_desc(a);
+    const mhd_batch *b = a->fwd;
+    std::vector<mhd::RunDesc> sd((size_t)2 * K * a->nruns);
+    for (int k = 0; k < K; ++k)
+        for (int r = 0; r < a->nruns; ++r) {
+            const AdjDesc &d = a->desc[r];
+            const size_t s8 = (size_t)8 * d.g.plane;
+            mhd::RunDesc A = b->desc[r], B = b->desc[r];
+            A.u = d.hist + k * s8;
+            A.us = d.hists + k * s8;
+            B.u = d.hist + (size_t)std::min(k + 1, K - 1) * s8;
+            B.us = d.hists + k * s8;
+            sd[(size_t)(2 * k) * a->nruns + r] = A;
+            sd[(size_t)(2 * k + 1) * a->nruns + r] = B;
+        }
+    if (a->slot_desc_cap < sd.size()) {
+        AHIP(a, hipStreamSynchronize(stream_of(a)));
+        AHIP(a, hipFree(a->d_slot_desc));
+        a->d_slot_desc = nullptr;
+        a->slot_desc_cap = 0;
+        AHIP(a, hipMalloc(&a->d_slot_desc, sizeof(mhd::RunDesc) * sd.size()));
+        a->slot_desc_cap = sd.size();
+    }
+    AHIP(a, hipMemcpyAsync(a->d_slot_desc, sd.data(), sizeof(mhd::RunDesc) * sd.size(), hipMemcpyHostToDevice,
+                           stream_of(a)));
+    return upload_desc(a);   // synchronises: sd stays alive until the copy is done
+}
+
+const mhd::RunDesc *slot_desc(const mhd_adjoint *a, int k, int which)
+{
+    return a->d_slot_desc + (size_t)(2 * k + which) * a->nruns;
 }
 
 int ensure_dts(mhd_adjoint *a, int nsteps)
@@ -1109,6 +1143,7 @@ int mhd_adjoint_destroy(mhd_adjoint *a)
     (void)hipFree(a->d_sfirst);
     (void)hipFree(a->hist);
     (void)hipFree(a->hists);
+    (void)hipFree(a->d_slot_desc);
     (void)hipFree(a->ckpt);
     (void)hipFree(a->dts);
     (void)hipFree(a->tpool);
@@ -1133,6 +1168,7 @@ int mhd_adjoint_set_eta_field(mhd_adjoint *a, int run, const double *host_eta)
     if (rc) return rc;
     if ((rc = mhd_batch_set_eta_field(a->fwd, run, host_eta))) return afail(a, rc, a->fwd->err);
     a->desc[run].eta = a->fwd->desc[run].eta;
+    a->forward_done = false;   // the recorded rollout used the previous field
     return upload_desc(a);
 }
 
@@ -1228,21 +1264,27 @@ int mhd_adjoint_backward(mhd_adjoint *a)
     hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
     for (int c = a->nckpt - 1; c >= 0; --c) {
         const int s0 = c * a->stride, s1 = std::min(a->nsteps, s0 + a->stride);
-        // recompute the segment with the recorded dt: history slot st - s0
-        // gets U^st and U* = BC(stage 1 of U^st); the segment's last step
-        // keeps both in the live buffers
-        if ((rc = copy_all(a, kCkpt, c, kU, 0))) return rc;
+        // recompute the segment with the recorded dt, each step's states
+        // written straight into history slot st - s0: U^st and U* =
+        // BC(stage 1 of U^st); U^{st+1} starts as a copy of U^st that stage 2
+        // updates in place
+        if ((rc = copy_all(a, kCkpt, c, kHist, 0))) return rc;
         if ((rc = set_replay(a, true, false, s0))) return rc;
-        for (int st = s0; st + 1 < s1; ++st) {
-            if ((rc = copy_all(a, kU, 0, kHist, st - s0))) return rc;
-            if ((rc = mhd_batch_step(b, 1))) return afail(a, rc, b->err);
-            if ((rc = copy_all(a, kUs, 0, kHistS, st - s0))) return rc;
+        for (int k = 0; k < s1 - s0; ++k) {
+            const mhd::RunDesc *A = slot_desc(a, k, 0), *B = slot_desc(a, k, 1);
+            AHIP(a, mhd::launch_stage_batch(1, b->exact, A, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
+                                            b->cur, s));
+            AHIP(a, mhd::launch_bc_batch(b->exact, A, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
+            if (s0 + k + 1 < s1) {
+                if ((rc = copy_all(a, kHist, k, kHist, k + 1))) return rc;
+                AHIP(a, mhd::launch_stage_batch(2, b->exact, B, b->d_item_run, b->total_items, b->xcds,
+                                                b->items_per_xcd, b->cur, s));
+                AHIP(a, mhd::launch_bc_batch(b->exact, B, b->nruns, b->max_nx, b->max_ny, false, true, b->cur ^ 1, s));
+                b->cur ^= 1;
+            }
         }
-        AHIP(a, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
-                                        b->items_per_xcd, b->cur, s));
-        AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
         for (int st = s1 - 1; st >= s0; --st) {
-            const int slot = st + 1 < s1 ? st - s0 : -1;
+            const int slot = st - s0;
             if ((rc = bc_transpose(a, kLam))) return rc;
             hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
             if ((rc = launch_face_adj(a, 2, st, slot))) return rc;
