@@ -832,11 +832,13 @@ struct mhd_adjoint {
     bool face_kernel_auto_sweep = false;
     double *pool = nullptr;       // lam, lamBar, lamU, lamC, lamE, geta, u0 per run
     double *hist = nullptr, *hists = nullptr, *ckpt = nullptr, *dts = nullptr;
-    // per history slot k, two copies of the batch's run descriptors: A_k
-    // (U = hist[k], U* = hists[k]) and B_k (U = hist[k+1], U* = hists[k]), so
-    // a recomputed step writes its states straight into the history
+    // per history slot k, three copies of the batch's run descriptors: A_k
+    // (U = hist[k], U* = hists[k]), B_k (U = hist[k+1], U* = hists[k]) and
+    // C_k (U = the live state, U* = hists[k]), so a step writes its states
+    // straight into the history
     mhd::RunDesc *d_slot_desc = nullptr;
     size_t slot_desc_cap = 0;   // descriptors
+    bool last_in_hist = false;  // the forward pass left its last segment in the history
     size_t hist_cap = 0, hists_cap = 0, ckpt_cap = 0, dts_cap = 0;   // doubles
     int nsteps = 0, stride = 0, nckpt = 0;
     bool forward_done = false;
@@ -996,18 +998,20 @@ int layout_storage(mhd_adjoint *a, int nsteps, int stride)
     a->stride = K;
     a->nckpt = nck;
     const mhd_batch *b = a->fwd;
-    std::vector<mhd::RunDesc> sd((size_t)2 * K * a->nruns);
+    std::vector<mhd::RunDesc> sd((size_t)3 * K * a->nruns);
     for (int k = 0; k < K; ++k)
         for (int r = 0; r < a->nruns; ++r) {
             const AdjDesc &d = a->desc[r];
             const size_t s8 = (size_t)8 * d.g.plane;
-            mhd::RunDesc A = b->desc[r], B = b->desc[r];
+            mhd::RunDesc A = b->desc[r], B = b->desc[r], C = b->desc[r];
             A.u = d.hist + k * s8;
             A.us = d.hists + k * s8;
             B.u = d.hist + (size_t)std::min(k + 1, K - 1) * s8;
             B.us = d.hists + k * s8;
-            sd[(size_t)(2 * k) * a->nruns + r] = A;
-            sd[(size_t)(2 * k + 1) * a->nruns + r] = B;
+            C.us = d.hists + k * s8;
+            sd[(size_t)(3 * k) * a->nruns + r] = A;
+            sd[(size_t)(3 * k + 1) * a->nruns + r] = B;
+            sd[(size_t)(3 * k + 2) * a->nruns + r] = C;
         }
     if (a->slot_desc_cap < sd.size()) {
         AHIP(a, hipStreamSynchronize(stream_of(a)));
@@ -1024,7 +1028,40 @@ int layout_storage(mhd_adjoint *a, int nsteps, int stride)
 
 const mhd::RunDesc *slot_desc(const mhd_adjoint *a, int k, int which)
 {
-    return a->d_slot_desc + (size_t)(2 * k + which) * a->nruns;
+    return a->d_slot_desc + (size_t)(3 * k + which) * a->nruns;
+}
+
+// Steps s0 .. s0+n-1 with every state written into the history: slot k gets
+// U^{s0+k} (slot 0 must hold U^{s0}) and U* of that step.  last_to_live: the
+// final step's U^{s0+n} goes to the live state (the forward pass's result);
+// otherwise the final step stops after stage 1 (the backward pass needs its
+// U* only).  Steps take dt as the control blocks say (record or replay).
+int history_steps(mhd_adjoint *a, int n, bool last_to_live)
+{
+    mhd_batch *b = a->fwd;
+    hipStream_t s = stream_of(a);
+    for (int k = 0; k < n; ++k) {
+        const mhd::RunDesc *A = slot_desc(a, k, 0);
+        AHIP(a, mhd::launch_stage_batch(1, b->exact, A, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
+                                        b->cur, s));
+        AHIP(a, mhd::launch_bc_batch(b->exact, A, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
+        const mhd::RunDesc *B = nullptr;
+        int rc;
+        if (k + 1 < n) {
+            if ((rc = copy_all(a, kHist, k, kHist, k + 1))) return rc;
+            B = slot_desc(a, k, 1);
+        } else if (last_to_live) {
+            if ((rc = copy_all(a, kHist, k, kU, 0))) return rc;
+            B = slot_desc(a, k, 2);
+        }
+        if (B) {
+            AHIP(a, mhd::launch_stage_batch(2, b->exact, B, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
+                                            b->cur, s));
+            AHIP(a, mhd::launch_bc_batch(b->exact, B, b->nruns, b->max_nx, b->max_ny, false, true, b->cur ^ 1, s));
+            b->cur ^= 1;
+        }
+    }
+    return MHD_OK;
 }
 
 int ensure_dts(mhd_adjoint *a, int nsteps)
@@ -1222,9 +1259,18 @@ int mhd_adjoint_forward(mhd_adjoint *a, int nsteps, int stride, int frozen_dt)
     if ((rc = copy_all(a, kU, 0, kCkpt, 0))) return rc;
     for (int c = 0; c < a->nckpt; ++c) {
         const int n = std::min(a->stride, nsteps - c * a->stride);
-        if ((rc = mhd_batch_step(b, n))) return afail(a, rc, b->err);
-        if (c + 1 < a->nckpt && (rc = copy_all(a, kU, 0, kCkpt, c + 1))) return rc;
+        if (c + 1 < a->nckpt) {
+            if ((rc = mhd_batch_step(b, n))) return afail(a, rc, b->err);
+            if ((rc = copy_all(a, kU, 0, kCkpt, c + 1))) return rc;
+        } else {
+            // the last segment keeps its states in the history: the backward
+            // pass starts there without recomputing it
+            if ((rc = mhd_batch_refresh_dt(b))) return afail(a, rc, b->err);
+            if ((rc = copy_all(a, kU, 0, kHist, 0))) return rc;
+            if ((rc = history_steps(a, n, true))) return rc;
+        }
     }
+    a->last_in_hist = true;
     if ((rc = set_replay(a, false, false, 0))) return rc;
     AHIP(a, hipStreamSynchronize(stream_of(a)));
     a->nsteps = nsteps;
@@ -1268,20 +1314,12 @@ int mhd_adjoint_backward(mhd_adjoint *a)
         // written straight into history slot st - s0: U^st and U* =
         // BC(stage 1 of U^st); U^{st+1} starts as a copy of U^st that stage 2
         // updates in place
-        if ((rc = copy_all(a, kCkpt, c, kHist, 0))) return rc;
-        if ((rc = set_replay(a, true, false, s0))) return rc;
-        for (int k = 0; k < s1 - s0; ++k) {
-            const mhd::RunDesc *A = slot_desc(a, k, 0), *B = slot_desc(a, k, 1);
-            AHIP(a, mhd::launch_stage_batch(1, b->exact, A, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
-                                            b->cur, s));
-            AHIP(a, mhd::launch_bc_batch(b->exact, A, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
-            if (s0 + k + 1 < s1) {
-                if ((rc = copy_all(a, kHist, k, kHist, k + 1))) return rc;
-                AHIP(a, mhd::launch_stage_batch(2, b->exact, B, b->d_item_run, b->total_items, b->xcds,
-                                                b->items_per_xcd, b->cur, s));
-                AHIP(a, mhd::launch_bc_batch(b->exact, B, b->nruns, b->max_nx, b->max_ny, false, true, b->cur ^ 1, s));
-                b->cur ^= 1;
-            }
+        if (c == a->nckpt - 1 && a->last_in_hist) {
+            a->last_in_hist = false;   // left there by the forward pass; later segments overwrite it
+        } else {
+            if ((rc = copy_all(a, kCkpt, c, kHist, 0))) return rc;
+            if ((rc = set_replay(a, true, false, s0))) return rc;
+            if ((rc = history_steps(a, s1 - s0, false))) return rc;
         }
         for (int st = s1 - 1; st >= s0; --st) {
             const int slot = st - s0;
