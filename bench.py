@@ -201,6 +201,7 @@ def adjoint_bench(nruns, steps, run_steps):
     a = mhd.ResistiveMHD2DAdjoint([c for c, _, _ in runs])
     for r, (c, kind, seed) in enumerate(runs):
         init(a, r, c, kind, seed)
+    cols = a.autotune()   # as the batched-forward line does; single runs keep their default
     buf = torch.empty(a._total(), dtype=torch.float64, device="cuda")
     a.forward(2)
     a.backward()
@@ -230,6 +231,7 @@ def adjoint_bench(nruns, steps, run_steps):
         "plain_fwd_adj_runs_per_s": nruns * steps / run_steps / t_plain,
         "speedup_vs_plain": t_plain / t_batch,
         "batched_s": t_batch, "plain_s": t_plain, "gradients_finite": bool(finite),
+        "autotuned_sweep_columns": cols,
         "note": "forward, seed transfer on the GPU and backward timed in both legs",
     }
 

@@ -313,6 +313,10 @@ int mhd_adjoint_synchronize(mhd_adjoint *a);
  * rounding of the atomic sums. */
 enum mhd_adj_face_kernel { MHD_ADJ_FACE_AUTO = 0, MHD_ADJ_FACE_PER_FACE = 1, MHD_ADJ_FACE_SWEEP = 2 };
 int mhd_adjoint_set_face_kernel(mhd_adjoint *a, int mode);
+/* Sweep length of the forward (and recompute) stage kernels picked by the
+ * batch autotuner (mhd_batch_autotune) on the initial states; a later
+ * mhd_adjoint_forward starts from U0 as always. */
+int mhd_adjoint_autotune(mhd_adjoint *a, int steps, int *columns);
 
 #ifdef __cplusplus
 }

@@ -1190,6 +1190,18 @@ int mhd_adjoint_destroy(mhd_adjoint *a)
     return MHD_OK;
 }
 
+int mhd_adjoint_autotune(mhd_adjoint *a, int steps, int *columns)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    int rc;
+    // tune on the initial states (the next forward pass starts from U0 again)
+    if ((rc = copy_all(a, kU0, 0, kU, 0))) return rc;
+    if ((rc = mhd_batch_apply_boundary(a->fwd))) return afail(a, rc, a->fwd->err);
+    if ((rc = mhd_batch_autotune(a->fwd, steps, columns))) return afail(a, rc, a->fwd->err);
+    a->forward_done = false;
+    return MHD_OK;
+}
+
 int mhd_adjoint_set_face_kernel(mhd_adjoint *a, int mode)
 {
     if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");

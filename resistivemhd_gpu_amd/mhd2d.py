@@ -43,6 +43,7 @@ EXPORTS = (
     "mhd_adjoint_get_grad_states", "mhd_save_subdomain", "mhd_load_subdomain", "mhd_adjoint_get_grad_etas",
     "mhd_adjoint_set_states_device", "mhd_adjoint_get_states_device", "mhd_adjoint_set_seeds_device",
     "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device", "mhd_adjoint_set_face_kernel",
+    "mhd_adjoint_autotune",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -179,6 +180,7 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_get_grad_states_device": (I, [P, P]),
         "mhd_adjoint_get_grad_etas_device": (I, [P, P]),
         "mhd_adjoint_set_face_kernel": (I, [P, I]),
+        "mhd_adjoint_autotune": (I, [P, I, ctypes.POINTER(I)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -709,6 +711,13 @@ class ResistiveMHD2DAdjoint:
         flat = np.empty(self._total())
         self._ok(self._lib.mhd_adjoint_get_grad_states(self._h, _dptr(flat)))
         return self._split(flat)
+
+    def autotune(self, steps=2):
+        """Sweep length of the forward stage kernels by the batch autotuner
+        (on the initial states); returns the chosen column count."""
+        cols = ctypes.c_int(0)
+        self._ok(self._lib.mhd_adjoint_autotune(self._h, int(steps), ctypes.byref(cols)))
+        return cols.value
 
     FACE_AUTO, FACE_PER_FACE, FACE_SWEEP = 0, 1, 2
 
