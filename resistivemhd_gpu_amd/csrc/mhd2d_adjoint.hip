@@ -17,18 +17,18 @@
 //   1. lam <- BC^T lam
 //   2. stage-2 faces: lamF from lam, then the face adjoint on the centred U*
 //      (flux_full_adj: MUSCL, HLLD, eta J) scattered to lamC and g_eta
-//   3. lamBar = shift^T lamC + lam_nonB / 2 ; lamU_nonB = lam_nonB / 2
+//   3. lamBar = shift^T lamC + lam_nonB / 2
 //   4. lamBar <- BC^T lamBar
-//   5. lamB* = lamBar_B + lam_B ; lamE = -dt curl^T lamB* ; lamU_B = lamB* ;
-//      lamU_nonB += lamBar_nonB
+//   5. lamB* = lamBar_B + lam_B ; lamE = -dt curl^T lamB*
 //   6. stage-1 faces: lamF from lamBar and lamE, face adjoint on the centred U
-//   7. lam = lamU + shift^T lamC  (the adjoint of U^n)
+//   7. lam = lamU + shift^T lamC  (the adjoint of U^n), lamU_nonB =
+//      lam_nonB / 2 + lamBar_nonB, lamU_B = lamB*
 //
-// The face pieces and their branch decisions (minmod, HLLD region, min/max)
-// come from the primal recomputed in exact arithmetic, so the adjoint
-// differentiates the branch the forward step took.  Contributions of faces
-// to cells are summed with hardware FP64 atomics (the gradient's last bits
-// depend on their order).  Boundaries: periodic x with periodic or symmetric
+// The face pieces' branch decisions (minmod, HLLD region, min/max) come
+// from a primal recomputed with IEEE arithmetic on every branch input, so
+// the adjoint differentiates the branch the forward step took.
+// Contributions of faces to cells are summed with FP64 atomics (the
+// gradient's last bits depend on their order).  Boundaries: periodic x with periodic or symmetric
 // y (the symmetric-x fill reads a stale CT face value and is not
 // differentiated).
 #include <hip/hip_runtime.h>
@@ -58,7 +58,7 @@ struct AdjDesc {
     int ntj;                      // 64-row tiles per column
     double *u, *us;               // U^n and U* (the forward batch's buffers)
     const double *eta;            // per-cell resistivity or nullptr (uniform ph.eta)
-    double *lam, *lamBar, *lamU, *lamC, *lamE, *geta;
+    double *lam, *lamBar, *lamC, *lamE, *geta;
     double *u0, *hist, *ckpt;     // initial state, segment states U^n, checkpoints
     double *hists;                // segment states U* (stage-1 results with their boundary)
     double *dts;                  // dt of every step of the rollout
@@ -115,7 +115,7 @@ struct AtomicAdd {
     }
 };
 
-enum Buf { kU0 = 0, kU = 1, kHist = 2, kCkpt = 3, kLam = 4, kLamBar = 5, kLamU = 6, kLamC = 7, kUs = 8, kHistS = 9 };
+enum Buf { kU0 = 0, kU = 1, kHist = 2, kCkpt = 3, kLam = 4, kLamBar = 5, kLamC = 7, kUs = 8, kHistS = 9 };
 
 __device__ __forceinline__ double *buf_ptr(const AdjDesc &d, int which, int slot)
 {
@@ -127,7 +127,6 @@ __device__ __forceinline__ double *buf_ptr(const AdjDesc &d, int which, int slot
     case kCkpt: return d.ckpt + slot * s8;
     case kLam: return d.lam;
     case kLamBar: return d.lamBar;
-    case kLamU: return d.lamU;
     case kUs: return d.us;
     case kHistS: return d.hists + slot * s8;
     default: return d.lamC;
@@ -573,7 +572,7 @@ __device__ __forceinline__ double shift_t(const AdjDesc &d, int v, int i, int j)
     return C[o];
 }
 
-// step 3: lamBar = shift^T lamC + lam_nonB / 2 ; lamU_nonB = lam_nonB / 2
+// step 3: lamBar = shift^T lamC + lam_nonB / 2
 __global__ __launch_bounds__(256) void cell2_kernel(const AdjDesc *runs, const long long *first, int nruns)
 {
     int r, i, j;
@@ -584,7 +583,6 @@ __global__ __launch_bounds__(256) void cell2_kernel(const AdjDesc *runs, const l
     for (int v = 0; v < 8; ++v) {
         const double half = non_b(v) ? 0.5 * d.lam[v * d.g.plane + o] : 0.0;
         d.lamBar[v * d.g.plane + o] = shift_t(d, v, i, j) + half;
-        d.lamU[v * d.g.plane + o] = half;
     }
 }
 
@@ -613,13 +611,14 @@ __global__ __launch_bounds__(256) void ct_adj_kernel(const AdjDesc *runs, const 
         le = -cy * lam_bstar(d, 4, i, j) + cy * lam_bstar(d, 4, i, j + 1) + cx * lam_bstar(d, 5, i, j)
            - cx * lam_bstar(d, 5, i + 1, j);
     d.lamE[o] = le;
-    d.lamU[4 * g.plane + o] = lam_bstar(d, 4, i, j);
-    d.lamU[5 * g.plane + o] = lam_bstar(d, 5, i, j);
-    for (int v = 0; v < 8; ++v)
-        if (non_b(v)) d.lamU[v * g.plane + o] += d.lamBar[v * g.plane + o];
 }
 
-// step 7: lam = lamU + shift^T lamC
+// step 7: lam = lamU + shift^T lamC, with the direct adjoint of U^n
+// lamU = lam_nonB / 2 + lamBar_nonB (the stage-2 average and stage 1's
+// identity term) and lamU_B = lam_B + lamBar_B on the CT-updated faces
+// (i, j >= 1), 0 elsewhere (CT overwrites B from B^n there).  In place: no
+// other thread reads this cell's lam (ct_adj, which reads neighbours, ran
+// before).
 __global__ __launch_bounds__(256) void cell1_kernel(const AdjDesc *runs, const long long *first, int nruns)
 {
     int r, i, j;
@@ -627,7 +626,11 @@ __global__ __launch_bounds__(256) void cell1_kernel(const AdjDesc *runs, const l
     const AdjDesc &d = runs[r];
     if (j >= d.g.ny) return;
     const size_t o = (size_t)i * d.g.pitch + j;
-    for (int v = 0; v < 8; ++v) d.lam[v * d.g.plane + o] = d.lamU[v * d.g.plane + o] + shift_t(d, v, i, j);
+    for (int v = 0; v < 8; ++v) {
+        const size_t ov = v * d.g.plane + o;
+        const double lu = non_b(v) ? 0.5 * d.lam[ov] + d.lamBar[ov] : lam_bstar(d, v, i, j);
+        d.lam[ov] = lu + shift_t(d, v, i, j);
+    }
 }
 
 // Sets every run's dt replay: dt_in = dts (or nullptr), dt_out = dts (or
@@ -830,7 +833,7 @@ struct mhd_adjoint {
     int sweep_cols = 0;
     int face_kernel = MHD_ADJ_FACE_AUTO;
     bool face_kernel_auto_sweep = false;
-    double *pool = nullptr;       // lam, lamBar, lamU, lamC, lamE, geta, u0 per run
+    double *pool = nullptr;       // lam, lamBar, lamC, lamE, geta, u0 per run
     double *hist = nullptr, *hists = nullptr, *ckpt = nullptr, *dts = nullptr;
     // per history slot k, three copies of the batch's run descriptors: A_k
     // (U = hist[k], U* = hists[k]), B_k (U = hist[k+1], U* = hists[k]) and
@@ -1136,7 +1139,7 @@ int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out)
         d.u = rd.u;
         d.us = rd.us;
         first[r + 1] = first[r] + (long long)rd.g.nx * d.ntj;
-        total += (size_t)(4 * 8 + 2 + 8) * rd.g.plane;
+        total += (size_t)(3 * 8 + 2 + 8) * rd.g.plane;   // lam, lamBar, lamC; lamE, geta; u0
     }
     a->total_tiles = first[nruns];
     hipError_t e;
@@ -1155,7 +1158,6 @@ int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out)
         const size_t p8 = (size_t)8 * d.g.plane;
         d.lam = a->pool + off; off += p8;
         d.lamBar = a->pool + off; off += p8;
-        d.lamU = a->pool + off; off += p8;
         d.lamC = a->pool + off; off += p8;
         d.lamE = a->pool + off; off += d.g.plane;
         d.geta = a->pool + off; off += d.g.plane;
