@@ -96,6 +96,11 @@ def cpu_baseline(nx, ny, steps, workload="harris"):
 BATCH_SIZES = (64, 128, 192, 256, 384, 512)
 
 
+def batch_workload(nruns):
+    return (f"{nruns} runs: Orszag-Tang / Harris 50/50, N in {list(BATCH_SIZES)} squared, "
+            "eta log-uniform [1e-4, 1e-2], seeded per run")
+
+
 def batch_configs(mhd, nruns, seed0=0):
     """The batched workload: run s draws (setup, N, eta) from a generator
     seeded with seed0 + s: Orszag-Tang or Harris sheet (50/50) on N x N
@@ -172,8 +177,7 @@ def batch_bench(nruns, steps, plain_steps, run_steps, numerics):
         "batched_zone_cycles_per_s": cells * steps / t_batch,
         "timed_steps": {"batched": steps, "plain": plain_steps},
         "autotuned_sweep_columns": sweep, "autotune_s": t_tune, "crashed": crashed,
-        "workload": f"{nruns} runs: Orszag-Tang / Harris 50/50, N in {list(BATCH_SIZES)} squared, "
-                    "eta log-uniform [1e-4, 1e-2], seeded per run",
+        "workload": batch_workload(nruns),
     }
 
 
@@ -351,7 +355,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=["harris", "turbulence", "batch", "adjoint", "gradcheck"],
                     default="harris")
-    ap.add_argument("--adj-steps", type=int, default=40, help="timed forward+adjoint steps per run")
+    ap.add_argument("--adj-steps", type=int, default=20, help="timed forward+adjoint steps per run")
     ap.add_argument("--check-nx", type=int, default=1024)
     ap.add_argument("--check-ny", type=int, default=512)
     ap.add_argument("--check-runs", type=int, default=32)
@@ -361,6 +365,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=40, help="timed batched steps")
     ap.add_argument("--plain-steps", type=int, default=10, help="timed steps per run of the plain loop")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form line in the default run")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the batched forward+adjoint line in the default run")
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=0, help="interior cells along x (0: 4096 harris, 8192 turbulence)")
@@ -392,7 +398,7 @@ def main():
         print(json.dumps({"metric": "batched forward+adjoint runs/s vs plain per-instance loop",
                           "value": res["batched_fwd_adj_runs_per_s"], "unit": "runs/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
-                          "config": {"workload": batch_configs.__doc__.split(":")[0].strip()}, "adjoint": res}),
+                          "config": {"workload": batch_workload(args.batch_runs)}, "adjoint": res}),
               flush=True)
         return
     if args.workload == "gradcheck":
@@ -507,6 +513,8 @@ def main():
         del src, dst
     if not multi and not args.no_batch and not args.no_cpu:
         line["batched"] = batch_bench(args.batch_runs, args.batch_steps, args.plain_steps, 500, args.numerics)
+    if not multi and not args.no_adjoint and not args.no_cpu and args.numerics == "exact":
+        line["batched_fwd_adjoint"] = adjoint_bench(args.batch_runs, args.adj_steps, 500)
     if not args.no_cpu:
         if multi:
             line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_nx, args.cpu_steps, "turbulence")
