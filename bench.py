@@ -319,7 +319,7 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
         U[..., 7] = U0[..., 7] - 0.5 * rho * (vx * vx + (U0[..., 2] / rho) ** 2) + 0.5 * rho * (u * u + v * v)
         return U
 
-    errs, tl_errs = [], []
+    errs, tl_errs, mags = [], [], []
     drng = np.random.default_rng(seed + 7)
     for k in range(ndirs):
         de = drng.normal(size=eta0.shape) * 1e-3
@@ -342,10 +342,12 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
         one.tangent(steps)
         tl = float((2.0 * mask * (one.getU(0) - target[0]) * one.get_tangent(0)).sum())
         tl_errs.append(abs(ad - tl) / abs(tl))
+        mags.append({"adjoint": ad, "tangent": tl, "fd": fd})
     return {
         "workload": f"harris {nx}x{ny}, batch {nruns}, {steps} steps", "directions": ndirs,
         "fd_rel_err_max": max(errs), "fd_rel_err_median": float(np.median(errs)), "fd_rel_errs": errs,
-        "tangent_rel_err_max": max(tl_errs), "tangent_rel_errs": tl_errs,
+        "tangent_rel_err_max": max(tl_errs), "tangent_rel_errs": tl_errs, "values": mags,
+        "loss": float((mask * (UN[0] - target[0]) ** 2).sum()),
         "fwd_adj_seconds_batch": t_grad, "setup_seconds": t1 - t0,
     }
 

@@ -81,3 +81,43 @@ def test_sweep_lengths_agree(sweep):
     U0 = boundary_applied(cfg, mhd.fill_harris(cfg, seed=5, noise_amp=1e-2))
     eta = mhd.fill_harris_eta(cfg)
     _compare(cfg, U0, 20, "exact", eta=eta, sweep=sweep, tag=f"sweep{sweep}")
+
+
+def _div_b_interior(cfg, U):
+    g = mhd.GHOST
+    bx, by = U[..., 4], U[..., 5]
+    return (bx[g:-g, g:-g] - bx[g - 1:-g - 1, g:-g]) / cfg.dx + (by[g:-g, g:-g] - by[g:-g, g - 1:-g - 1]) / cfg.dy
+
+
+def test_harris_4096x2048_full_size_properties():
+    """The headline case at its full size (Harris sheet 4096 x 2048,
+    periodic x, symmetric y, per-cell eta, 20 steps): the discrete
+    divergence of B stays at round-off, and shifting the start (state and
+    eta plane) along the periodic x axis shifts the result bitwise."""
+    cfg = mhd.harris_config(4096, 2048)
+    steps, g = 20, mhd.GHOST
+    print(f"case harris4096x2048-properties: steps={steps}", flush=True)
+    sim = mhd.ResistiveMHD2D(cfg)
+    sim.initHarris(seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3)
+    U0 = sim.getU()
+    eta = mhd.fill_harris_eta(cfg, 1e-3, 5e-3)
+    sim.steps(steps)
+    assert not sim.checkCalculationIsCrashed()
+    U1 = sim.getU()
+    bscale = np.abs(U1[..., 4:6]).max() / min(cfg.dx, cfg.dy)
+    div = np.abs(_div_b_interior(cfg, U1)).max()
+    print(f"max|divB| {div:.2e} (B/dx {bscale:.2e})", flush=True)
+    assert div < 1e-12 * bscale
+    sx = 1024
+    Us = U0.copy()
+    Us[g:-g] = np.roll(U0[g:-g], sx, axis=0)
+    es = eta.copy()
+    es[g:-g] = np.roll(eta[g:-g], sx, axis=0)
+    del U0
+    sim2 = mhd.ResistiveMHD2D(cfg)
+    sim2.initializeU(Us)
+    sim2.setEtaField(es)
+    del Us, es
+    sim2.steps(steps)
+    assert np.array_equal(np.roll(U1[g:-g, g:-g], sx, axis=0), sim2.getU()[g:-g, g:-g])
+    assert sim2.dt == sim.dt
