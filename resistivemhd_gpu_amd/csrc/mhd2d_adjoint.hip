@@ -879,14 +879,28 @@ int upload_desc(mhd_adjoint *a)
 dim3 tiles_grid(const mhd_adjoint *a) { return dim3((unsigned)((a->total_tiles + 3) / 4)); }
 dim3 tiles_grid1(const mhd_adjoint *a) { return dim3((unsigned)a->total_tiles); }
 
-// Sweep items: a 64-row strip of a run and `cols` face columns; enough items
-// for about 16 rounds of resident workgroups (two per CU), 8 to 64 columns.
+// Sweep items: a 64-row strip of a run and `cols` face columns.  cols (8 to
+// 64) minimises rounds of resident workgroups (two per CU) times the
+// columns each item sweeps plus its start-up (about three columns' worth:
+// loading five state columns, flushing four accumulator columns), so the
+// last round is not left half empty.
 int layout_sweep(mhd_adjoint *a)
 {
     long long strips_cols = 0;
     for (const AdjDesc &d : a->desc) strips_cols += (long long)d.ntj * (d.g.nx - 4);
-    const long long target = 16LL * 2 * std::max(1, a->fwd->num_cu);
-    a->sweep_cols = (int)std::min<long long>(64, std::max<long long>(8, strips_cols / target));
+    const long long slots = 2LL * std::max(1, a->fwd->num_cu);
+    const long long target = 16 * slots;
+    double best_cost = 1e300;
+    a->sweep_cols = 8;
+    for (int cols = 8; cols <= 64; ++cols) {
+        long long items = 0;
+        for (const AdjDesc &d : a->desc) items += (long long)d.ntj * ((d.g.nx - 4 + cols - 1) / cols);
+        const double cost = (double)((items + slots - 1) / slots) * (cols + 3);
+        if (cost < best_cost - 1e-9) {
+            best_cost = cost;
+            a->sweep_cols = cols;
+        }
+    }
     std::vector<long long> first(a->nruns + 1, 0);
     for (int r = 0; r < a->nruns; ++r) {
         const AdjDesc &d = a->desc[r];
