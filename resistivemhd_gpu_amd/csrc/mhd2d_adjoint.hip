@@ -836,9 +836,9 @@ struct mhd_adjoint {
     double *pool = nullptr;       // lam, lamBar, lamC, lamE, geta, u0 per run
     double *hist = nullptr, *hists = nullptr, *ckpt = nullptr, *dts = nullptr;
     // per history slot k, three copies of the batch's run descriptors: A_k
-    // (U = hist[k], U* = hists[k]), B_k (U = hist[k+1], U* = hists[k]) and
-    // C_k (U = the live state, U* = hists[k]), so a step writes its states
-    // straight into the history
+    // (U = hist[k], U* = hists[k]), B_k (as A_k, stage 2 writing hist[k+1])
+    // and C_k (as A_k, stage 2 writing the live state), so a step reads and
+    // writes its states straight in the history
     mhd::RunDesc *d_slot_desc = nullptr;
     size_t slot_desc_cap = 0;   // descriptors
     bool last_in_hist = false;  // the forward pass left its last segment in the history
@@ -1020,12 +1020,13 @@ int layout_storage(mhd_adjoint *a, int nsteps, int stride)
         for (int r = 0; r < a->nruns; ++r) {
             const AdjDesc &d = a->desc[r];
             const size_t s8 = (size_t)8 * d.g.plane;
-            mhd::RunDesc A = b->desc[r], B = b->desc[r], C = b->desc[r];
+            mhd::RunDesc A = b->desc[r];
             A.u = d.hist + k * s8;
             A.us = d.hists + k * s8;
-            B.u = d.hist + (size_t)std::min(k + 1, K - 1) * s8;
-            B.us = d.hists + k * s8;
-            C.us = d.hists + k * s8;
+            A.uo = nullptr;
+            mhd::RunDesc B = A, C = A;
+            B.uo = d.hist + (size_t)std::min(k + 1, K - 1) * s8;
+            C.uo = b->desc[r].u;
             sd[(size_t)(3 * k) * a->nruns + r] = A;
             sd[(size_t)(3 * k + 1) * a->nruns + r] = B;
             sd[(size_t)(3 * k + 2) * a->nruns + r] = C;
@@ -1062,19 +1063,21 @@ int history_steps(mhd_adjoint *a, int n, bool last_to_live)
         AHIP(a, mhd::launch_stage_batch(1, b->exact, A, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
                                         b->cur, s));
         AHIP(a, mhd::launch_bc_batch(b->exact, A, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
-        const mhd::RunDesc *B = nullptr;
-        int rc;
+        // stage 2 reads U^n and U* from slot k and writes U^{n+1} into slot
+        // k+1 (or the live state); the boundary fill follows it there
+        const mhd::RunDesc *B = nullptr, *after = nullptr;
         if (k + 1 < n) {
-            if ((rc = copy_all(a, kHist, k, kHist, k + 1))) return rc;
             B = slot_desc(a, k, 1);
+            after = slot_desc(a, k + 1, 0);
         } else if (last_to_live) {
-            if ((rc = copy_all(a, kHist, k, kU, 0))) return rc;
             B = slot_desc(a, k, 2);
+            after = b->d_desc;
         }
         if (B) {
             AHIP(a, mhd::launch_stage_batch(2, b->exact, B, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
                                             b->cur, s));
-            AHIP(a, mhd::launch_bc_batch(b->exact, B, b->nruns, b->max_nx, b->max_ny, false, true, b->cur ^ 1, s));
+            AHIP(a, mhd::launch_bc_batch(b->exact, after, b->nruns, b->max_nx, b->max_ny, false, true, b->cur ^ 1,
+                                         s));
             b->cur ^= 1;
         }
     }

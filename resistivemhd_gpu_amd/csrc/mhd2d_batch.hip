@@ -122,9 +122,7 @@ int default_seg(const mhd_batch *b)
 int mhd_batch_refresh_dt(mhd_batch *b)
 {
     if (b->dt_valid) return MHD_OK;
-    for (int r = 0; r < b->nruns; ++r)
-        BHIP(b, mhd::launch_dt_all(b->exact, false, b->desc[r].u, b->desc[r].ctrl, b->cur, b->desc[r].g,
-                                   b->desc[r].ph, b->stream));
+    BHIP(b, mhd::launch_dt_all_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->cur, b->stream));
     b->dt_valid = true;
     return MHD_OK;
 }

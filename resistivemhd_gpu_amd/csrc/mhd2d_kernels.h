@@ -25,6 +25,7 @@ struct RunDesc {
     Grid g;
     Phys ph;
     double *u, *us, *eta, *side;
+    double *uo;   // stage 2 writes U^{n+1} here instead of over u (nullptr: in place)
     Ctrl *ctrl;
     int bc_x, bc_y;
     int first_item;
@@ -62,6 +63,9 @@ hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const 
 hipError_t launch_bc_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int max_ny, bool stage_state,
                            bool with_dt, int slot, hipStream_t s);
 int stage_batch_waves_per_cu(bool exact);
+// Per-cell dt (and the NaN scan) of every run's state into its control
+// block's slot, one launch for the whole batch.
+hipError_t launch_dt_all_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int slot, hipStream_t s);
 hipError_t launch_unpack(const double *aos, double *u, const Grid &g, hipStream_t s);
 hipError_t launch_pack(const double *u, double *aos, const Grid &g, hipStream_t s);
 hipError_t launch_scatter_plane(const double *src, double *dst, const Grid &g, hipStream_t s);

@@ -470,7 +470,8 @@ void rk_stage_batch_kernel(const RunDesc *__restrict__ runs, const int *__restri
     if (STAGE == 1)
         stage_item<1, EXACT, 0>(item - d.first_item, d.u, d.us, d.u, d.eta, d.side, nullptr, d.ctrl, g, ph, cur);
     else
-        stage_item<2, EXACT, 0>(item - d.first_item, d.us, d.u, d.u, d.eta, d.side, nullptr, d.ctrl, g, ph, cur);
+        stage_item<2, EXACT, 0>(item - d.first_item, d.us, d.uo ? d.uo : d.u, d.u, d.eta, d.side, nullptr, d.ctrl, g,
+                                ph, cur);
 }
 
 // ------------------------------------------------------------ boundaries
@@ -652,8 +653,8 @@ __global__ __launch_bounds__(256) void bc_y_batch_kernel(const RunDesc *__restri
 // INTERIOR: the dt of the MPI library's calculateDt, interior cells only
 // (M/resistiveMHD2D.cu:489-524); the NaN scan still covers every cell.
 template <bool EXACT, bool INTERIOR>
-__global__ __launch_bounds__(256) void dt_all_kernel(const double *__restrict__ u, Ctrl *ctrl, int slot, Grid g,
-                                                     Phys ph)
+__device__ __forceinline__ void dt_all_body(const double *__restrict__ u, Ctrl *ctrl, int slot, const Grid &g,
+                                            const Phys &ph)
 {
     double dloc = __longlong_as_double((long long)kInfBits);
     int nanflag = 0;
@@ -675,6 +676,34 @@ __global__ __launch_bounds__(256) void dt_all_kernel(const double *__restrict__ 
     accumulate_dt(ctrl, slot, dloc);
     const unsigned long long any = __ballot(nanflag);
     if ((threadIdx.x & 63) == 0 && any) atomicOr(&ctrl->crashed, 1);
+}
+
+template <bool EXACT, bool INTERIOR>
+__global__ __launch_bounds__(256) void dt_all_kernel(const double *__restrict__ u, Ctrl *ctrl, int slot, Grid g,
+                                                     Phys ph)
+{
+    dt_all_body<EXACT, INTERIOR>(u, ctrl, slot, g, ph);
+}
+
+// The same for every run of a batch (blockIdx.y = run).
+template <bool EXACT>
+__global__ __launch_bounds__(256) void dt_all_batch_kernel(const RunDesc *__restrict__ runs, int slot)
+{
+    const RunDesc &d = runs[blockIdx.y];
+    const Grid g = d.g;
+    const Phys ph = d.ph;
+    dt_all_body<EXACT, false>(d.u, d.ctrl, slot, g, ph);
+}
+
+__global__ void ctrl_reset_batch_kernel(const RunDesc *__restrict__ runs, int nruns, int slot)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nruns) {
+        Ctrl *c = runs[r].ctrl;
+        c->acc[slot] = kInfBits;
+        c->first_nan[slot] = 0;
+        c->crashed = 0;
+    }
 }
 
 __global__ void ctrl_reset_kernel(Ctrl *ctrl, int slot)
@@ -991,6 +1020,16 @@ hipError_t launch_dt_all(bool exact, bool interior, const double *u, Ctrl *ctrl,
     if (exact) { if (interior) MHD_LAUNCH_DT(true, true); else MHD_LAUNCH_DT(true, false); }
     else { if (interior) MHD_LAUNCH_DT(false, true); else MHD_LAUNCH_DT(false, false); }
 #undef MHD_LAUNCH_DT
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_all_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(ctrl_reset_batch_kernel, dim3((nruns + 63) / 64), dim3(64), 0, s, runs, nruns, slot);
+    // a few blocks per run: the batch as a whole fills the machine
+    const dim3 grid(max_nx < 64 ? max_nx : 64, nruns);
+    if (exact) hipLaunchKernelGGL((dt_all_batch_kernel<true>), grid, dim3(256), 0, s, runs, slot);
+    else hipLaunchKernelGGL((dt_all_batch_kernel<false>), grid, dim3(256), 0, s, runs, slot);
     return hipGetLastError();
 }
 
