@@ -278,7 +278,7 @@ int mhd_adjoint_set_state(mhd_adjoint *a, int run, const double *host_aos);     
 int mhd_adjoint_set_dts(mhd_adjoint *a, int run, const double *host_dts, int nsteps);
 int mhd_adjoint_get_dts(mhd_adjoint *a, int run, double *host_dts, int nsteps);
 int mhd_adjoint_forward(mhd_adjoint *a, int nsteps, int stride, int frozen_dt);
-int mhd_adjoint_get_state(mhd_adjoint *a, int run, double *host_aos);   /* U_N, valid until the backward pass */
+int mhd_adjoint_get_state(mhd_adjoint *a, int run, double *host_aos);   /* U_N (until the next forward or tangent) */
 int mhd_adjoint_set_seed(mhd_adjoint *a, int run, const double *host_aos);   /* dL/dU_N */
 int mhd_adjoint_backward(mhd_adjoint *a);
 int mhd_adjoint_get_grad_state(mhd_adjoint *a, int run, double *host_aos);   /* dL/dU0 */
