@@ -240,7 +240,7 @@ def adjoint_bench(nruns, steps, run_steps):
     }
 
 
-def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
+def gradient_check(nruns, nx, ny, steps, ndirs, seed=0, degenerate_tol=0.0, fd_h=1e-4):
     """Harris sheet nx x ny, batch of nruns: 'observed' final states from a
     seeded eta field per run (sum of 6 Gaussians, amplitudes log-uniform in
     [1e-4, 5e-3]); loss = |(rho, Bx, By)(T) - observed|^2 from eta = 1e-3;
@@ -249,7 +249,9 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
     directions with every dt frozen at its recorded value.  The initial vx
     carries seeded noise of amplitude 1e-3 like vy, so that MUSCL's limiter
     is not at its kink (vx = 0 exactly has directional derivatives but no
-    gradient)."""
+    gradient).  degenerate_tol > 0 switches the adjoint and the tangent to
+    the degenerate-limit derivative at HLLD star states whose denominator
+    is below that fraction of rho (S - u)^2."""
     import numpy as np
     import resistivemhd_gpu_amd as mhd
     g = mhd.GHOST
@@ -285,6 +287,7 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
     mask[g:-g, g:-g, [0, 4, 5]] = 1.0
 
     a = mhd.ResistiveMHD2DAdjoint(cfgs)
+    a.set_degenerate_tol(degenerate_tol)
     for r in range(nruns):
         a.setU0(r, U0)
         a.setEtaField(r, eta0)
@@ -303,6 +306,7 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
     del a
 
     one = mhd.ResistiveMHD2DAdjoint([cfg])
+    one.set_degenerate_tol(degenerate_tol)
 
     def loss(U, eta):
         one.setU0(0, U)
@@ -319,15 +323,20 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
         U[..., 7] = U0[..., 7] - 0.5 * rho * (vx * vx + (U0[..., 2] / rho) ** 2) + 0.5 * rho * (u * u + v * v)
         return U
 
-    errs, tl_errs, mags = [], [], []
+    errs, tl_errs, mags, spreads = [], [], [], []
     drng = np.random.default_rng(seed + 7)
     for k in range(ndirs):
         de = drng.normal(size=eta0.shape) * 1e-3
         dvx = drng.normal(size=rho.shape) * 1e-3
         dvy = drng.normal(size=rho.shape) * 1e-3
         ad = float((geta * de).sum() + (gvx * dvx)[g:-g, g:-g].sum() + (gvy * dvy)[g:-g, g:-g].sum())
-        h = 1e-4
-        fd = (loss(with_velocity(dvx, dvy, h), eta0 + h * de) - loss(with_velocity(dvx, dvy, -h), eta0 - h * de)) / (2 * h)
+        def central(h):
+            return (loss(with_velocity(dvx, dvy, h), eta0 + h * de)
+                    - loss(with_velocity(dvx, dvy, -h), eta0 - h * de)) / (2 * h)
+        fd = central(fd_h)
+        # the differences' own spread between steps h and 10 h: where the map
+        # is not smooth at that scale, no derivative can match them
+        spreads.append(abs(central(10 * fd_h) - fd) / abs(fd))
         errs.append(abs(ad - fd) / abs(fd))
         # tangent-linear model along the same direction (interior cells; the
         # boundary fill of U0 is part of the rollout)
@@ -345,8 +354,10 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0):
         mags.append({"adjoint": ad, "tangent": tl, "fd": fd})
     return {
         "workload": f"harris {nx}x{ny}, batch {nruns}, {steps} steps", "directions": ndirs,
+        "degenerate_tol": degenerate_tol, "fd_h": fd_h,
         "fd_rel_err_max": max(errs), "fd_rel_err_median": float(np.median(errs)), "fd_rel_errs": errs,
         "tangent_rel_err_max": max(tl_errs), "tangent_rel_errs": tl_errs, "values": mags,
+        "fd_step_spread": spreads, "fd_step_spread_median": float(np.median(spreads)),
         "loss": float((mask * (UN[0] - target[0]) ** 2).sum()),
         "fwd_adj_seconds_batch": t_grad, "setup_seconds": t1 - t0,
     }
@@ -363,6 +374,9 @@ def main():
     ap.add_argument("--check-runs", type=int, default=32)
     ap.add_argument("--check-steps", type=int, default=2000)
     ap.add_argument("--check-dirs", type=int, default=16)
+    ap.add_argument("--degenerate-tol", type=float, default=0.0,
+                    help="gradcheck: derivative rule at degenerate HLLD star states (0: exact)")
+    ap.add_argument("--fd-h", type=float, default=1e-4, help="gradcheck: central-difference step")
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=40, help="timed batched steps")
     ap.add_argument("--plain-steps", type=int, default=10, help="timed steps per run of the plain loop")
@@ -404,7 +418,8 @@ def main():
               flush=True)
         return
     if args.workload == "gradcheck":
-        res = gradient_check(args.check_runs, args.check_nx, args.check_ny, args.check_steps, args.check_dirs)
+        res = gradient_check(args.check_runs, args.check_nx, args.check_ny, args.check_steps, args.check_dirs,
+                             degenerate_tol=args.degenerate_tol, fd_h=args.fd_h)
         print(json.dumps({"metric": "adjoint gradient vs tangent-linear model and central differences",
                           "value": res["tangent_rel_err_max"], "unit": "relative", "higher_is_better": False,
                           "dtype": "f64", "data": "synthetic", "gradcheck": res}), flush=True)

@@ -313,6 +313,17 @@ int mhd_adjoint_synchronize(mhd_adjoint *a);
  * rounding of the atomic sums. */
 enum mhd_adj_face_kernel { MHD_ADJ_FACE_AUTO = 0, MHD_ADJ_FACE_PER_FACE = 1, MHD_ADJ_FACE_SWEEP = 2 };
 int mhd_adjoint_set_face_kernel(mhd_adjoint *a, int mode);
+/* Derivative rule at degenerate HLLD star states (no counterpart in the
+ * reference, which has no derivatives).  The star-state quotients of
+ * P/hlld.cu:332-338 and 345-351 divide by rho (S - u)(S - SM) - Bx^2 + EPS,
+ * which is zero up to rounding where the transverse field vanishes and the
+ * fast speed equals the Alfven speed; their derivatives there are of the
+ * order of 1 / rounding noise.  With tol > 0 the tangent and the adjoint
+ * take, on the faces where |rho (S - u)(S - SM) - Bx^2| < tol rho (S - u)^2,
+ * the derivative of the degenerate limit (tangential star components equal
+ * to the outer ones, Miyoshi & Kusano 2005).  The forward step is unchanged.
+ * tol = 0 (the default) differentiates the step exactly.  tol in [0, 1). */
+int mhd_adjoint_set_degenerate_tol(mhd_adjoint *a, double tol);
 /* Sweep length of the forward (and recompute) stage kernels picked by the
  * batch autotuner (mhd_batch_autotune) on the initial states; a later
  * mhd_adjoint_forward starts from U0 as always. */

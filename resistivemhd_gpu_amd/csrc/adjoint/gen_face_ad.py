@@ -125,7 +125,7 @@ def parse(path):
     return params, inputs, outputs, low.ops, segs
 
 
-C_OP = {"add": "{0} + {1}", "sub": "{0} - {1}", "mul": "{0} * {1}", "div": "{0} / {1}", "neg": "-{0}",
+C_OP = {"dlim": "{0}", "add": "{0} + {1}", "sub": "{0} - {1}", "mul": "{0} * {1}", "div": "{0} / {1}", "neg": "-{0}",
         "copy": "{0}", "sqrt": "sqrt({0})", "fabs": "fabs({0})", "tmin": "(({1}) < ({0}) ? ({1}) : ({0}))",
         "tmax": "(({0}) < ({1}) ? ({1}) : ({0}))", "sgn": "(double)(({0} > 0.0) - ({0} < 0.0))",
         "ind": "(double){0}"}
@@ -152,21 +152,28 @@ def groups(guards):
 # reads, transitively, keeps IEEE arithmetic, so every branch replays the
 # forward step's decision exactly.
 C_OP_FAST = dict(C_OP, div="{0} * mhd_ad_rcp({1})", sqrt="mhd_ad_sqrt({0})")
-BRANCH_OPS = ("tmin", "tmax", "sgn", "ind")
+BRANCH_OPS = ("tmin", "tmax", "sgn", "ind", "dlim")
+
+
+def names_in(args):
+    """Variable names an op reads: its operands, and the names inside the
+    condition text of an ind op."""
+    return {n for a in args if not is_lit(a) for n in re.findall(r"[A-Za-z_][A-Za-z_0-9]*", a)}
 
 
 def branch_inputs(ops, guards):
-    """Names some branch (comparison op or guard) depends on, transitively."""
+    """Names some branch (comparison op or guard) depends on, transitively.
+    dlim(x, limit, c) branches on c only."""
     need = set()
     for dst, op, args in ops:
         if op in BRANCH_OPS:
-            need.update(a for a in args if not is_lit(a))
+            need.update(names_in(args[2:] if op == "dlim" else args))
     for g in guards or []:
         if g:
-            need.update(re.findall(r"[A-Za-z_][A-Za-z_0-9]*", g))
+            need.update(names_in([g]))
     for dst, op, args in reversed(ops):
         if dst in need:
-            need.update(a for a in args if not is_lit(a))
+            need.update(names_in(args))
     return need
 
 
@@ -245,6 +252,8 @@ def tangent_ops(ops, d):
             e = f"(({a}) < ({b}) ? {d(b)} : {d(a)})"
         elif op in ("sgn", "ind"):
             e = "0.0"
+        elif op == "dlim":
+            e = f"({args[2]} != 0.0 ? {d(b)} : {d(a)})"
         else:
             raise ValueError(op)
         out.append(f"{dst}_d = {e};")
@@ -288,12 +297,33 @@ def reverse_op(dst, op, args, active, ren):
         return out
     elif op in ("sgn", "ind"):
         return out
+    elif op == "dlim":
+        c = V(args[2])
+        if active(b):
+            out.append(f"    if ({c} != 0.0) {b}_b += {t};")
+        if active(a):
+            out.append(f"    if ({c} == 0.0) {a}_b += {t};")
+        return out
     else:
         raise ValueError(op)
     for v, e in acc:
         if active(v):
             out.append(f"    {v}_b += {e};")
     return out
+
+
+def grad_live(ops, outputs):
+    """Names whose adjoint can be nonzero: those that reach an output
+    through differentiable operands (not through sgn, ind or the condition
+    of dlim).  Reverse statements into any other name would add terms that
+    are zero in value but that IEEE arithmetic cannot fold away."""
+    live = set(outputs)
+    for dst, op, args in reversed(ops):
+        if dst in live:
+            if op in ("sgn", "ind"):
+                continue
+            live.update(a for a in (args[:2] if op == "dlim" else args) if not is_lit(a))
+    return live
 
 
 def segmented_adjoint_lines(params, inputs, outputs, ops, segs):
@@ -303,7 +333,8 @@ def segmented_adjoint_lines(params, inputs, outputs, ops, segs):
     with the first pass) right before its reverse, which bounds the live set
     to one segment's intermediates."""
     cst = set(params)
-    active = lambda a: not (is_lit(a) or a in cst)
+    live = grad_live(ops, outputs)
+    active = lambda a: not (is_lit(a) or a in cst) and a in live
     out = []
     names = list(inputs) + [dst for dst, _, _ in ops]
     for n in names:
@@ -338,7 +369,8 @@ def segmented_adjoint_lines(params, inputs, outputs, ops, segs):
                 out.append(f"    const double {dst}_s{k} = {C_OP[op].format(*[ren.get(x, x) for x in args])};")
         for n in reversed(idx):
             dst, op, args = ops[n]
-            out += reverse_op(dst, op, args, active, ren)
+            if dst in live:
+                out += reverse_op(dst, op, args, active, ren)
         out.append("    }")
     for i, v in enumerate(inputs):
         out.append(f"    in_b[{i}] = {v}_b;")
@@ -352,7 +384,8 @@ def re_sub_name(text, name, repl):
 
 def adjoint_lines(params, inputs, outputs, ops, guards=None):
     cst = set(params)
-    active = lambda a: not (is_lit(a) or a in cst)
+    live = grad_live(ops, outputs)
+    active = lambda a: not (is_lit(a) or a in cst) and a in live
     out = []
     names = list(inputs) + [dst for dst, _, _ in ops]
     for n in names:
@@ -365,7 +398,8 @@ def adjoint_lines(params, inputs, outputs, ops, guards=None):
         body = []
         for n in reversed(idx):
             dst, op, args = ops[n]
-            body += reverse_op(dst, op, args, active, {})
+            if dst in live:
+                body += reverse_op(dst, op, args, active, {})
         if g is None:
             out += body
         else:

@@ -62,6 +62,7 @@ struct AdjDesc {
     double *u0, *hist, *ckpt;     // initial state, segment states U^n, checkpoints
     double *hists;                // segment states U* (stage-1 results with their boundary)
     double *dts;                  // dt of every step of the rollout
+    double dtol;                  // degenerate star-state threshold of the face derivatives (0: exact)
     // tangent-linear model: direction (dU0, deta), tangent state dU, dU*,
     // tangent face fluxes dF, dG and corner EMF dE
     double *du0, *deta, *du, *dus, *dF, *dG, *dE;
@@ -251,8 +252,10 @@ __device__ __forceinline__ bool non_b(int v) { return v != 4 && v != 5; }
 // One wavefront per block: at ~500 registers a SIMD holds one such wave, and
 // single-wave blocks let it take the next tile as soon as its own ends.
 // slot >= 0: the primal state of this stage is history slot `slot` (U^n for
-// stage 1, U* for stage 2); slot < 0: the live buffers u / us.
-template <int STAGE, int DIR>
+// stage 1, U* for stage 2); slot < 0: the live buffers u / us.  DEGEN:
+// the degenerate star-state rule is on (d.dtol > 0); without it the rule's
+// test folds away at compile time.
+template <int STAGE, int DIR, bool DEGEN>
 __global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const long long *first, int nruns,
                                                        int step, int slot)
 {
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const
     const Soa S{prim, g.pitch, g.plane};
     const EtaAcc E{d.eta, g.pitch, d.ph.eta};
     AtomicAdd add{d.lamC, d.geta, g.pitch, g.plane};
-    const double par[3] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps};
+    const double par[4] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps, DEGEN ? d.dtol : 0.0};
     const double rpar[2] = {d.ph.dx, d.ph.dy};
     mhd_ad::flux_full_adj(S, E, DIR, i, j, par, rpar, lamF, add);
 }
@@ -448,7 +451,7 @@ __device__ __forceinline__ void sweep_issue(const SweepRun &g, int wave, int cs,
     }
 }
 
-template <int STAGE>
+template <int STAGE, bool DEGEN>
 __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs, const long long *sfirst, int nruns,
                                                              int cols, int step, int slot)
 {
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs
     const int j = j0 + lane;
     const double dt = d.dts[step];
     const double coef = (STAGE == 2 ? 0.5 : 1.0) * dt / (wave == 0 ? d.ph.dx : d.ph.dy);
-    const double par[3] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps};
+    const double par[4] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps, DEGEN ? d.dtol : 0.0};
     const double rpar[2] = {d.ph.dx, d.ph.dy};
     const double eta_uniform = d.ph.eta;
 
@@ -691,7 +694,7 @@ __global__ __launch_bounds__(256) void face_tlm_kernel(const AdjDesc *runs, cons
     const Soa dS{stage == 2 ? d.dus : d.du, g.pitch, g.plane};
     const EtaAcc E{d.eta, g.pitch, d.ph.eta};
     const EtaAcc dE{d.deta, g.pitch, 0.0};
-    const double par[3] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps};
+    const double par[4] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps, d.dtol};
     const double rpar[2] = {d.ph.dx, d.ph.dy};
     double F[8], dF[8];
     mhd_ad::flux_full_tlm(S, dS, E, dE, DIR, i, j, par, rpar, F, dF);
@@ -833,6 +836,7 @@ struct mhd_adjoint {
     int sweep_cols = 0;
     int face_kernel = MHD_ADJ_FACE_AUTO;
     bool face_kernel_auto_sweep = false;
+    double dtol = 0.0;            // degenerate star-state threshold (AdjDesc::dtol of every run)
     double *pool = nullptr;       // lam, lamBar, lamC, lamE, geta, u0 per run
     double *hist = nullptr, *hists = nullptr, *ckpt = nullptr, *dts = nullptr;
     // per history slot k, three copies of the batch's run descriptors: A_k
@@ -922,27 +926,32 @@ bool use_sweep(const mhd_adjoint *a)
     return a->face_kernel == MHD_ADJ_FACE_SWEEP || (a->face_kernel == MHD_ADJ_FACE_AUTO && a->face_kernel_auto_sweep);
 }
 
-// The four face-adjoint launches of one step (stage s, both directions).
+template <int STAGE, bool DEGEN>
+void launch_face_adj_t(const mhd_adjoint *a, int step, int slot, hipStream_t s)
+{
+    if (use_sweep(a)) {
+        hipLaunchKernelGGL((face_adj_sweep_kernel<STAGE, DEGEN>), dim3((unsigned)a->sweep_items), dim3(128), 0, s,
+                           a->d_desc, a->d_sfirst, a->nruns, a->sweep_cols, step, slot);
+    } else {
+        const dim3 tg1 = tiles_grid1(a), blk(64);
+        hipLaunchKernelGGL((face_adj_kernel<STAGE, 0, DEGEN>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step,
+                           slot);
+        hipLaunchKernelGGL((face_adj_kernel<STAGE, 1, DEGEN>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step,
+                           slot);
+    }
+}
+
+// The face-adjoint launches of one step (stage s, both directions).
 int launch_face_adj(mhd_adjoint *a, int stage, int step, int slot)
 {
     hipStream_t s = stream_of(a);
-    if (use_sweep(a)) {
-        const dim3 grid((unsigned)a->sweep_items), blk(128);
-        if (stage == 2)
-            hipLaunchKernelGGL(face_adj_sweep_kernel<2>, grid, blk, 0, s, a->d_desc, a->d_sfirst, a->nruns,
-                               a->sweep_cols, step, slot);
-        else
-            hipLaunchKernelGGL(face_adj_sweep_kernel<1>, grid, blk, 0, s, a->d_desc, a->d_sfirst, a->nruns,
-                               a->sweep_cols, step, slot);
+    const bool degen = a->dtol > 0.0;
+    if (stage == 2) {
+        if (degen) launch_face_adj_t<2, true>(a, step, slot, s);
+        else launch_face_adj_t<2, false>(a, step, slot, s);
     } else {
-        const dim3 tg1 = tiles_grid1(a), blk(64);
-        if (stage == 2) {
-            hipLaunchKernelGGL((face_adj_kernel<2, 0>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step, slot);
-            hipLaunchKernelGGL((face_adj_kernel<2, 1>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step, slot);
-        } else {
-            hipLaunchKernelGGL((face_adj_kernel<1, 0>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step, slot);
-            hipLaunchKernelGGL((face_adj_kernel<1, 1>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step, slot);
-        }
+        if (degen) launch_face_adj_t<1, true>(a, step, slot, s);
+        else launch_face_adj_t<1, false>(a, step, slot, s);
     }
     AHIP(a, hipGetLastError());
     return MHD_OK;
@@ -1228,6 +1237,15 @@ int mhd_adjoint_set_face_kernel(mhd_adjoint *a, int mode)
         return afail(a, MHD_ERR_INVALID, "mode must be MHD_ADJ_FACE_AUTO, _PER_FACE or _SWEEP");
     a->face_kernel = mode;
     return MHD_OK;
+}
+
+int mhd_adjoint_set_degenerate_tol(mhd_adjoint *a, double tol)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (!(tol >= 0.0 && tol < 1.0)) return afail(a, MHD_ERR_INVALID, "tol must be in [0, 1)");
+    a->dtol = tol;
+    for (AdjDesc &d : a->desc) d.dtol = tol;
+    return upload_desc(a);
 }
 
 int mhd_adjoint_set_eta_field(mhd_adjoint *a, int run, const double *host_eta)

@@ -43,7 +43,7 @@ EXPORTS = (
     "mhd_adjoint_get_grad_states", "mhd_save_subdomain", "mhd_load_subdomain", "mhd_adjoint_get_grad_etas",
     "mhd_adjoint_set_states_device", "mhd_adjoint_get_states_device", "mhd_adjoint_set_seeds_device",
     "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device", "mhd_adjoint_set_face_kernel",
-    "mhd_adjoint_autotune",
+    "mhd_adjoint_autotune", "mhd_adjoint_set_degenerate_tol",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -181,6 +181,7 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_get_grad_etas_device": (I, [P, P]),
         "mhd_adjoint_set_face_kernel": (I, [P, I]),
         "mhd_adjoint_autotune": (I, [P, I, ctypes.POINTER(I)]),
+        "mhd_adjoint_set_degenerate_tol": (I, [P, ctypes.c_double]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -725,6 +726,15 @@ class ResistiveMHD2DAdjoint:
         """Kernel form of the face adjoints: FACE_AUTO, FACE_PER_FACE or
         FACE_SWEEP (results agree to the rounding of atomic sums)."""
         self._ok(self._lib.mhd_adjoint_set_face_kernel(self._h, int(mode)))
+
+    def set_degenerate_tol(self, tol):
+        """Derivative rule at degenerate HLLD star states.  tol = 0 (the
+        default) differentiates the step exactly.  tol > 0: on faces where
+        |rho (S - u)(S - SM) - Bx^2| < tol rho (S - u)^2 the tangent and the
+        adjoint take the derivative of Miyoshi & Kusano's degenerate limit
+        (tangential star components equal to the outer ones); the forward
+        step is unchanged."""
+        self._ok(self._lib.mhd_adjoint_set_degenerate_tol(self._h, float(tol)))
 
     def grad_eta_all(self):
         n = [self.cfgs[r].nx * self.cfgs[r].ny for r in range(len(self.cfgs))]

@@ -9,7 +9,10 @@ configurations of the same grid.  The forward runs the HIP library's batched
 step with every dt recorded and sqrt(nsteps) checkpoints; backward runs the
 hand-written discrete adjoint (mhd_adjoint_backward) with those dt frozen and
 returns dL/dU0 and dL/deta.  Compose U0 from primitive fields with torch ops
-to get gradients with respect to the initial velocity.
+to get gradients with respect to the initial velocity.  degenerate_tol > 0
+differentiates degenerate HLLD star states by their Miyoshi-Kusano limit
+(ResistiveMHD2DAdjoint.set_degenerate_tol); 0 differentiates the step
+exactly.
 """
 import numpy as np
 import torch
@@ -19,13 +22,14 @@ from .mhd2d import ResistiveMHD2DAdjoint
 
 class MHDRollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, U0, eta, cfgs, nsteps, stride):
+    def forward(ctx, U0, eta, cfgs, nsteps, stride, degenerate_tol):
         if U0.dtype != torch.float64 or eta.dtype != torch.float64:
             raise TypeError("the step is FP64: pass float64 tensors")
         B = U0.shape[0]
         if len(cfgs) != B or eta.shape[0] != B:
             raise ValueError("one configuration and one eta field per run")
         adj = ResistiveMHD2DAdjoint(cfgs)
+        adj.set_degenerate_tol(degenerate_tol)
         etan = eta.detach().to("cpu").numpy()
         for r in range(B):
             adj.setEtaField(r, etan[r])
@@ -60,9 +64,9 @@ class MHDRollout(torch.autograd.Function):
             adj.backward()
             gU0 = torch.from_numpy(np.stack(adj.grad_state_all()))
             geta = torch.from_numpy(np.stack(adj.grad_eta_all()))
-        return gU0, geta, None, None, None
+        return gU0, geta, None, None, None, None
 
 
-def rollout(U0, eta, cfgs, nsteps, stride=0):
+def rollout(U0, eta, cfgs, nsteps, stride=0, degenerate_tol=0.0):
     """Final states after nsteps RK2 steps, differentiable in U0 and eta."""
-    return MHDRollout.apply(U0, eta, cfgs, nsteps, stride)
+    return MHDRollout.apply(U0, eta, cfgs, nsteps, stride, float(degenerate_tol))

@@ -51,8 +51,8 @@ def p(a):
     return a.ctypes.data_as(D) if a is not None else None
 
 
-def par_of(cfg):
-    return np.array([cfg.gamma - 1.0, cfg.gamma, cfg.eps]), np.array([cfg.dx, cfg.dy])
+def par_of(cfg, dtol=0.0):
+    return np.array([cfg.gamma - 1.0, cfg.gamma, cfg.eps, dtol]), np.array([cfg.dx, cfg.dy])
 
 
 def random_face_inputs(rng):
@@ -112,7 +112,7 @@ def test_primal_equals_oracle_fluxes(ad, case):
 
 def test_face_flux_tangent_matches_finite_differences(ad):
     rng = np.random.default_rng(11)
-    par = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20])
+    par = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20, 0.0])
     errs = []
     for _ in range(200):
         x = random_face_inputs(rng)
@@ -133,7 +133,7 @@ def test_face_flux_tangent_matches_finite_differences(ad):
 
 def test_face_flux_adjoint_dot_product(ad):
     rng = np.random.default_rng(12)
-    par = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20])
+    par = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20, 0.0])
     for _ in range(200):
         x = random_face_inputs(rng)
         v = rng.normal(size=32)
@@ -205,3 +205,176 @@ def test_assembled_face_adjoint_dot_product(ad, d):
             lhs = dF @ lamF
             rhs = (shift_transpose(cfg, lam_c) * dU).sum() + (geta * deta).sum()
             assert abs(lhs - rhs) <= 1e-11 * max(1.0, abs(lhs)), (i, j, lhs, rhs)
+
+
+def test_adjoint_branches_read_ieee_values():
+    """The adjoint's own primal pass uses refined reciprocal and square-root
+    estimates only where no branch looks: every name an HLLD region
+    indicator, a guard or a min/max/sign selection reads is computed,
+    transitively, with IEEE division and sqrt.  (S1L = SM - sqrt(bXL^2 /
+    rho1L) decides between the outer and inner star regions; an estimate
+    there could let the adjoint differentiate a region the forward step did
+    not take.)"""
+    import re
+    text = open(os.path.join(CSRC, "mhd2d_face_ad.h")).read()
+    body = text[text.index("inline void face_flux_adj("):]
+    body = body[:body.index("\n}\n")]
+    primal = body[:body.index("_b = 0.0;")]
+    defs = {}
+    for m in re.finditer(r"^\s*(?:const double )?(\w+) = (.*);$", primal, re.M):
+        if m.group(2) != "0.0":
+            defs[m.group(1)] = m.group(2)
+    ident = re.compile(r"[A-Za-z_]\w*")
+    roots = set()
+    for name, expr in defs.items():
+        if "<" in expr or ">" in expr:          # ind, tmin, tmax, sgn
+            roots.update(ident.findall(expr))
+    for g in re.findall(r"^\s*if \((.*)\) \{$", primal, re.M):
+        roots.update(ident.findall(g))
+    assert {"S1L", "S1R", "SM", "SL", "SR"} <= roots
+    seen, todo, bad = set(), list(roots), []
+    while todo:
+        n = todo.pop()
+        if n in seen or n not in defs:
+            continue
+        seen.add(n)
+        if "mhd_ad_" in defs[n]:
+            bad.append((n, defs[n]))
+        todo += ident.findall(defs[n])
+    assert not bad, bad
+
+
+def listing_flux(x, par, limit=False):
+    """face_flux.ssa run as Python statements (IEEE double, the listing's own
+    evaluation order), independently of the generated C.  limit=True
+    evaluates dlim(x, limit, c) as its limit wherever c holds."""
+    import math
+    ns = {"gm1": float(par[0]), "gamma": float(par[1]), "eps": float(par[2]), "dtol": float(par[3]),
+          "sqrt": math.sqrt, "fabs": abs}
+    ns["tmin"] = lambda a, b: b if b < a else a
+    ns["tmax"] = lambda a, b: b if a < b else a
+    ns["sgn"] = lambda a: float((a > 0.0) - (a < 0.0))
+    ns["ind"] = lambda c: float(c)
+    ns["minmod"] = lambda a, b: ns["sgn"](a) * ns["tmax"](ns["tmin"](abs(a), ns["sgn"](a) * b), ns["eps"])
+    ns["dlim"] = (lambda v, lim, c: lim if c != 0.0 else v) if limit else (lambda v, lim, c: v)
+    outputs = []
+    for line in open(os.path.join(CSRC, "adjoint", "face_flux.ssa")):
+        line = line.split("#", 1)[0].strip()
+        if line.startswith("inputs:"):
+            for k, name in enumerate(line.split(":", 1)[1].split()):
+                ns[name] = float(x[k])
+        elif line.startswith("outputs:"):
+            outputs = line.split(":", 1)[1].split()
+        elif line and not line.split("=")[0].strip().endswith(("params:", "guard:", "segment:")) \
+                and ":" not in line.split("=")[0]:
+            exec(line, ns)
+    return np.array([ns[o] for o in outputs])
+
+
+def test_listing_run_as_python_equals_generated_primal(ad):
+    rng = np.random.default_rng(31)
+    par = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20, 0.0])
+    out = np.empty(7)
+    for _ in range(50):
+        x = random_face_inputs(rng)
+        ad.h_face_flux(p(x), p(par), p(out))
+        assert np.array_equal(out, listing_flux(x, par))
+
+
+def degenerate_face():
+    """Four cells whose face has degenerate left and right star states, as
+    the Harris sheet's far field does: the normal field dominates (Bx^2 /
+    rho = 5 > c_s^2), the transverse field reconstructs to exactly 0 on both
+    sides, and rho, u and p reconstruct to equal values on both sides, so
+    rho (S - u)(S - SM) - Bx^2 is zero up to rounding.  Every limiter is
+    off its kink (B_t = -4t, -t, t, 4t: the middle difference is the
+    smallest, and t = 2^-10 makes the reconstruction exactly 0)."""
+    g = 5.0 / 3.0
+    t = 2.0 ** -10
+    rho = [0.2, 0.202, 0.203, 0.205]
+    u = [0.0, 2e-4, 3e-4, 5e-4]
+    v = [1e-3, -2e-3, 5e-4, 1.5e-3]
+    w = [2e-4, -1e-4, 3e-4, 0.0]
+    by = [-4 * t, -t, t, 4 * t]
+    bz = [-8 * t, -2 * t, 2 * t, 8 * t]
+    pr = [0.1, 0.101, 0.1015, 0.1025]
+    cells = []
+    for k in range(4):
+        e = pr[k] / (g - 1) + 0.5 * rho[k] * (u[k] ** 2 + v[k] ** 2 + w[k] ** 2) + 0.5 * (1.0 + by[k] ** 2 + bz[k] ** 2)
+        cells += [rho[k], rho[k] * u[k], rho[k] * v[k], rho[k] * w[k], -1.0, by[k], bz[k], e]
+    return np.array(cells)
+
+
+def test_degenerate_rule_matches_differences_of_the_limit(ad):
+    """At a degenerate face the reference's star-state quotients are rounding
+    noise: their central differences change by O(1) between steps h = 1e-5
+    and 1e-7, so no derivative rule can match them.  With the rule on, the
+    tangent is the derivative of the flux whose star tangential components
+    are the outer ones (Miyoshi & Kusano's degenerate limit): central
+    differences of the listing run as Python with that substitution are
+    steady in h and agree with it."""
+    x = degenerate_face()
+    rng = np.random.default_rng(41)
+    on = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20, 1e-8])
+    off = on.copy()
+    off[3] = 0.0
+    out, d_on = np.empty(7), np.empty(7)
+    ad.h_face_flux(p(x), p(on), p(out))
+    assert np.array_equal(out, listing_flux(x, off))      # the value is the reference's
+    assert np.array_equal(out, listing_flux(x, on, limit=True))   # and equals the limit's here
+
+    def fd(par, limit, dx, h):
+        return (listing_flux(x + h * dx, par, limit) - listing_flux(x - h * dx, par, limit)) / (2 * h)
+    worst_on, ref_spread = 0.0, 0.0
+    for _ in range(6):
+        dx = rng.normal(size=32) * np.abs(x) + rng.normal(size=32) * 1e-7
+        ad.h_face_flux_tlm(p(x), p(dx), p(on), p(out), p(d_on))
+        lim = fd(on, True, dx, 1e-7)
+        scale = np.abs(lim).max()
+        worst_on = max(worst_on, np.abs(d_on - lim).max() / scale,
+                       np.abs(fd(on, True, dx, 1e-5) - lim).max() / scale)
+        ref_spread = max(ref_spread, np.abs(fd(off, False, dx, 1e-5) - fd(off, False, dx, 1e-7)).max() / scale)
+    assert worst_on < 1e-7, worst_on
+    assert ref_spread > 0.1, ref_spread
+
+
+def test_degenerate_rule_is_inactive_on_generic_faces(ad):
+    """Random faces are far from the degenerate set: the tangent and adjoint
+    with the rule on equal those with it off bit for bit."""
+    rng = np.random.default_rng(43)
+    on = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20, 1e-8])
+    off = on.copy()
+    off[3] = 0.0
+    o1, o2, d1, d2, b1, b2 = (np.empty(n) for n in (7, 7, 7, 7, 32, 32))
+    for _ in range(100):
+        x = random_face_inputs(rng)
+        v, w = rng.normal(size=32), rng.normal(size=7)
+        ad.h_face_flux_tlm(p(x), p(v), p(on), p(o1), p(d1))
+        ad.h_face_flux_tlm(p(x), p(v), p(off), p(o2), p(d2))
+        ad.h_face_flux_adj(p(x), p(on), p(w), p(b1))
+        ad.h_face_flux_adj(p(x), p(off), p(w), p(b2))
+        assert np.array_equal(d1, d2) and np.array_equal(b1, b2)
+
+
+def test_degenerate_rule_adjoint_is_transpose_of_tangent(ad):
+    """Dot-product identity with the rule on, at the degenerate face and at
+    faces of the Harris sheet start (where the far field is degenerate)."""
+    rng = np.random.default_rng(47)
+    on = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20, 1e-8])
+    xs = [degenerate_face()]
+    cfg = mhd.harris_config(64, 32)
+    U = boundary_applied(cfg, mhd.fill_harris(cfg, seed=3, noise_amp=1e-3))
+    inp = np.empty(32)
+    for i, j in ((10, 3), (30, 28), (50, 5)):
+        for c in range(4):
+            inp[8 * c:8 * c + 8] = U[i - 1 + c, j]
+        inp[4::8] = 0.5 * (U[i - 2:i + 2, j, 4] + U[i - 1:i + 3, j, 4])
+        xs.append(inp.copy())
+    out, out_d, xb = np.empty(7), np.empty(7), np.empty(32)
+    for x in xs:
+        for _ in range(5):
+            v, w = rng.normal(size=32), rng.normal(size=7)
+            ad.h_face_flux_tlm(p(x), p(v), p(on), p(out), p(out_d))
+            ad.h_face_flux_adj(p(x), p(on), p(w), p(xb))
+            lhs, rhs = out_d @ w, v @ xb
+            assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs), np.abs(out_d).max() * np.abs(w).max()), (lhs, rhs)
