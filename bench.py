@@ -374,8 +374,9 @@ def main():
     ap.add_argument("--check-runs", type=int, default=32)
     ap.add_argument("--check-steps", type=int, default=2000)
     ap.add_argument("--check-dirs", type=int, default=16)
-    ap.add_argument("--degenerate-tol", type=float, default=0.0,
-                    help="gradcheck: derivative rule at degenerate HLLD star states (0: exact)")
+    ap.add_argument("--degenerate-tol", type=float, default=1e-8,
+                    help="gradcheck: derivative rule at degenerate HLLD star states (0: the exact derivative, "
+                         "which is rounding noise at the Harris far field's faces)")
     ap.add_argument("--fd-h", type=float, default=1e-4, help="gradcheck: central-difference step")
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=40, help="timed batched steps")
