@@ -367,6 +367,14 @@ struct SweepAdd {
                                __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 };
+}  // namespace
+
+template <>
+struct mhd_ad::add_skips_zero<SweepAdd> {
+    static constexpr bool value = false;
+};
+
+namespace {
 
 // Item -> run, first row, face columns [ia, ib).
 __device__ __forceinline__ bool sweep_item(const AdjDesc *runs, const long long *sfirst, int nruns, int cols,

@@ -109,6 +109,14 @@ MHD_HD inline void resist_slot(int dir, int i, int j, int s, int &ci, int &cj, i
     }
 }
 
+// Whether an adder skips zero contributions: a global atomic costs a memory
+// transaction, so a zero one is worth a test; an LDS add is cheaper than
+// the test and the exec-mask change around it.
+template <class Add>
+struct add_skips_zero {
+    static constexpr bool value = true;
+};
+
 // Scatter of one face's input adjoints: in_b (4 cells x 8 slots, rotated
 // for y faces) and the resistive stencil's remaining slots 8..13.
 template <class Add>
@@ -121,12 +129,12 @@ MHD_HD inline void scatter_face(int dir, int i, int j, const double *in_b, const
         MHD_UNROLL
         for (int v = 0; v < 8; ++v) {
             const double b = in_b[8 * c + v];
-            if (b != 0.0) add(ci, cj, dir == 0 ? v : rot_y(v), b);
+            if (!add_skips_zero<Add>::value || b != 0.0) add(ci, cj, dir == 0 ? v : rot_y(v), b);
         }
     }
     MHD_UNROLL
     for (int s = 8; s < 14; ++s) {
-        if (rin_b[s] == 0.0) continue;
+        if (add_skips_zero<Add>::value && rin_b[s] == 0.0) continue;
         int ci, cj, v;
         resist_slot(dir, i, j, s, ci, cj, v);
         add(ci, cj, v, rin_b[s]);
