@@ -135,6 +135,7 @@ def batch_bench(nruns, steps, plain_steps, run_steps, numerics):
     b.applyBoundary()
     t0 = time.perf_counter()
     sweep = b.autotune(steps=2)
+    waves = b.waves_per_cu()
     t_tune = time.perf_counter() - t0
     b.steps(4)
     b.synchronize()
@@ -176,7 +177,8 @@ def batch_bench(nruns, steps, plain_steps, run_steps, numerics):
         "batched_ms_per_step": t_batch / steps * 1e3, "plain_ms_per_step": t_plain / plain_steps * 1e3,
         "batched_zone_cycles_per_s": cells * steps / t_batch,
         "timed_steps": {"batched": steps, "plain": plain_steps},
-        "autotuned_sweep_columns": sweep, "autotune_s": t_tune, "crashed": crashed,
+        "autotuned_sweep_columns": sweep, "autotuned_waves_per_cu": waves, "autotune_s": t_tune,
+        "crashed": crashed,
         "workload": batch_workload(nruns),
     }
 

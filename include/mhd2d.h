@@ -249,10 +249,18 @@ int mhd_batch_set_sweep_length(mhd_batch *b, int columns);
 int mhd_batch_get_sweep_length(const mhd_batch *b, int *columns);
 /* replay steps from a captured hipGraph (default on) */
 int mhd_batch_set_graph(mhd_batch *b, int on);
+/* Launch shape of the batched stage kernels: 0 (the default) launches one
+ * wavefront per work item; w > 0 launches w wavefronts per CU that loop
+ * over the work list (a persistent launch). */
+int mhd_batch_set_waves_per_cu(mhd_batch *b, int waves_per_cu);
+int mhd_batch_get_waves_per_cu(const mhd_batch *b, int *waves_per_cu);
 /* Autotuner: times 2*steps batch steps for each candidate sweep length on
- * the batch's current states, keeps the fastest (written to *columns if not
- * NULL) and restores every state, time and dt, so the runs continue as if
- * it had not been called. */
+ * the batch's current states, then for each launch shape (one wavefront
+ * per item, or half or all of the resident wavefront slots per CU) at the
+ * best length.  It keeps the fastest pair (the length written to *columns
+ * if not NULL, the shape readable by mhd_batch_get_waves_per_cu) and
+ * restores every state, time and dt, so the runs continue as if it had not
+ * been called. */
 int mhd_batch_autotune(mhd_batch *b, int steps, int *columns);
 
 /* ---------------------------------------------------------------- adjoint

@@ -60,6 +60,14 @@ void drop_graph(mhd_batch *b)
     }
 }
 
+// Wavefronts per XCD of the stage launches for b->waves_per_cu.
+void set_launch_shape(mhd_batch *b)
+{
+    const int cu_per_xcd = std::max(1, (b->num_cu > 0 ? b->num_cu : 256) / b->xcds);
+    b->blocks_per_xcd = b->waves_per_cu <= 0 ? b->items_per_xcd
+                                              : std::max(1, std::min(b->items_per_xcd, cu_per_xcd * b->waves_per_cu));
+}
+
 // Work list for sweep length L: items of run r are [first_item, first_item +
 // nstrips * nsegs), strips fastest as in the single-run kernel.
 int build_work_list(mhd_batch *b, int L)
@@ -77,6 +85,7 @@ int build_work_list(mhd_batch *b, int L)
     b->total_items = (int)item_run.size();
     b->xcds = b->num_cu >= 64 ? b->num_cu / 32 : 1;
     b->items_per_xcd = (b->total_items + b->xcds - 1) / b->xcds;
+    set_launch_shape(b);
     if (b->total_items > b->item_cap) {
         BHIP(b, hipStreamSynchronize(b->stream));
         BHIP(b, hipFree(b->d_item_run));
@@ -134,10 +143,10 @@ int refresh_dt(mhd_batch *b) { return mhd_batch_refresh_dt(b); }
 int enqueue_step(mhd_batch *b, int cur)
 {
     BHIP(b, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
-                                    b->items_per_xcd, cur, b->stream));
+                                    b->items_per_xcd, b->blocks_per_xcd, cur, b->stream));
     BHIP(b, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, b->stream));
     BHIP(b, mhd::launch_stage_batch(2, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
-                                    b->items_per_xcd, cur, b->stream));
+                                    b->items_per_xcd, b->blocks_per_xcd, cur, b->stream));
     BHIP(b, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, false, true, cur ^ 1,
                                  b->stream));
     return MHD_OK;
@@ -507,42 +516,80 @@ int mhd_batch_autotune(mhd_batch *b, int steps, int *columns)
         return MHD_OK;
     };
     const int saved_cur = b->cur;
+    // times steps * kGraphSteps steps with sweep length L and w resident
+    // wavefronts per CU (0: one per item), then restores the snapshot
+    auto trial = [&](int L, int w, double &t) -> int {
+        b->waves_per_cu = w;
+        int r;
+        if ((r = build_work_list(b, L))) return r;
+        if ((r = mhd_batch_step(b, kGraphSteps))) return r;   // warm-up and graph capture
+        hipError_t err = hipStreamSynchronize(b->stream);
+        if (err != hipSuccess) return bfail(b, MHD_ERR_HIP, std::string("autotune: ") + hipGetErrorString(err));
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((r = mhd_batch_step(b, steps * kGraphSteps))) return r;
+        err = hipStreamSynchronize(b->stream);
+        if (err != hipSuccess) return bfail(b, MHD_ERR_HIP, std::string("autotune: ") + hipGetErrorString(err));
+        t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (b->cur != saved_cur) b->cur ^= 1;   // keep the slot parity of the saved control blocks
+        return snapshot(true);
+    };
+    // sweep lengths at the current launch shape, then launch shapes at the
+    // best sweep length: one wavefront per item, or a persistent launch
+    // with half or all of the resident wavefront slots per CU
     static const int cand[] = {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256};
-    int best = b->seg;
-    double best_t = 1e300;
+    const int w0 = b->waves_per_cu;
+    int best = b->seg, best_w = w0;
+    double best_t = 1e300, t = 0.0;
     rc = snapshot(false);
     for (int L : cand) {
         if (rc) break;
         if (L > b->max_nx && L != cand[0]) break;
-        if ((rc = build_work_list(b, L))) break;
-        if ((rc = mhd_batch_step(b, kGraphSteps))) break;   // warm-up and graph capture
-        if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) {
-            rc = bfail(b, MHD_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-            break;
-        }
-        const auto t0 = std::chrono::steady_clock::now();
-        if ((rc = mhd_batch_step(b, steps * kGraphSteps))) break;
-        if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) {
-            rc = bfail(b, MHD_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-            break;
-        }
-        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if ((rc = trial(L, w0, t))) break;
         if (t < best_t) {
             best_t = t;
             best = L;
         }
-        if (b->cur != saved_cur) b->cur ^= 1;   // keep the slot parity of the saved control blocks
-        if ((rc = snapshot(true))) break;
+    }
+    const int occ = std::max(1, mhd::stage_batch_waves_per_cu(b->exact));
+    const int wcand[] = {0, std::max(1, occ / 2), occ};
+    for (int k = 0; k < 3; ++k) {
+        const int w = wcand[k];
+        if (rc) break;
+        if (w == w0 || (k == 2 && w == wcand[1])) continue;   // timed already
+        if ((rc = trial(best, w, t))) break;
+        if (t < best_t) {
+            best_t = t;
+            best_w = w;
+        }
     }
     if (!rc) rc = snapshot(true);
     (void)hipStreamSynchronize(b->stream);
     (void)hipFree(save);
     (void)hipFree(save_ctrl);
     b->cur = saved_cur;
+    b->waves_per_cu = best_w;
     if (rc) return rc;
     b->seg_request = best;
     if ((rc = build_work_list(b, best))) return rc;
     if (columns) *columns = best;
+    return MHD_OK;
+}
+
+int mhd_batch_set_waves_per_cu(mhd_batch *b, int waves)
+{
+    if (!b) return bfail(nullptr, MHD_ERR_INVALID, "NULL batch");
+    if (waves < 0 || waves > 64) return bfail(b, MHD_ERR_INVALID, "waves_per_cu must be in 0 .. 64");
+    BHIP(b, hipStreamSynchronize(b->stream));
+    b->waves_per_cu = waves;
+    set_launch_shape(b);
+    drop_graph(b);
+    return MHD_OK;
+}
+
+int mhd_batch_get_waves_per_cu(const mhd_batch *b, int *waves)
+{
+    if (!b || !waves) return bfail(nullptr, MHD_ERR_INVALID, "NULL argument");
+    *waves = b->waves_per_cu;
     return MHD_OK;
 }
 

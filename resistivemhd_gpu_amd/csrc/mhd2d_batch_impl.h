@@ -24,6 +24,9 @@ struct mhd_batch {
     bool exact = true;
     int seg_request = 0, seg = 0;
     int total_items = 0, xcds = 1, items_per_xcd = 0;
+    // stage launch shape: waves_per_cu resident wavefronts per CU looping
+    // over the work list, or 0 for one wavefront per item
+    int waves_per_cu = 0, blocks_per_xcd = 0;
     int max_nx = 0, max_ny = 0;
     int cur = 0;
     bool dt_valid = false;

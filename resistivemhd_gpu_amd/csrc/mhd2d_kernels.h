@@ -56,10 +56,12 @@ hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *
 // Resident 64-lane stage workgroups per CU (the lower of the two stages).
 int stage_waves_per_cu(bool exact);
 // Batched stage over a work list of total_items items (item_run: run of
-// each item) and the batched boundary pair of every run's U* (stage_state)
-// or U, optionally reducing the ghost cells' dt into slot.
+// each item), items_per_xcd per XCD taken by blocks_per_xcd wavefronts per
+// XCD (equal: one wavefront per item; fewer: persistent), and the batched
+// boundary pair of every run's U* (stage_state) or U, optionally reducing
+// the ghost cells' dt into slot.
 hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const int *item_run, int total_items,
-                              int xcds, int items_per_xcd, int cur, hipStream_t s);
+                              int xcds, int items_per_xcd, int blocks_per_xcd, int cur, hipStream_t s);
 hipError_t launch_bc_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int max_ny, bool stage_state,
                            bool with_dt, int slot, hipStream_t s);
 int stage_batch_waves_per_cu(bool exact);

@@ -455,15 +455,16 @@ void rk_stage_kernel(const double *__restrict__ uin, double *uout, const double 
 // Batched form: one launch advances every run of a batch by one stage.  The
 // work list holds the items of all runs back to back; item_run maps an item
 // to its run, whose descriptor gives the grid, constants and buffers.  Each
-// run keeps its own dt (its own control block).
+// run keeps its own dt (its own control block).  XCD x owns the items
+// [x * items_per_xcd, (x + 1) * items_per_xcd); its blocks_per_xcd
+// wavefronts take them in turn.  blocks_per_xcd == items_per_xcd is one
+// wavefront per item (PERSIST false: no loop, whose bookkeeping costs the
+// exact kernel spill space); fewer makes the launch persistent (each
+// wavefront loops over items).
 template <int STAGE, bool EXACT>
-__global__ __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
-void rk_stage_batch_kernel(const RunDesc *__restrict__ runs, const int *__restrict__ item_run, int total_items,
-                           int xcds, int items_per_xcd, int cur)
+__device__ __forceinline__ void batch_item(const RunDesc *__restrict__ runs, const int *__restrict__ item_run,
+                                           int item, int cur)
 {
-    const int item = xcd_item(xcds, items_per_xcd);
-    if (item >= total_items) return;
     const RunDesc &d = runs[item_run[item]];
     const Grid g = d.g;
     const Phys ph = d.ph;
@@ -472,6 +473,25 @@ void rk_stage_batch_kernel(const RunDesc *__restrict__ runs, const int *__restri
     else
         stage_item<2, EXACT, 0>(item - d.first_item, d.us, d.uo ? d.uo : d.u, d.u, d.eta, d.side, nullptr, d.ctrl, g,
                                 ph, cur);
+}
+
+template <int STAGE, bool EXACT, bool PERSIST>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
+void rk_stage_batch_kernel(const RunDesc *__restrict__ runs, const int *__restrict__ item_run, int total_items,
+                           int xcds, int items_per_xcd, int blocks_per_xcd, int cur)
+{
+    if (!PERSIST) {
+        const int item = xcd_item(xcds, items_per_xcd);
+        if (item < total_items) batch_item<STAGE, EXACT>(runs, item_run, item, cur);
+        return;
+    }
+    const int x = (int)(blockIdx.x % xcds);
+    for (int k = (int)(blockIdx.x / xcds); k < items_per_xcd; k += blocks_per_xcd) {
+        const int item = x * items_per_xcd + k;
+        if (item >= total_items) return;
+        batch_item<STAGE, EXACT>(runs, item_run, item, cur);
+    }
 }
 
 // ------------------------------------------------------------ boundaries
@@ -903,12 +923,18 @@ hipError_t launch_stage(int stage, bool exact, int scheme, const double *uin, do
 }
 
 hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const int *item_run, int total_items,
-                              int xcds, int items_per_xcd, int cur, hipStream_t s)
+                              int xcds, int items_per_xcd, int blocks_per_xcd, int cur, hipStream_t s)
 {
-    const int blocks = xcds * items_per_xcd;
-#define MHD_LAUNCH_BATCH(ST, EX)                                                                                \
-    hipLaunchKernelGGL((rk_stage_batch_kernel<ST, EX>), dim3(blocks), dim3(64), 0, s, runs, item_run, total_items, \
-                       xcds, items_per_xcd, cur)
+    const int blocks = xcds * blocks_per_xcd;
+#define MHD_LAUNCH_BATCH(ST, EX)                                                                                 \
+    do {                                                                                                         \
+        if (blocks_per_xcd < items_per_xcd)                                                                      \
+            hipLaunchKernelGGL((rk_stage_batch_kernel<ST, EX, true>), dim3(blocks), dim3(64), 0, s, runs, item_run, \
+                               total_items, xcds, items_per_xcd, blocks_per_xcd, cur);                           \
+        else                                                                                                     \
+            hipLaunchKernelGGL((rk_stage_batch_kernel<ST, EX, false>), dim3(blocks), dim3(64), 0, s, runs,        \
+                               item_run, total_items, xcds, items_per_xcd, blocks_per_xcd, cur);                 \
+    } while (0)
     if (stage == 1) {
         if (exact) MHD_LAUNCH_BATCH(1, true); else MHD_LAUNCH_BATCH(1, false);
     } else {
@@ -937,11 +963,11 @@ int stage_batch_waves_per_cu(bool exact)
 {
     int a = 0, b = 0;
     if (exact) {
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_batch_kernel<1, true>, 64, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_batch_kernel<2, true>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_batch_kernel<1, true, false>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_batch_kernel<2, true, false>, 64, 0);
     } else {
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_batch_kernel<1, false>, 64, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_batch_kernel<2, false>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, rk_stage_batch_kernel<1, false, false>, 64, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, rk_stage_batch_kernel<2, false, false>, 64, 0);
     }
     const int w = a < b ? a : b;
     return w > 0 ? w : 1;

@@ -34,7 +34,8 @@ EXPORTS = (
     "mhd_batch_get_state", "mhd_batch_set_eta_field", "mhd_batch_apply_boundary", "mhd_batch_init_orszag_tang",
     "mhd_batch_init_harris", "mhd_batch_step", "mhd_batch_synchronize", "mhd_batch_get_dt", "mhd_batch_get_time",
     "mhd_batch_check_crashed", "mhd_batch_set_numerics", "mhd_batch_set_sweep_length",
-    "mhd_batch_get_sweep_length", "mhd_batch_set_graph", "mhd_batch_autotune",
+    "mhd_batch_get_sweep_length", "mhd_batch_set_graph", "mhd_batch_autotune", "mhd_batch_set_waves_per_cu",
+    "mhd_batch_get_waves_per_cu",
     "mhd_adjoint_create", "mhd_adjoint_destroy", "mhd_adjoint_last_error", "mhd_adjoint_set_eta_field",
     "mhd_adjoint_set_state", "mhd_adjoint_set_dts", "mhd_adjoint_get_dts", "mhd_adjoint_forward",
     "mhd_adjoint_get_state", "mhd_adjoint_set_seed", "mhd_adjoint_backward", "mhd_adjoint_get_grad_state",
@@ -150,6 +151,8 @@ def load_library(path=LIB_PATH):
         "mhd_batch_get_sweep_length": (I, [P, ctypes.POINTER(I)]),
         "mhd_batch_set_graph": (I, [P, I]),
         "mhd_batch_autotune": (I, [P, I, ctypes.POINTER(I)]),
+        "mhd_batch_set_waves_per_cu": (I, [P, I]),
+        "mhd_batch_get_waves_per_cu": (I, [P, ctypes.POINTER(I)]),
         "mhd_adjoint_create": (I, [C, I, ctypes.POINTER(P)]),
         "mhd_adjoint_destroy": (I, [P]),
         "mhd_adjoint_last_error": (ctypes.c_char_p, [P]),
@@ -560,7 +563,19 @@ class ResistiveMHD2DBatch:
     def set_graph(self, on=True):
         self._ok(self._lib.mhd_batch_set_graph(self._h, 1 if on else 0))
 
+    def set_waves_per_cu(self, waves):
+        """Launch shape of the stage kernels: 0 = one wavefront per work
+        item; w > 0 = w wavefronts per CU looping over the work list."""
+        self._ok(self._lib.mhd_batch_set_waves_per_cu(self._h, int(waves)))
+
+    def waves_per_cu(self):
+        v = ctypes.c_int()
+        self._ok(self._lib.mhd_batch_get_waves_per_cu(self._h, ctypes.byref(v)))
+        return v.value
+
     def autotune(self, steps=2):
+        """Sweep length, then launch shape, timed on the batch itself;
+        returns the sweep length (waves_per_cu() gives the shape)."""
         v = ctypes.c_int()
         self._ok(self._lib.mhd_batch_autotune(self._h, int(steps), ctypes.byref(v)))
         return v.value

@@ -103,6 +103,34 @@ def test_batch_graph_eager_sweep_and_autotune_agree():
         assert a.totalTime(r) == b.totalTime(r)
 
 
+@pytest.mark.parametrize("numerics", ["exact", "fast"])
+def test_batch_persistent_launch_shapes_agree(numerics):
+    """Persistent stage launches (w wavefronts per CU looping over the work
+    list, including fewer wavefronts than one XCD's items) end bitwise where
+    one wavefront per item ends; the shape the autotuner picks is valid."""
+    runs = _mixed_runs()
+    print(f"case batch-persistent numerics={numerics}", flush=True)
+    ref = _batch(runs, numerics)
+    ref.steps(9)
+    for w in (1, 3, 8):
+        b = _batch(runs, numerics)
+        b.set_sweep_length(11)
+        b.set_waves_per_cu(w)
+        assert b.waves_per_cu() == w
+        b.steps(5)
+        b.set_sweep_length(24)      # a new work list keeps the shape
+        b.steps(4)
+        for r in range(len(runs)):
+            assert np.array_equal(ref.getU(r), b.getU(r)), (w, r)
+            assert ref.totalTime(r) == b.totalTime(r)
+    b = _batch(runs, numerics)
+    b.autotune(steps=1)
+    assert b.waves_per_cu() >= 0
+    b.steps(9)
+    for r in range(len(runs)):
+        assert np.array_equal(ref.getU(r), b.getU(r)), r
+
+
 def test_batch_init_and_boundary():
     cfgs = [mhd.orszag_tang_config(48), mhd.harris_config(64, 32)]
     b = mhd.ResistiveMHD2DBatch(cfgs)
