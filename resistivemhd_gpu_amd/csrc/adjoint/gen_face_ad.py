@@ -260,55 +260,64 @@ def tangent_ops(ops, d):
     return out
 
 
-def reverse_op(dst, op, args, active, ren):
+def reverse_op(dst, op, args, active, ren, first):
     """Adjoint statements of one three-address op; ren maps a primal name to
-    the local (recomputed) copy its value is read from."""
+    the local (recomputed) copy its value is read from.  first holds the
+    adjoints already written in program order: the first contribution to
+    an adjoint is assigned rather than added to its 0.0, because IEEE
+    arithmetic cannot fold 0.0 + x (it is +0.0 for x = -0.0)."""
+    def acc(v, e):
+        if v in first:
+            return f"{v}_b += {e};"
+        first.add(v)
+        return f"{v}_b = {e};"
     out = []
     a = args[0] if args else None
     b = args[1] if len(args) > 1 else None
     t = f"{dst}_b"
     V = lambda x: ren.get(x, x)
-    acc = []
+    terms = []
     if op == "add":
-        acc = [(a, t), (b, t)]
+        terms = [(a, t), (b, t)]
     elif op == "sub":
-        acc = [(a, t), (b, f"-{t}")]
+        terms = [(a, t), (b, f"-{t}")]
     elif op == "mul":
-        acc = [(a, f"{t} * {V(b)}"), (b, f"{t} * {V(a)}")]
+        terms = [(a, f"{t} * {V(b)}"), (b, f"{t} * {V(a)}")]
     elif op == "div":
         out.append(f"    const double {dst}_r = mhd_ad_rcp({V(b)});")
-        acc = [(a, f"{t} * {dst}_r"), (b, f"-{t} * {V(dst)} * {dst}_r")]
+        terms = [(a, f"{t} * {dst}_r"), (b, f"-{t} * {V(dst)} * {dst}_r")]
     elif op == "neg":
-        acc = [(a, f"-{t}")]
+        terms = [(a, f"-{t}")]
     elif op == "copy":
-        acc = [(a, t)]
+        terms = [(a, t)]
     elif op == "sqrt":
-        acc = [(a, f"({V(dst)} > 0.0 ? 0.5 * {t} * mhd_ad_rcp({V(dst)}) : 0.0)")]
+        terms = [(a, f"({V(dst)} > 0.0 ? 0.5 * {t} * mhd_ad_rcp({V(dst)}) : 0.0)")]
     elif op == "fabs":
-        acc = [(a, f"({V(a)} > 0.0 ? {t} : ({V(a)} < 0.0 ? -{t} : 0.0))")]
+        terms = [(a, f"({V(a)} > 0.0 ? {t} : ({V(a)} < 0.0 ? -{t} : 0.0))")]
     elif op in ("tmin", "tmax"):
         sel = f"({V(b)}) < ({V(a)})" if op == "tmin" else f"({V(a)}) < ({V(b)})"
         if active(a) and active(b):
-            out.append(f"    if ({sel}) {b}_b += {t}; else {a}_b += {t};")
+            sb, sa = acc(b, t), acc(a, t)
+            out.append(f"    if ({sel}) {sb} else {sa}")
         elif active(a):
-            out.append(f"    if (!({sel})) {a}_b += {t};")
+            out.append(f"    if (!({sel})) {acc(a, t)}")
         elif active(b):
-            out.append(f"    if ({sel}) {b}_b += {t};")
+            out.append(f"    if ({sel}) {acc(b, t)}")
         return out
     elif op in ("sgn", "ind"):
         return out
     elif op == "dlim":
         c = V(args[2])
         if active(b):
-            out.append(f"    if ({c} != 0.0) {b}_b += {t};")
+            out.append(f"    if ({c} != 0.0) {acc(b, t)}")
         if active(a):
-            out.append(f"    if ({c} == 0.0) {a}_b += {t};")
+            out.append(f"    if ({c} == 0.0) {acc(a, t)}")
         return out
     else:
         raise ValueError(op)
-    for v, e in acc:
+    for v, e in terms:
         if active(v):
-            out.append(f"    {v}_b += {e};")
+            out.append(f"    {acc(v, e)}")
     return out
 
 
@@ -339,8 +348,10 @@ def segmented_adjoint_lines(params, inputs, outputs, ops, segs):
     names = list(inputs) + [dst for dst, _, _ in ops]
     for n in names:
         out.append(f"    double {n}_b = 0.0;")
+    first = set()
     for i, v in enumerate(outputs):
-        out.append(f"    {v}_b += out_b[{i}];")
+        out.append(f"    {v}_b = out_b[{i}];")
+        first.add(v)
     for k in sorted(set(segs), reverse=True):
         idx = [n for n, sg in enumerate(segs) if sg == k]
         defined = {ops[n][0] for n in idx}
@@ -370,7 +381,7 @@ def segmented_adjoint_lines(params, inputs, outputs, ops, segs):
         for n in reversed(idx):
             dst, op, args = ops[n]
             if dst in live:
-                out += reverse_op(dst, op, args, active, ren)
+                out += reverse_op(dst, op, args, active, ren, first)
         out.append("    }")
     for i, v in enumerate(inputs):
         out.append(f"    in_b[{i}] = {v}_b;")
@@ -390,8 +401,10 @@ def adjoint_lines(params, inputs, outputs, ops, guards=None):
     names = list(inputs) + [dst for dst, _, _ in ops]
     for n in names:
         out.append(f"    double {n}_b = 0.0;")
+    first = set()
     for i, v in enumerate(outputs):
-        out.append(f"    {v}_b += out_b[{i}];")
+        out.append(f"    {v}_b = out_b[{i}];")
+        first.add(v)
     if guards is None:
         guards = [None] * len(ops)
     for g, idx in reversed(groups(guards)):
@@ -399,7 +412,7 @@ def adjoint_lines(params, inputs, outputs, ops, guards=None):
         for n in reversed(idx):
             dst, op, args = ops[n]
             if dst in live:
-                body += reverse_op(dst, op, args, active, {})
+                body += reverse_op(dst, op, args, active, {}, first)
         if g is None:
             out += body
         else:
