@@ -206,15 +206,19 @@ def primal_lines(params, inputs, ops, guards=None, fast=False):
 
 
 def tangent_lines(params, inputs, ops, guards=None):
-    cst = set(params)
-    d = lambda a: "0.0" if (is_lit(a) or a in cst) else f"{a}_d"
+    """Tangent statements.  Names whose tangent is identically zero (the
+    parameters, sgn and ind, and ops of such names only) enter the rules as
+    the literal 0.0, and terms multiplying it are left out: IEEE arithmetic
+    cannot drop 0.0 * x or 0.0 + x by itself."""
+    zero = set(params)
+    d = lambda a: "0.0" if (is_lit(a) or a in zero) else f"{a}_d"
     out = []
     for i, v in enumerate(inputs):
         out.append(f"    const double {v}_d = in_d[{i}];")
     if guards is None:
         guards = [None] * len(ops)
     for g, idx in groups(guards):
-        body = tangent_ops([ops[n] for n in idx], d)
+        body = tangent_ops([ops[n] for n in idx], d, zero)
         if g is None:
             out += [f"    const double {x}" for x in body]
         else:
@@ -225,37 +229,49 @@ def tangent_lines(params, inputs, ops, guards=None):
     return out
 
 
-def tangent_ops(ops, d):
+def tangent_ops(ops, d, zero):
+    """Forward-mode rules; a dst whose tangent comes out as 0.0 joins zero."""
+    Z = "0.0"
     out = []
     for dst, op, args in ops:
         a = args[0] if args else None
         b = args[1] if len(args) > 1 else None
+        da = d(a) if a is not None else Z
+        db = d(b) if b is not None else Z
         if op == "add":
-            e = f"{d(a)} + {d(b)}"
+            terms = [x for x in (da, db) if x != Z]
+            e = " + ".join(terms) if terms else Z
         elif op == "sub":
-            e = f"{d(a)} - {d(b)}"
+            e = da if db == Z else (f"-{db}" if da == Z else f"{da} - {db}")
         elif op == "mul":
-            e = f"{d(a)} * {b} + {a} * {d(b)}"
+            terms = ([f"{da} * {b}"] if da != Z else []) + ([f"{a} * {db}"] if db != Z else [])
+            e = " + ".join(terms) if terms else Z
         elif op == "div":
-            e = f"({d(a)} - {dst} * {d(b)}) * mhd_ad_rcp({b})"
+            if db == Z:
+                e = Z if da == Z else f"{da} * mhd_ad_rcp({b})"
+            else:
+                num = f"-{dst} * {db}" if da == Z else f"{da} - {dst} * {db}"
+                e = f"({num}) * mhd_ad_rcp({b})"
         elif op == "neg":
-            e = f"-{d(a)}"
+            e = Z if da == Z else f"-{da}"
         elif op == "copy":
-            e = d(a)
+            e = da
         elif op == "sqrt":
-            e = f"({dst} > 0.0 ? 0.5 * {d(a)} * mhd_ad_rcp({dst}) : 0.0)"
+            e = Z if da == Z else f"({dst} > 0.0 ? 0.5 * {da} * mhd_ad_rcp({dst}) : 0.0)"
         elif op == "fabs":
-            e = f"({a} > 0.0 ? {d(a)} : ({a} < 0.0 ? -{d(a)} : 0.0))"
+            e = Z if da == Z else f"({a} > 0.0 ? {da} : ({a} < 0.0 ? -{da} : 0.0))"
         elif op == "tmin":
-            e = f"(({b}) < ({a}) ? {d(b)} : {d(a)})"
+            e = Z if da == Z and db == Z else f"(({b}) < ({a}) ? {db} : {da})"
         elif op == "tmax":
-            e = f"(({a}) < ({b}) ? {d(b)} : {d(a)})"
+            e = Z if da == Z and db == Z else f"(({a}) < ({b}) ? {db} : {da})"
         elif op in ("sgn", "ind"):
-            e = "0.0"
+            e = Z
         elif op == "dlim":
-            e = f"({args[2]} != 0.0 ? {d(b)} : {d(a)})"
+            e = Z if da == Z and db == Z else f"({args[2]} != 0.0 ? {db} : {da})"
         else:
             raise ValueError(op)
+        if e == Z:
+            zero.add(dst)
         out.append(f"{dst}_d = {e};")
     return out
 
