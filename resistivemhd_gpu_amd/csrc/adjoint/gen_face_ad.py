@@ -92,10 +92,9 @@ class Lowering:
 
 
 def parse(path):
-    """Returns params, inputs, outputs, ops and the segment index of each op
-    (`segment:` lines split a listing into segments for the adjoint)."""
+    """Returns params, inputs, outputs and ops; parse.guards holds the guard
+    (`guard:` lines) each op is under."""
     params, inputs, outputs, stmts = [], [], [], []
-    seg = 0
     guard = None
     for line in open(path):
         line = line.split("#", 1)[0].strip()
@@ -107,22 +106,19 @@ def parse(path):
             inputs = line.split(":", 1)[1].split()
         elif line.startswith("outputs:"):
             outputs = line.split(":", 1)[1].split()
-        elif line.startswith("segment:"):
-            seg += 1 if stmts else 0
         elif line.startswith("guard:"):
             g = line.split(":", 1)[1].strip()
             guard = None if g == "none" else g
         else:
-            stmts.append((seg, guard, ast.parse(line).body[0]))
+            stmts.append((guard, ast.parse(line).body[0]))
     low = Lowering(params)
-    segs, guards = [], []
-    for sg, gd, st in stmts:
+    guards = []
+    for gd, st in stmts:
         n0 = len(low.ops)
         low.expr(st.value, dst=st.targets[0].id)
-        segs += [sg] * (len(low.ops) - n0)
         guards += [gd] * (len(low.ops) - n0)
     parse.guards = guards
-    return params, inputs, outputs, low.ops, segs
+    return params, inputs, outputs, low.ops
 
 
 C_OP = {"dlim": "{0}", "add": "{0} + {1}", "sub": "{0} - {1}", "mul": "{0} * {1}", "div": "{0} / {1}", "neg": "-{0}",
@@ -351,64 +347,6 @@ def grad_live(ops, outputs):
     return live
 
 
-def segmented_adjoint_lines(params, inputs, outputs, ops, segs):
-    """Checkpointed reverse mode: the primal runs once for the values that
-    cross segment boundaries; each segment is then recomputed from its
-    inputs (behind a register barrier, so the recomputation is not merged
-    with the first pass) right before its reverse, which bounds the live set
-    to one segment's intermediates."""
-    cst = set(params)
-    live = grad_live(ops, outputs)
-    active = lambda a: not (is_lit(a) or a in cst) and a in live
-    out = []
-    names = list(inputs) + [dst for dst, _, _ in ops]
-    for n in names:
-        out.append(f"    double {n}_b = 0.0;")
-    first = set()
-    for i, v in enumerate(outputs):
-        out.append(f"    {v}_b = out_b[{i}];")
-        first.add(v)
-    for k in sorted(set(segs), reverse=True):
-        idx = [n for n, sg in enumerate(segs) if sg == k]
-        defined = {ops[n][0] for n in idx}
-        used = []
-        for n in idx:
-            for a in ops[n][2]:
-                if active(a) and a not in defined and a not in used and not a.startswith("("):
-                    used.append(a)
-        ren = {}
-        out.append(f"    {{  // segment {k}: recompute, then reverse")
-        for a in used:
-            ren[a] = f"{a}_s{k}"
-            out.append(f"    double {a}_s{k} = {a};")
-            out.append(f"    MHD_AD_BARRIER({a}_s{k});")
-        for n in idx:
-            dst, op, args = ops[n]
-            ren[dst] = f"{dst}_s{k}"
-        for n in idx:
-            dst, op, args = ops[n]
-            if op == "ind":
-                cond = args[0]
-                for nm in sorted(ren, key=len, reverse=True):
-                    cond = re_sub_name(cond, nm, ren[nm])
-                out.append(f"    const double {dst}_s{k} = (double){cond};")
-            else:
-                out.append(f"    const double {dst}_s{k} = {C_OP[op].format(*[ren.get(x, x) for x in args])};")
-        for n in reversed(idx):
-            dst, op, args = ops[n]
-            if dst in live:
-                out += reverse_op(dst, op, args, active, ren, first)
-        out.append("    }")
-    for i, v in enumerate(inputs):
-        out.append(f"    in_b[{i}] = {v}_b;")
-    return out
-
-
-def re_sub_name(text, name, repl):
-    import re
-    return re.sub(r"\b" + re.escape(name) + r"\b", repl, text)
-
-
 def adjoint_lines(params, inputs, outputs, ops, guards=None):
     cst = set(params)
     live = grad_live(ops, outputs)
@@ -441,7 +379,7 @@ def adjoint_lines(params, inputs, outputs, ops, guards=None):
 
 
 def emit(name, path):
-    params, inputs, outputs, ops, segs = parse(path)
+    params, inputs, outputs, ops = parse(path)
     guards = parse.guards
     src = os.path.basename(path)
     n_in, n_out, n_par = len(inputs), len(outputs), len(params)
@@ -470,12 +408,9 @@ def emit(name, path):
               f"MHD_HD inline void {name}_adj(const double *in, const double *par, const double *out_b,"
               f" double *in_b)", "{"]
     lines += primal_lines(params, inputs, ops, guards, fast=True)
-    if len(set(segs)) > 1:
-        lines += segmented_adjoint_lines(params, inputs, outputs, ops, segs)
-    else:
-        lines += ["    {", "    MHD_AD_CONTRACT"]
-        lines += adjoint_lines(params, inputs, outputs, ops, guards)
-        lines += ["    }"]
+    lines += ["    {", "    MHD_AD_CONTRACT"]
+    lines += adjoint_lines(params, inputs, outputs, ops, guards)
+    lines += ["    }"]
     lines += ["}", ""]
     return lines
 
@@ -506,14 +441,6 @@ def main(argv):
             "",
             "namespace mhd_ad {",
             "",
-            "// Reciprocal for the derivative factors of the tangent and adjoint",
-            "// passes and for the adjoint's primal values that no branch depends on",
-            "// (every branch input keeps IEEE division, so branch decisions replay",
-            "// the forward step exactly): on the GPU the hardware estimate refined",
-            "// by two Newton steps (within an ulp or two); on the host 1/x.",
-            "// Register barrier: the compiler must treat x as a new value, so a",
-            "// segment recomputed for the reverse pass is not merged with (and does",
-            "// not keep alive) the first primal pass.",
             "// The tangent and adjoint passes may contract a*b+c to one FMA (the",
             "// build keeps contraction off elsewhere, for bitwise parity).",
             "#if defined(__clang__)",
@@ -521,12 +448,12 @@ def main(argv):
             "#else",
             "#define MHD_AD_CONTRACT",
             "#endif",
-            "#if defined(__HIP_DEVICE_COMPILE__)",
-            "#define MHD_AD_BARRIER(x) asm volatile(\"\" : \"+v\"(x))",
-            "#else",
-            "#define MHD_AD_BARRIER(x) ((void)0)",
-            "#endif",
             "",
+            "// Reciprocal for the derivative factors of the tangent and adjoint",
+            "// passes and for the adjoint's primal values that no branch depends on",
+            "// (every branch input keeps IEEE division, so branch decisions replay",
+            "// the forward step exactly): on the GPU the hardware estimate refined",
+            "// by two Newton steps (within an ulp or two); on the host 1/x.",
             "MHD_HD inline double mhd_ad_rcp(double x)",
             "{",
             "#if defined(__HIP_DEVICE_COMPILE__)",
