@@ -406,7 +406,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build runs the step on a single MI355X "
+        print("bench.py measures one GPU only: the step runs on a single MI355X "
               "(multi-GPU sharding is a design, not a built path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 

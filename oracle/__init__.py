@@ -1,3 +1,3 @@
 """CPU oracle of the reference's resistive-MHD RK2 step (test infrastructure
-only: used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
-leg as the checker; never part of the product path)."""
+only: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+compare against it; never part of the product path)."""
