@@ -267,18 +267,24 @@ def test_batched_adjoint_and_stride_agree():
             np.testing.assert_allclose(ge, grads[r][1], rtol=1e-6, atol=1e-9 * np.abs(ge).max())
 
 
-def test_torch_autograd_function():
+@pytest.mark.parametrize("case", ["ot", "harris-degenerate"])
+def test_torch_autograd_function(case):
+    """The torch.autograd.Function against the C ABI, with the exact
+    derivative (Orszag-Tang) and with the degenerate star-state rule on the
+    Harris far field (the tolerance passes through)."""
     import torch
     from resistivemhd_gpu_amd.torch_adjoint import rollout
-    cfg, U0 = _ot(24)
+    cfg, U0 = _ot(24) if case == "ot" else _harris_far_field(48, 24)
+    tol = 0.0 if case == "ot" else 1e-8
     eta = _eta_field(cfg, 5)
     W = _weights(cfg, 6)
     U0t = torch.tensor(U0[None], dtype=torch.float64, device="cuda", requires_grad=True)
     etat = torch.tensor(eta[None], dtype=torch.float64, device="cuda", requires_grad=True)
-    UN = rollout(U0t, etat, [cfg], 12)
+    UN = rollout(U0t, etat, [cfg], 12, degenerate_tol=tol)
     loss = (UN * torch.tensor(W[None], device="cuda")).sum()
     loss.backward()
     a = mhd.ResistiveMHD2DAdjoint([cfg])
+    a.set_degenerate_tol(tol)
     a.setU0(0, U0)
     a.setEtaField(0, eta)
     a.forward(12)
