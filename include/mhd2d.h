@@ -230,6 +230,10 @@ const char *mhd_batch_last_error(const mhd_batch *b);   /* b may be NULL */
 int mhd_batch_size(const mhd_batch *b, int *nruns);
 int mhd_batch_set_state(mhd_batch *b, int run, const double *host_aos);
 int mhd_batch_get_state(mhd_batch *b, int run, double *host_aos);
+/* ResistiveMHD2D::save (P/resistiveMHD_2D.cu:198-244) of one run, in
+ * mhd_save's layout ({dir}/{name}_{step}.bin), and the matching reader. */
+int mhd_batch_save(mhd_batch *b, int run, const char *dir, const char *name, int step);
+int mhd_batch_load(mhd_batch *b, int run, const char *path);
 int mhd_batch_set_eta_field(mhd_batch *b, int run, const double *host_eta);   /* NULL: uniform cfg.eta */
 /* the configured boundary of every run (x then y) */
 int mhd_batch_apply_boundary(mhd_batch *b);

@@ -13,7 +13,9 @@
 // mhd_batch_autotune, which times candidates on the batch itself.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cerrno>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -340,6 +342,52 @@ int mhd_batch_get_state(mhd_batch *b, int run, double *host_aos)
     BHIP(b, hipMemcpyAsync(host_aos, b->aos, bytes, hipMemcpyDeviceToHost, b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     return MHD_OK;
+}
+
+// ResistiveMHD2D::save (P/resistiveMHD_2D.cu:198-244) for one run: the
+// same file layout as mhd_save.
+int mhd_batch_save(mhd_batch *b, int run, const char *dir, const char *name, int step)
+{
+    int rc = check_run(b, run);
+    if (rc) return rc;
+    if (!dir || !name) return bfail(b, MHD_ERR_INVALID, "NULL argument");
+    const Grid &g = b->desc[run].g;
+    std::vector<double> buf;
+    try {
+        buf.resize(8 * (size_t)g.nx * g.ny);
+    } catch (...) {
+        return bfail(b, MHD_ERR_NOMEM, "host allocation failed");
+    }
+    if ((rc = mhd_batch_get_state(b, run, buf.data()))) return rc;
+    const std::string path = std::string(dir) + "/" + name + "_" + std::to_string(step) + ".bin";
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return bfail(b, MHD_ERR_IO, "cannot open " + path + ": " + std::strerror(errno));
+    const size_t n = std::fwrite(buf.data(), sizeof(double), buf.size(), f);
+    const int closed = std::fclose(f);
+    if (n != buf.size() || closed != 0) return bfail(b, MHD_ERR_IO, "short write to " + path);
+    return MHD_OK;
+}
+
+int mhd_batch_load(mhd_batch *b, int run, const char *path)
+{
+    int rc = check_run(b, run);
+    if (rc) return rc;
+    if (!path) return bfail(b, MHD_ERR_INVALID, "NULL argument");
+    const Grid &g = b->desc[run].g;
+    std::vector<double> buf;
+    try {
+        buf.resize(8 * (size_t)g.nx * g.ny);
+    } catch (...) {
+        return bfail(b, MHD_ERR_NOMEM, "host allocation failed");
+    }
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return bfail(b, MHD_ERR_IO, std::string("cannot open ") + path + ": " + std::strerror(errno));
+    const size_t n = std::fread(buf.data(), sizeof(double), buf.size(), f);
+    const bool extra = std::fgetc(f) != EOF;
+    std::fclose(f);
+    if (n != buf.size() || extra)
+        return bfail(b, MHD_ERR_IO, std::string(path) + ": size does not match nx*ny*8 doubles");
+    return mhd_batch_set_state(b, run, buf.data());
 }
 
 int mhd_batch_set_eta_field(mhd_batch *b, int run, const double *host_eta)

@@ -35,7 +35,7 @@ EXPORTS = (
     "mhd_batch_init_harris", "mhd_batch_step", "mhd_batch_synchronize", "mhd_batch_get_dt", "mhd_batch_get_time",
     "mhd_batch_check_crashed", "mhd_batch_set_numerics", "mhd_batch_set_sweep_length",
     "mhd_batch_get_sweep_length", "mhd_batch_set_graph", "mhd_batch_autotune", "mhd_batch_set_waves_per_cu",
-    "mhd_batch_get_waves_per_cu",
+    "mhd_batch_get_waves_per_cu", "mhd_batch_save", "mhd_batch_load",
     "mhd_adjoint_create", "mhd_adjoint_destroy", "mhd_adjoint_last_error", "mhd_adjoint_set_eta_field",
     "mhd_adjoint_set_state", "mhd_adjoint_set_dts", "mhd_adjoint_get_dts", "mhd_adjoint_forward",
     "mhd_adjoint_get_state", "mhd_adjoint_set_seed", "mhd_adjoint_backward", "mhd_adjoint_get_grad_state",
@@ -137,6 +137,8 @@ def load_library(path=LIB_PATH):
         "mhd_batch_size": (I, [P, ctypes.POINTER(I)]),
         "mhd_batch_set_state": (I, [P, I, D]),
         "mhd_batch_get_state": (I, [P, I, D]),
+        "mhd_batch_save": (I, [P, I, ctypes.c_char_p, ctypes.c_char_p, I]),
+        "mhd_batch_load": (I, [P, I, ctypes.c_char_p]),
         "mhd_batch_set_eta_field": (I, [P, I, D]),
         "mhd_batch_apply_boundary": (I, [P]),
         "mhd_batch_init_orszag_tang": (I, [P, I]),
@@ -506,6 +508,14 @@ class ResistiveMHD2DBatch:
         out = np.empty(self.shape(run), dtype=np.float64)
         self._ok(self._lib.mhd_batch_get_state(self._h, run, _dptr(out)))
         return out
+
+    def save(self, run, directoryname, filenameWithoutStep, step):
+        """ResistiveMHD2D.save of one run: {dir}/{name}_{step}.bin."""
+        self._ok(self._lib.mhd_batch_save(self._h, run, os.fsencode(directoryname),
+                                          os.fsencode(filenameWithoutStep), int(step)))
+
+    def load(self, run, path):
+        self._ok(self._lib.mhd_batch_load(self._h, run, os.fsencode(path)))
 
     def setEtaField(self, run, eta):
         if eta is None:

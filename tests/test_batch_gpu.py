@@ -141,3 +141,24 @@ def test_batch_init_and_boundary():
     assert np.array_equal(b.getU(1), boundary_applied(cfgs[1], mhd.fill_harris(cfgs[1], seed=2, noise_amp=1e-3)))
     with pytest.raises(mhd.MHDError):
         b.initOrszagTang(5)
+
+
+def test_batch_save_matches_single_handle_and_loads_back(tmp_path):
+    """A run's snapshot is byte for byte the file a single handle writes for
+    the same state (the reference's save layout), and loading it restores
+    the run; a file of the wrong size is refused."""
+    runs = _mixed_runs()[:3]
+    print("case batch-save-load", flush=True)
+    b = _batch(runs)
+    b.steps(5)
+    for r, (c, U0, eta) in enumerate(runs):
+        s = _single(c, U0, eta, 5, "exact")
+        b.save(r, str(tmp_path), f"batch{r}", 5)
+        s.save(str(tmp_path), f"single{r}", 5)
+        assert (tmp_path / f"batch{r}_5.bin").read_bytes() == (tmp_path / f"single{r}_5.bin").read_bytes(), r
+    U1 = b.getU(1)
+    b.steps(3)
+    b.load(1, str(tmp_path / "batch1_5.bin"))
+    assert np.array_equal(b.getU(1), U1)
+    with pytest.raises(mhd.MHDError):
+        b.load(0, str(tmp_path / "batch1_5.bin"))   # another run's grid size
