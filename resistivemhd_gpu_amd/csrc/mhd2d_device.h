@@ -99,6 +99,27 @@ __device__ __forceinline__ double div_by(double a, double b, const SharedDen &d)
     return __builtin_amdgcn_div_fixup(q, b, a);
 }
 
+// The neighbouring lane's value, as __shfl_up(x, 1, 64) (lane_prev: lane
+// l-1's, lane 0 keeps its own) and __shfl_down(x, 1, 64) (lane_next: lane
+// l+1's, lane 63 keeps its own), by DPP wavefront shifts (wave_shr:1,
+// wave_shl:1): two VALU moves instead of two LDS permutes and their wait.
+__device__ __forceinline__ double dpp_shift(double x, int ctrl)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = (int)(b & 0xffffffff), hi = (int)(b >> 32);
+    int l2, h2;
+    if (ctrl == 0x138) {
+        l2 = __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xF, 0xF, false);
+        h2 = __builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xF, 0xF, false);
+    } else {
+        l2 = __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xF, 0xF, false);
+        h2 = __builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xF, 0xF, false);
+    }
+    return __longlong_as_double(((long long)h2 << 32) | (unsigned int)l2);
+}
+__device__ __forceinline__ double lane_prev(double x) { return dpp_shift(x, 0x138); }
+__device__ __forceinline__ double lane_next(double x) { return dpp_shift(x, 0x130); }
+
 // P/hlld.cu:26-32
 __device__ __forceinline__ double sgn(double x) { return (double)((x > 0.0) - (x < 0.0)); }
 

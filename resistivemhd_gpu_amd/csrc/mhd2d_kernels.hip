@@ -177,8 +177,8 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             const double sp = minmod(B.px - A.px, C.px - B.px, ph.eps);
 
             // resistive terms of cell c-1 (EX/bhattacharjee/main_bhattacharjee.cu:142-241)
-            const double bxc_up = __shfl_down(B.bxc, 1, 64), bxc_dn = __shfl_up(B.bxc, 1, 64);
-            const double bz_up = __shfl_down(B.bz, 1, 64), bz_dn = __shfl_up(B.bz, 1, 64);
+            const double bxc_up = lane_next(B.bxc), bxc_dn = lane_prev(B.bxc);
+            const double bz_up = lane_next(B.bz), bz_dn = lane_prev(B.bz);
             const double eta = eta_prev;
             double jX, jY, jZ;
             if (EXACT) {
@@ -249,12 +249,12 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             Flux8 G;
             {
                 const double q0 = B.rho, q1 = B.v, q2 = B.w, q3 = B.u, q4 = B.bz, q5 = B.bxc, q6 = B.py;
-                const double m0 = __shfl_up(q0, 1, 64), m1 = __shfl_up(q1, 1, 64), m2 = __shfl_up(q2, 1, 64);
-                const double m3 = __shfl_up(q3, 1, 64), m4 = __shfl_up(q4, 1, 64), m5 = __shfl_up(q5, 1, 64);
-                const double m6 = __shfl_up(q6, 1, 64);
-                const double p0 = __shfl_down(q0, 1, 64), p1 = __shfl_down(q1, 1, 64), p2 = __shfl_down(q2, 1, 64);
-                const double p3 = __shfl_down(q3, 1, 64), p4 = __shfl_down(q4, 1, 64), p5 = __shfl_down(q5, 1, 64);
-                const double p6 = __shfl_down(q6, 1, 64);
+                const double m0 = lane_prev(q0), m1 = lane_prev(q1), m2 = lane_prev(q2);
+                const double m3 = lane_prev(q3), m4 = lane_prev(q4), m5 = lane_prev(q5);
+                const double m6 = lane_prev(q6);
+                const double p0 = lane_next(q0), p1 = lane_next(q1), p2 = lane_next(q2);
+                const double p3 = lane_next(q3), p4 = lane_next(q4), p5 = lane_next(q5);
+                const double p6 = lane_next(q6);
                 const double t0 = minmod(q0 - m0, p0 - q0, ph.eps), t1 = minmod(q1 - m1, p1 - q1, ph.eps);
                 const double t2 = minmod(q2 - m2, p2 - q2, ph.eps), t3 = minmod(q3 - m3, p3 - q3, ph.eps);
                 const double t4 = minmod(q4 - m4, p4 - q4, ph.eps), t5 = minmod(q5 - m5, p5 - q5, ph.eps);
@@ -262,21 +262,21 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                 Face7 Lg, Rg;
                 Lg.rho = q0 + 0.5 * t0; Lg.u = q1 + 0.5 * t1; Lg.v = q2 + 0.5 * t2; Lg.w = q3 + 0.5 * t3;
                 Lg.by = q4 + 0.5 * t4; Lg.bz = q5 + 0.5 * t5; Lg.p = q6 + 0.5 * t6;
-                Rg.rho = __shfl_down(q0 - 0.5 * t0, 1, 64);
-                Rg.u = __shfl_down(q1 - 0.5 * t1, 1, 64);
-                Rg.v = __shfl_down(q2 - 0.5 * t2, 1, 64);
-                Rg.w = __shfl_down(q3 - 0.5 * t3, 1, 64);
-                Rg.by = __shfl_down(q4 - 0.5 * t4, 1, 64);
-                Rg.bz = __shfl_down(q5 - 0.5 * t5, 1, 64);
-                Rg.p = __shfl_down(q6 - 0.5 * t6, 1, 64);
-                const double bn = 0.5 * (B.byc + __shfl_down(B.byc, 1, 64));
+                Rg.rho = lane_next(q0 - 0.5 * t0);
+                Rg.u = lane_next(q1 - 0.5 * t1);
+                Rg.v = lane_next(q2 - 0.5 * t2);
+                Rg.w = lane_next(q3 - 0.5 * t3);
+                Rg.by = lane_next(q4 - 0.5 * t4);
+                Rg.bz = lane_next(q5 - 0.5 * t5);
+                Rg.p = lane_next(q6 - 0.5 * t6);
+                const double bn = 0.5 * (B.byc + lane_next(B.byc));
                 const Flux8 Gr = hlld<EXACT>(Lg, Rg, bn, ph, inv_gm1);
                 // rotate back (P/hlld.cu:170-194)
                 G.f0 = Gr.f0; G.f1 = Gr.f3; G.f2 = Gr.f1; G.f3 = Gr.f2;
                 G.f4 = Gr.f6; G.f5 = Gr.f4; G.f6 = Gr.f5; G.f7 = Gr.f7;
                 if (MULTI) {
                     // face (c-1, j+1/2): cells j and j+1 (M/flux_solver.cu:158-189)
-                    const double tA1 = __shfl_down(tA, 1, 64), tB1 = __shfl_down(tB, 1, 64);
+                    const double tA1 = lane_next(tA), tB1 = lane_next(tB);
                     const double gjX = EXACT ? (bz_up - B.bz) / ph.dy : (bz_up - B.bz) * (2.0 * inv_2dy);
                     const double gjZ = 0.5 * (((tA - tB) + tA1) - tB1);
                     const double gJX = ph.eta * gjX, gJZ = ph.eta * gjZ;
@@ -286,8 +286,8 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     G.f6 -= gJX;
                     G.f7 += gJZBX - gJXBZ;
                 } else {
-                    const double eJZ1 = __shfl_down(eJZ, 1, 64), eJX1 = __shfl_down(eJX, 1, 64);
-                    const double eJZBX1 = __shfl_down(eJZBX, 1, 64), eJXBZ1 = __shfl_down(eJXBZ, 1, 64);
+                    const double eJZ1 = lane_next(eJZ), eJX1 = lane_next(eJX);
+                    const double eJZBX1 = lane_next(eJZBX), eJXBZ1 = lane_next(eJXBZ);
                     G.f4 += 0.5 * (eJZ + eJZ1);
                     G.f6 -= 0.5 * (eJX + eJX1);
                     G.f7 += 0.5 * (eJZBX + eJZBX1) - 0.5 * (eJXBZ + eJXBZ1);
@@ -298,21 +298,21 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             // stage-2 CT reuses stage-1 fluxes, P/resistiveMHD_2D.cu:167-168,190)
             double E = 0.0;
             if (!MULTI && STAGE == 1 && c >= i0 + 1) {
-                const double f5up = __shfl_down(F.f5, 1, 64);
+                const double f5up = lane_next(F.f5);
                 E = 0.25 * (Gprev.f4 + G.f4 + (-1.0 * F.f5) + (-1.0 * f5up));
             }
 
             if (c >= i0 + 2) {
                 const int s = c - 2;
-                const double g0m = __shfl_up(Gprev.f0, 1, 64), g1m = __shfl_up(Gprev.f1, 1, 64);
-                const double g2m = __shfl_up(Gprev.f2, 1, 64), g3m = __shfl_up(Gprev.f3, 1, 64);
-                const double g6m = __shfl_up(Gprev.f6, 1, 64), g7m = __shfl_up(Gprev.f7, 1, 64);
-                const double Em = __shfl_up(E, 1, 64);
+                const double g0m = lane_prev(Gprev.f0), g1m = lane_prev(Gprev.f1);
+                const double g2m = lane_prev(Gprev.f2), g3m = lane_prev(Gprev.f3);
+                const double g6m = lane_prev(Gprev.f6), g7m = lane_prev(Gprev.f7);
+                const double Em = lane_prev(E);
                 if (MULTI) {
                     // face-smoothed B of the stage input (shift + back,
                     // M/resistiveMHD2D.cu:597-690): cells s = A and s+1 = B
-                    const double g4m = __shfl_up(Gprev.f4, 1, 64), g5m = __shfl_up(Gprev.f5, 1, 64);
-                    const double byc_up = __shfl_down(A.byc, 1, 64);
+                    const double g4m = lane_prev(Gprev.f4), g5m = lane_prev(Gprev.f5);
+                    const double byc_up = lane_next(A.byc);
                     const double sbx = 0.5 * (A.bxc + B.bxc);
                     const double sby = 0.5 * (A.byc + byc_up);
                     double obx = 0.0, oby = 0.0;
@@ -322,7 +322,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                         const double bxs = PL(uold, 4)[on], bxm = PL(uold, 4)[on - g.pitch];
                         const double bxp = PL(uold, 4)[on + g.pitch];
                         const double bys = PL(uold, 5)[on], bym1 = PL(uold, 5)[on - 1];
-                        const double byp = __shfl_down(bys, 1, 64);
+                        const double byp = lane_next(bys);
                         const double cxs = 0.5 * (bxs + bxm), cxp = 0.5 * (bxp + bxs);
                         const double cys = 0.5 * (bys + bym1), cyp = 0.5 * (byp + bys);
                         obx = 0.5 * (cxs + cxp);
