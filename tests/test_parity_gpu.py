@@ -121,3 +121,25 @@ def test_harris_4096x2048_full_size_properties():
     sim2.steps(steps)
     assert np.array_equal(np.roll(U1[g:-g, g:-g], sx, axis=0), sim2.getU()[g:-g, g:-g])
     assert sim2.dt == sim.dt
+
+
+def test_harris_4096x2048_configured_length():
+    """The headline run at its configured length (Harris sheet 4096 x 2048,
+    2000 steps): it does not crash, every value stays finite, and the
+    discrete divergence of B stays at round-off throughout."""
+    cfg = mhd.harris_config(4096, 2048)
+    print("case harris4096x2048-2000-steps", flush=True)
+    sim = mhd.ResistiveMHD2D(cfg)
+    sim.initHarris(seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3)
+    t = 0.0
+    for chunk in range(4):
+        sim.steps(500)
+        assert not sim.checkCalculationIsCrashed(), chunk
+        U = sim.getU()
+        assert np.all(np.isfinite(U)), chunk
+        bscale = np.abs(U[..., 4:6]).max() / min(cfg.dx, cfg.dy)
+        div = np.abs(_div_b_interior(cfg, U)).max()
+        print(f"steps {500 * (chunk + 1)}: time {sim.totalTime:.4f} max|divB| {div:.2e}", flush=True)
+        assert div < 1e-12 * bscale, chunk
+        assert sim.totalTime > t
+        t = sim.totalTime
