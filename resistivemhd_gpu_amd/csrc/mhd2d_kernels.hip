@@ -566,65 +566,63 @@ __global__ __launch_bounds__(256) void bc_x_kernel(double *u, Ctrl *ctrl, int sl
 }
 
 // Periodic y (P/boundary.cu:74-141) or symmetric y (S/boundary.cu:107-205),
-// one thread per column, after the x boundary.
+// after the x boundary.  One ghost cell per lane: lanes 6c .. 6c+5 of a
+// wavefront take column col0 + c, ghost rows 0, 1, 2 and ny-3, ny-2, ny-1
+// (lanes 60..63 idle), so a load touches the two 24-byte ghost runs of ten
+// neighbouring columns instead of 64 columns' worth of lines.
+constexpr int kBcyCols = 10;   // columns per wavefront
+
 template <int KIND, bool EXACT>
-__device__ __forceinline__ void bc_y_col(int i, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph)
+__device__ __forceinline__ void bc_y_cells(int col0, double *u, Ctrl *ctrl, int slot, const Grid &g,
+                                           const Phys &ph)
 {
     double dloc = __longlong_as_double((long long)kInfBits);
-    if (i < g.nx) {
+    const int lane = threadIdx.x & 63;
+    const int q = lane % 6;
+    const int i = col0 + lane / 6;
+    if (lane < 6 * kBcyCols && i < g.nx) {
         const int ny = g.ny;
         const size_t row = (size_t)i * g.pitch;
-        const int dst[6] = {0, 1, 2, ny - 3, ny - 2, ny - 1};
-        int src[6];
-        if (KIND == 0) { src[0] = ny - 6; src[1] = ny - 5; src[2] = ny - 4; src[3] = 3; src[4] = 4; src[5] = 5; }
-        else           { src[0] = 5; src[1] = 4; src[2] = 3; src[3] = ny - 4; src[4] = ny - 5; src[5] = ny - 6; }
-        double cell[6][8];
+        const int dst = q < 3 ? q : ny - 6 + q;   // 0, 1, 2, ny-3, ny-2, ny-1
+        const int src = KIND == 0 ? (q < 3 ? ny - 6 + q : q)           // ny-6 .. ny-4, 3 .. 5
+                                  : (q < 3 ? 5 - q : ny - 1 - q);      // 5, 4, 3, ny-4, ny-5, ny-6
+        double cell[8];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                if (KIND == 1 && v == 5) continue;
-                cell[q][v] = PL(u, v)[row + src[q]];
-                PL(u, v)[row + dst[q]] = cell[q][v];
-            }
+        for (int v = 0; v < 8; ++v) {
+            if (KIND == 1 && v == 5) continue;
+            cell[v] = PL(u, v)[row + src];
+            PL(u, v)[row + dst] = cell[v];
         }
         if (KIND == 1) {
+            // By of the ghost rows from the zero divergence of the mirrored
+            // Bx: rows 2, 1, 0 from rows 3, 3, 4 and rows ny-3, ny-2, ny-1
+            // from rows ny-4, ny-5, ny-6, each added in turn
             const double *bx = PL(u, 4);
             double *by = PL(u, 5);
+            double b;
             if (i > 0) {
-                // Bx of ghost rows 2, 1, 0 equals that of rows 3, 4, 5 (and
-                // rows ny-3.. that of ny-4..), column i-1 included
                 const size_t rm = (size_t)(i - 1) * g.pitch;
-                const double d3 = bx[row + 3] - bx[rm + 3];
-                const double d4 = bx[row + 4] - bx[rm + 4];
-                const double dA = bx[row + ny - 4] - bx[rm + ny - 4];
-                const double dB = bx[row + ny - 5] - bx[rm + ny - 5];
-                const double dC = bx[row + ny - 6] - bx[rm + ny - 6];
-                cell[2][5] = by[row + 3] + d3 / ph.dx * ph.dy;
-                cell[1][5] = cell[2][5] + d3 / ph.dx * ph.dy;
-                cell[0][5] = cell[1][5] + d4 / ph.dx * ph.dy;
-                cell[3][5] = by[row + ny - 4] - dA / ph.dx * ph.dy;
-                cell[4][5] = cell[3][5] - dB / ph.dx * ph.dy;
-                cell[5][5] = cell[4][5] - dC / ph.dx * ph.dy;
+                if (q < 3) {
+                    const double d3 = bx[row + 3] - bx[rm + 3];
+                    b = by[row + 3] + d3 / ph.dx * ph.dy;                  // row 2
+                    if (q < 2) b = b + d3 / ph.dx * ph.dy;                  // row 1
+                    if (q < 1) b = b + (bx[row + 4] - bx[rm + 4]) / ph.dx * ph.dy;   // row 0
+                } else {
+                    const double dA = bx[row + ny - 4] - bx[rm + ny - 4];
+                    b = by[row + ny - 4] - dA / ph.dx * ph.dy;             // row ny-3
+                    if (q > 3) b = b - (bx[row + ny - 5] - bx[rm + ny - 5]) / ph.dx * ph.dy;   // row ny-2
+                    if (q > 4) b = b - (bx[row + ny - 6] - bx[rm + ny - 6]) / ph.dx * ph.dy;   // row ny-1
+                }
             } else {
-                cell[2][5] = by[row + 3];
-                cell[1][5] = cell[2][5];
-                cell[0][5] = cell[1][5];
-                cell[3][5] = by[row + ny - 4];
-                cell[4][5] = cell[3][5];
-                cell[5][5] = cell[4][5];
+                b = q < 3 ? by[row + 3] : by[row + ny - 4];
             }
-#pragma unroll
-            for (int q = 0; q < 6; ++q) by[row + dst[q]] = cell[q][5];
+            cell[5] = b;
+            by[row + dst] = b;
         }
         if (ctrl) {
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const double d = cell_dt<EXACT>(cell[q][0], cell[q][1], cell[q][2], cell[q][3], cell[q][4], cell[q][5],
-                                         cell[q][6], cell[q][7], ph);
-                if (d < dloc) dloc = d;
-                if (q == 0 && i == 0 && isnan(d)) ctrl->first_nan[slot] = 1;   // cell (0, 0)
-            }
+            const double d = cell_dt<EXACT>(cell[0], cell[1], cell[2], cell[3], cell[4], cell[5], cell[6], cell[7], ph);
+            if (d < dloc) dloc = d;
+            if (q == 0 && i == 0 && isnan(d)) ctrl->first_nan[slot] = 1;   // cell (0, 0)
         }
     }
     if (ctrl) accumulate_dt(ctrl, slot, dloc);
@@ -633,7 +631,8 @@ __device__ __forceinline__ void bc_y_col(int i, double *u, Ctrl *ctrl, int slot,
 template <int KIND, bool EXACT>
 __global__ __launch_bounds__(256) void bc_y_kernel(double *u, Ctrl *ctrl, int slot, Grid g, Phys ph)
 {
-    bc_y_col<KIND, EXACT>(blockIdx.x * blockDim.x + threadIdx.x, u, ctrl, slot, g, ph);
+    bc_y_cells<KIND, EXACT>((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kBcyCols, u, ctrl, slot, g,
+                            ph);
 }
 
 // Batched boundaries: blockIdx.y is the run; which state (U or U*) and
@@ -660,12 +659,12 @@ __global__ __launch_bounds__(256) void bc_y_batch_kernel(const RunDesc *__restri
     const RunDesc &d = runs[blockIdx.y];
     const Grid g = d.g;
     const Phys ph = d.ph;
-    if ((int)(blockIdx.x * blockDim.x) >= g.nx) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)(blockIdx.x * (blockDim.x >> 6)) * kBcyCols >= g.nx) return;   // whole workgroup past this run's columns
+    const int col0 = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kBcyCols;
     double *u = stage_state ? d.us : d.u;
     Ctrl *ctrl = with_dt ? d.ctrl : nullptr;
-    if (d.bc_y == 0) bc_y_col<0, EXACT>(i, u, ctrl, slot, g, ph);
-    else bc_y_col<1, EXACT>(i, u, ctrl, slot, g, ph);
+    if (d.bc_y == 0) bc_y_cells<0, EXACT>(col0, u, ctrl, slot, g, ph);
+    else bc_y_cells<1, EXACT>(col0, u, ctrl, slot, g, ph);
 }
 
 // Per-cell dt over all nx*ny cells (P/resistiveMHD_2D.cu:277-286) and the
@@ -947,7 +946,7 @@ hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const 
 hipError_t launch_bc_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int max_ny, bool stage_state,
                            bool with_dt, int slot, hipStream_t s)
 {
-    const dim3 gx((max_ny + 255) / 256, nruns), gy((max_nx + 255) / 256, nruns);
+    const dim3 gx((max_ny + 255) / 256, nruns), gy((max_nx + 4 * kBcyCols - 1) / (4 * kBcyCols), nruns);
     const int st = stage_state ? 1 : 0, wd = with_dt ? 1 : 0;
     if (exact) {
         hipLaunchKernelGGL(bc_x_batch_kernel<true>, gx, dim3(256), 0, s, runs, st, wd, slot);
@@ -1026,7 +1025,7 @@ hipError_t launch_bc_x(int kind, bool exact, double *u, Ctrl *ctrl, int slot, co
 hipError_t launch_bc_y(int kind, bool exact, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
                        hipStream_t s)
 {
-    const int blocks = (g.nx + 255) / 256;
+    const int blocks = (g.nx + 4 * kBcyCols - 1) / (4 * kBcyCols);   // four wavefronts of ten columns each
     if (kind == 0) {
         if (exact) hipLaunchKernelGGL((bc_y_kernel<0, true>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
         else hipLaunchKernelGGL((bc_y_kernel<0, false>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
