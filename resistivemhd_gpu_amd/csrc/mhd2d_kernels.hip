@@ -495,74 +495,73 @@ void rk_stage_batch_kernel(const RunDesc *__restrict__ runs, const int *__restri
 }
 
 // ------------------------------------------------------------ boundaries
-// Periodic x (P/boundary.cu:4-71) or symmetric x (S/boundary.cu:4-104), one
-// thread per row.  The symmetric bX rebuild reads By of the interior columns
-// the mirror copied from (identical values), so one pass suffices.
+// Periodic x (P/boundary.cu:4-71) or symmetric x (S/boundary.cu:4-104).
+// One ghost cell per lane: a wavefront takes one ghost column q (columns 0,
+// 1, 2, nx-3, nx-2, nx-1) and 64 consecutive rows, so each plane's load and
+// store is one coalesced run.  The symmetric bX rebuild reads By of the
+// interior columns the mirror copied from (identical values), so one pass
+// suffices.
 template <int KIND, bool EXACT>
-__device__ __forceinline__ void bc_x_row(int j, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph)
+__device__ __forceinline__ void bc_x_cells(int q, int j, double *u, Ctrl *ctrl, int slot, const Grid &g,
+                                           const Phys &ph)
 {
     double dloc = __longlong_as_double((long long)kInfBits);
     if (j < g.ny) {
         const int nx = g.nx;
-        const int dst[6] = {0, 1, 2, nx - 3, nx - 2, nx - 1};
-        int src[6];
-        if (KIND == 0) { src[0] = nx - 6; src[1] = nx - 5; src[2] = nx - 4; src[3] = 3; src[4] = 4; src[5] = 5; }
-        else           { src[0] = 5; src[1] = 4; src[2] = 3; src[3] = nx - 4; src[4] = nx - 5; src[5] = nx - 6; }
-        double cell[6][8];
+        const size_t pitch = g.pitch;
+        const int dst = q < 3 ? q : nx - 6 + q;   // 0, 1, 2, nx-3, nx-2, nx-1
+        const int src = KIND == 0 ? (q < 3 ? nx - 6 + q : q)           // nx-6 .. nx-4, 3 .. 5
+                                  : (q < 3 ? 5 - q : nx - 1 - q);      // 5, 4, 3, nx-4, nx-5, nx-6
+        double cell[8];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) {
-#pragma unroll
-            for (int v = 0; v < 8; ++v) {
-                if (KIND == 1 && v == 4) continue;
-                cell[q][v] = PL(u, v)[(size_t)src[q] * g.pitch + j];
-                PL(u, v)[(size_t)dst[q] * g.pitch + j] = cell[q][v];
-            }
+        for (int v = 0; v < 8; ++v) {
+            if (KIND == 1 && v == 4) continue;
+            cell[v] = PL(u, v)[(size_t)src * pitch + j];
+            PL(u, v)[(size_t)dst * pitch + j] = cell[v];
         }
         if (KIND == 1) {
+            // bX of the ghost columns from the zero divergence of the
+            // mirrored By: columns 2, 1, 0 from columns 3, 3, 4 and columns
+            // nx-3, nx-2, nx-1 from columns nx-4, nx-5, nx-6, each added in
+            // turn
             const double *by = PL(u, 5);
             double *bx = PL(u, 4);
+            double b;
             if (j > 0) {
-                // By of ghost columns 2, 1, 0 equals that of columns 3, 4, 5
-                const double dby3 = by[(size_t)3 * g.pitch + j] - by[(size_t)3 * g.pitch + j - 1];
-                const double dby4 = by[(size_t)4 * g.pitch + j] - by[(size_t)4 * g.pitch + j - 1];
-                const double dby5 = by[(size_t)5 * g.pitch + j] - by[(size_t)5 * g.pitch + j - 1];
-                cell[2][4] = bx[(size_t)3 * g.pitch + j] + dby3 / ph.dy * ph.dx;
-                cell[1][4] = cell[2][4] + dby3 / ph.dy * ph.dx;
-                cell[0][4] = cell[1][4] + dby4 / ph.dy * ph.dx;
-                (void)dby5;
-                const double dbyA = by[(size_t)(nx - 4) * g.pitch + j] - by[(size_t)(nx - 4) * g.pitch + j - 1];
-                const double dbyB = by[(size_t)(nx - 5) * g.pitch + j] - by[(size_t)(nx - 5) * g.pitch + j - 1];
-                const double dbyC = by[(size_t)(nx - 6) * g.pitch + j] - by[(size_t)(nx - 6) * g.pitch + j - 1];
-                cell[3][4] = bx[(size_t)(nx - 4) * g.pitch + j] - dbyA / ph.dy * ph.dx;
-                cell[4][4] = cell[3][4] - dbyB / ph.dy * ph.dx;
-                cell[5][4] = cell[4][4] - dbyC / ph.dy * ph.dx;
+                if (q < 3) {
+                    const size_t c3 = (size_t)3 * pitch + j;
+                    const double d3 = by[c3] - by[c3 - 1];
+                    b = bx[c3] + d3 / ph.dy * ph.dx;                                       // column 2
+                    if (q < 2) b = b + d3 / ph.dy * ph.dx;                                 // column 1
+                    if (q < 1) b = b + (by[c3 + pitch] - by[c3 + pitch - 1]) / ph.dy * ph.dx;   // column 0
+                } else {
+                    const size_t cA = (size_t)(nx - 4) * pitch + j;
+                    b = bx[cA] - (by[cA] - by[cA - 1]) / ph.dy * ph.dx;                    // column nx-3
+                    if (q > 3) b = b - (by[cA - pitch] - by[cA - pitch - 1]) / ph.dy * ph.dx;           // nx-2
+                    if (q > 4) b = b - (by[cA - 2 * pitch] - by[cA - 2 * pitch - 1]) / ph.dy * ph.dx;   // nx-1
+                }
             } else {
-                cell[2][4] = bx[(size_t)3 * g.pitch + j];
-                cell[1][4] = cell[2][4];
-                cell[0][4] = cell[1][4];
-                cell[3][4] = bx[(size_t)(nx - 4) * g.pitch + j];
-                cell[4][4] = cell[3][4];
-                cell[5][4] = cell[4][4];
+                b = q < 3 ? bx[(size_t)3 * pitch] : bx[(size_t)(nx - 4) * pitch];
             }
-#pragma unroll
-            for (int q = 0; q < 6; ++q) bx[(size_t)dst[q] * g.pitch + j] = cell[q][4];
+            cell[4] = b;
+            bx[(size_t)dst * pitch + j] = b;
         }
         if (ctrl && j >= kGhost && j < g.ny - kGhost) {
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                const double d = cell_dt<EXACT>(cell[q][0], cell[q][1], cell[q][2], cell[q][3], cell[q][4], cell[q][5],
-                                         cell[q][6], cell[q][7], ph);
-                if (d < dloc) dloc = d;
-            }
+            const double d = cell_dt<EXACT>(cell[0], cell[1], cell[2], cell[3], cell[4], cell[5], cell[6], cell[7], ph);
+            if (d < dloc) dloc = d;
         }
     }
     if (ctrl) accumulate_dt(ctrl, slot, dloc);
 }
 
+// Wavefront w of the x boundary: ghost column w % 6, rows (w / 6) * 64 ..
+__device__ __forceinline__ int bcx_wave() { return (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)); }
+
 template <int KIND, bool EXACT>
 __global__ __launch_bounds__(256) void bc_x_kernel(double *u, Ctrl *ctrl, int slot, Grid g, Phys ph)
 {
-    bc_x_row<KIND, EXACT>(blockIdx.x * blockDim.x + threadIdx.x, u, ctrl, slot, g, ph);
+    const int w = bcx_wave();
+    bc_x_cells<KIND, EXACT>(w % 6, (w / 6) * 64 + (int)(threadIdx.x & 63), u, ctrl, slot, g, ph);
 }
 
 // Periodic y (P/boundary.cu:74-141) or symmetric y (S/boundary.cu:107-205),
@@ -644,12 +643,13 @@ __global__ __launch_bounds__(256) void bc_x_batch_kernel(const RunDesc *__restri
     const RunDesc &d = runs[blockIdx.y];
     const Grid g = d.g;
     const Phys ph = d.ph;
-    if ((int)(blockIdx.x * blockDim.x) >= g.ny) return;   // whole workgroup past this run's rows
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)(blockIdx.x * (blockDim.x >> 6) / 6) * 64 >= g.ny) return;   // whole workgroup past this run's rows
+    const int w = bcx_wave();
+    const int q = w % 6, j = (w / 6) * 64 + (int)(threadIdx.x & 63);
     double *u = stage_state ? d.us : d.u;
     Ctrl *ctrl = with_dt ? d.ctrl : nullptr;
-    if (d.bc_x == 0) bc_x_row<0, EXACT>(j, u, ctrl, slot, g, ph);
-    else bc_x_row<1, EXACT>(j, u, ctrl, slot, g, ph);
+    if (d.bc_x == 0) bc_x_cells<0, EXACT>(q, j, u, ctrl, slot, g, ph);
+    else bc_x_cells<1, EXACT>(q, j, u, ctrl, slot, g, ph);
 }
 
 template <bool EXACT>
@@ -946,7 +946,7 @@ hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const 
 hipError_t launch_bc_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int max_ny, bool stage_state,
                            bool with_dt, int slot, hipStream_t s)
 {
-    const dim3 gx((max_ny + 255) / 256, nruns), gy((max_nx + 4 * kBcyCols - 1) / (4 * kBcyCols), nruns);
+    const dim3 gx((6 * ((max_ny + 63) / 64) + 3) / 4, nruns), gy((max_nx + 4 * kBcyCols - 1) / (4 * kBcyCols), nruns);
     const int st = stage_state ? 1 : 0, wd = with_dt ? 1 : 0;
     if (exact) {
         hipLaunchKernelGGL(bc_x_batch_kernel<true>, gx, dim3(256), 0, s, runs, st, wd, slot);
@@ -1011,7 +1011,7 @@ hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *
 hipError_t launch_bc_x(int kind, bool exact, double *u, Ctrl *ctrl, int slot, const Grid &g, const Phys &ph,
                        hipStream_t s)
 {
-    const int blocks = (g.ny + 255) / 256;
+    const int blocks = (6 * ((g.ny + 63) / 64) + 3) / 4;   // one wavefront per ghost column and 64 rows
     if (kind == 0) {
         if (exact) hipLaunchKernelGGL((bc_x_kernel<0, true>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
         else hipLaunchKernelGGL((bc_x_kernel<0, false>), dim3(blocks), dim3(256), 0, s, u, ctrl, slot, g, ph);
