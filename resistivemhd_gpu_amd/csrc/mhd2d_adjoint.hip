@@ -63,6 +63,9 @@ struct AdjDesc {
     double *hists;                // segment states U* (stage-1 results with their boundary)
     double *dts;                  // dt of every step of the rollout
     double dtol;                  // degenerate star-state threshold of the face derivatives (0: exact)
+    long long aos;                // offset of this run's cells in the concatenated AoS transfer layout
+    int swj;                      // sweep form: 64-row strips per column, face rows 2 + 64 s ..
+    int tail0;                    // sweep form: first face row left to the per-face tail, 2 + 64 swj
     // tangent-linear model: direction (dU0, deta), tangent state dU, dU*,
     // tangent face fluxes dF, dG and corner EMF dE
     double *du0, *deta, *du, *dus, *dF, *dG, *dE;
@@ -146,6 +149,38 @@ __global__ __launch_bounds__(256) void copy_kernel(const AdjDesc *runs, const lo
     const size_t o = (size_t)i * d.g.pitch + j;
 #pragma unroll
     for (int v = 0; v < 8; ++v) b[v * d.g.plane + o] = a[v * d.g.plane + o];
+}
+
+// Every run's state or adjoint plane set to or from the concatenated AoS
+// transfer layout (8 doubles per cell, k = j + i*ny, runs back to back at
+// AdjDesc::aos), one launch for the whole batch.
+template <bool TO_AOS>
+__global__ __launch_bounds__(256) void aos_all_kernel(const AdjDesc *runs, const long long *first, int nruns, int which,
+                                                      double *aos)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    if (j >= d.g.ny) return;
+    double *p = buf_ptr(d, which, 0) + (size_t)i * d.g.pitch + j;
+    double *c = aos + d.aos + ((size_t)i * d.g.ny + j) * 8;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        if (TO_AOS) c[v] = p[v * d.g.plane];
+        else p[v * d.g.plane] = c[v];
+    }
+}
+
+// Every run's dL/deta plane to the concatenated layout (nx*ny per run, runs
+// back to back at AdjDesc::aos / 8), one launch for the whole batch.
+__global__ __launch_bounds__(256) void geta_all_kernel(const AdjDesc *runs, const long long *first, int nruns,
+                                                       double *dst)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    if (j >= d.g.ny) return;
+    dst[d.aos / 8 + (size_t)i * d.g.ny + j] = d.geta[(size_t)i * d.g.pitch + j];
 }
 
 // mask bits: 1 lamC, 2 geta
@@ -255,16 +290,13 @@ __device__ __forceinline__ bool non_b(int v) { return v != 4 && v != 5; }
 // stage 1, U* for stage 2); slot < 0: the live buffers u / us.  DEGEN:
 // the degenerate star-state rule is on (d.dtol > 0); without it the rule's
 // test folds away at compile time.
+// Adjoint of face (i, j) of direction DIR: its flux adjoint from the stage's
+// cell adjoints, then the flux's input adjoints added to lamC / geta with
+// global atomics.
 template <int STAGE, int DIR, bool DEGEN>
-__global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const long long *first, int nruns,
-                                                       int step, int slot)
+__device__ __forceinline__ void face_adj_at(const AdjDesc &d, int i, int j, int step, int slot)
 {
-    int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
-    const AdjDesc &d = runs[r];
     const Grid &g = d.g;
-    if (DIR == 0 ? (i < 2 || i > g.nx - 4 || j < 2 || j > g.ny - 3) : (i < 2 || i > g.nx - 3 || j < 2 || j > g.ny - 4))
-        return;
     const double dt = d.dts[step];
     const double coef = (STAGE == 2 ? 0.5 : 1.0) * dt / (DIR == 0 ? d.ph.dx : d.ph.dy);
     const double *L = STAGE == 2 ? d.lam : d.lamBar;
@@ -294,9 +326,61 @@ __global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const
     mhd_ad::flux_full_adj(S, E, DIR, i, j, par, rpar, lamF, add);
 }
 
+template <int STAGE, int DIR, bool DEGEN>
+__global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const long long *first, int nruns,
+                                                       int step, int slot)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    const Grid &g = d.g;
+    if (DIR == 0 ? (i < 2 || i > g.nx - 4 || j < 2 || j > g.ny - 3) : (i < 2 || i > g.nx - 3 || j < 2 || j > g.ny - 4))
+        return;
+    face_adj_at<STAGE, DIR, DEGEN>(d, i, j, step, slot);
+}
+
+// Face rows a run's sweep strips leave over (rows tail0 .. ny-3 of the x
+// faces, tail0 .. ny-4 of the y faces, every face column), one face per
+// lane, both directions in one launch: blocks below xblocks take x faces,
+// the rest y faces.  Per direction, run r owns lanes [tfirst[r],
+// tfirst[r+1]), column-major over the tail rows.
+__device__ __forceinline__ int tail_rows(const AdjDesc &d, int dir)
+{
+    return max(0, (dir == 0 ? d.g.ny - 2 : d.g.ny - 3) - d.tail0);
+}
+
+template <int STAGE, int DIR, bool DEGEN>
+__device__ __forceinline__ void face_adj_tail(const AdjDesc *runs, const long long *tfirst, int nruns, long long t,
+                                              int step, int slot)
+{
+    if (t >= tfirst[nruns]) return;
+    int lo = 0, hi = nruns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tfirst[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const AdjDesc &d = runs[lo];
+    const int rows = tail_rows(d, DIR);
+    const long long k = t - tfirst[lo];
+    face_adj_at<STAGE, DIR, DEGEN>(d, 2 + (int)(k / rows), d.tail0 + (int)(k % rows), step, slot);
+}
+
+template <int STAGE, bool DEGEN>
+__global__ __launch_bounds__(64) void face_adj_tail_kernel(const AdjDesc *runs, const long long *tfirst, int nruns,
+                                                            unsigned xblocks, int step, int slot)
+{
+    if (blockIdx.x < xblocks)
+        face_adj_tail<STAGE, 0, DEGEN>(runs, tfirst, nruns, (long long)blockIdx.x * 64 + threadIdx.x, step, slot);
+    else
+        face_adj_tail<STAGE, 1, DEGEN>(runs, tfirst + nruns + 1, nruns,
+                                       (long long)(blockIdx.x - xblocks) * 64 + threadIdx.x, step, slot);
+}
+
 // ------------------------------------------------ face adjoints, sweep form
-// A workgroup of two wavefronts takes one item: the 64 rows j0 .. j0+63 of
-// one run and the face columns ia .. ib-1.  Wave 0 computes the x faces and
+// A workgroup of two wavefronts takes one item: the 64 face rows j0 .. j0+63
+// (j0 = 2 + 64 s) of one run and the face columns ia .. ib-1; the few rows
+// past a run's last strip go to face_adj_tail_kernel.  Wave 0 computes the x faces and
 // wave 1 the y faces of each column, in step.  Everything a column's faces
 // read sits in LDS rings (rows j0-2 .. j0+65; state columns i-2 .. i+2,
 // adjoint and resistivity columns as the stencils need).  The columns the
@@ -390,9 +474,9 @@ __device__ __forceinline__ bool sweep_item(const AdjDesc *runs, const long long 
     }
     r = lo;
     const long long lt = t - sfirst[lo];
-    const int ntj = runs[lo].ntj;
-    j0 = (int)(lt % ntj) * 64;
-    ia = 2 + (int)(lt / ntj) * cols;
+    const int swj = runs[lo].swj;
+    j0 = 2 + (int)(lt % swj) * 64;
+    ia = 2 + (int)(lt / swj) * cols;
     ib = min(ia + cols, runs[lo].g.nx - 2);
     return true;
 }
@@ -841,6 +925,9 @@ struct mhd_adjoint {
     // sweep form of the face adjoints: item prefix per run, columns per item
     long long *d_sfirst = nullptr;
     long long sweep_items = 0;
+    // face rows the strips leave over: lane prefix per run, per direction
+    long long *d_tfirst = nullptr;
+    long long tail_faces[2] = {0, 0};
     int sweep_cols = 0;
     int face_kernel = MHD_ADJ_FACE_AUTO;
     bool face_kernel_auto_sweep = false;
@@ -896,17 +983,40 @@ dim3 tiles_grid1(const mhd_adjoint *a) { return dim3((unsigned)a->total_tiles); 
 // columns each item sweeps plus its start-up (about three columns' worth:
 // loading five state columns, flushing four accumulator columns), so the
 // last round is not left half empty.
+// Face rows up to this many past a run's last full 64-row strip go to the
+// per-face tail kernel instead of a mostly idle extra strip (x faces span
+// ny - 4 rows, 64k + 2 for a 64k-row interior).
+constexpr int kSwTailRows = 16;
+
 int layout_sweep(mhd_adjoint *a)
 {
+    for (AdjDesc &d : a->desc) {
+        const int rows = d.g.ny - 4;   // x-face rows 2 .. ny-3 (the y faces stop a row earlier)
+        d.swj = rows / 64 + (rows % 64 > kSwTailRows ? 1 : 0);
+        d.tail0 = 2 + 64 * d.swj;
+    }
+    std::vector<long long> tfirst(2 * (a->nruns + 1), 0);
+    for (int r = 0; r < a->nruns; ++r) {
+        const AdjDesc &d = a->desc[r];
+        const long long tx = (long long)(d.g.nx - 5) * std::max(0, d.g.ny - 2 - d.tail0);
+        const long long ty = (long long)(d.g.nx - 4) * std::max(0, d.g.ny - 3 - d.tail0);
+        tfirst[r + 1] = tfirst[r] + tx;
+        tfirst[a->nruns + 1 + r + 1] = tfirst[a->nruns + 1 + r] + ty;
+    }
+    a->tail_faces[0] = tfirst[a->nruns];
+    a->tail_faces[1] = tfirst[2 * a->nruns + 1];
+    AHIP(a, hipMalloc(&a->d_tfirst, sizeof(long long) * tfirst.size()));
+    AHIP(a, hipMemcpyAsync(a->d_tfirst, tfirst.data(), sizeof(long long) * tfirst.size(), hipMemcpyHostToDevice,
+                           stream_of(a)));
     long long strips_cols = 0;
-    for (const AdjDesc &d : a->desc) strips_cols += (long long)d.ntj * (d.g.nx - 4);
+    for (const AdjDesc &d : a->desc) strips_cols += (long long)d.swj * (d.g.nx - 4);
     const long long slots = 2LL * std::max(1, a->fwd->num_cu);
     const long long target = 16 * slots;
     double best_cost = 1e300;
     a->sweep_cols = 8;
     for (int cols = 8; cols <= 64; ++cols) {
         long long items = 0;
-        for (const AdjDesc &d : a->desc) items += (long long)d.ntj * ((d.g.nx - 4 + cols - 1) / cols);
+        for (const AdjDesc &d : a->desc) items += (long long)d.swj * ((d.g.nx - 4 + cols - 1) / cols);
         const double cost = (double)((items + slots - 1) / slots) * (cols + 3);
         if (cost < best_cost - 1e-9) {
             best_cost = cost;
@@ -916,7 +1026,7 @@ int layout_sweep(mhd_adjoint *a)
     std::vector<long long> first(a->nruns + 1, 0);
     for (int r = 0; r < a->nruns; ++r) {
         const AdjDesc &d = a->desc[r];
-        first[r + 1] = first[r] + (long long)d.ntj * ((d.g.nx - 4 + a->sweep_cols - 1) / a->sweep_cols);
+        first[r + 1] = first[r] + (long long)d.swj * ((d.g.nx - 4 + a->sweep_cols - 1) / a->sweep_cols);
     }
     a->sweep_items = first[a->nruns];
     AHIP(a, hipMalloc(&a->d_sfirst, sizeof(long long) * (a->nruns + 1)));
@@ -938,8 +1048,13 @@ template <int STAGE, bool DEGEN>
 void launch_face_adj_t(const mhd_adjoint *a, int step, int slot, hipStream_t s)
 {
     if (use_sweep(a)) {
-        hipLaunchKernelGGL((face_adj_sweep_kernel<STAGE, DEGEN>), dim3((unsigned)a->sweep_items), dim3(128), 0, s,
-                           a->d_desc, a->d_sfirst, a->nruns, a->sweep_cols, step, slot);
+        if (a->sweep_items)
+            hipLaunchKernelGGL((face_adj_sweep_kernel<STAGE, DEGEN>), dim3((unsigned)a->sweep_items), dim3(128), 0, s,
+                               a->d_desc, a->d_sfirst, a->nruns, a->sweep_cols, step, slot);
+        const unsigned xb = (unsigned)((a->tail_faces[0] + 63) / 64), yb = (unsigned)((a->tail_faces[1] + 63) / 64);
+        if (xb + yb)
+            hipLaunchKernelGGL((face_adj_tail_kernel<STAGE, DEGEN>), dim3(xb + yb), dim3(64), 0, s, a->d_desc,
+                               a->d_tfirst, a->nruns, xb, step, slot);
     } else {
         const dim3 tg1 = tiles_grid1(a), blk(64);
         hipLaunchKernelGGL((face_adj_kernel<STAGE, 0, DEGEN>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step,
@@ -1173,6 +1288,7 @@ int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out)
         d.u = rd.u;
         d.us = rd.us;
         first[r + 1] = first[r] + (long long)rd.g.nx * d.ntj;
+        d.aos = r ? a->desc[r - 1].aos + 8LL * a->desc[r - 1].g.nx * a->desc[r - 1].g.ny : 0;
         total += (size_t)(3 * 8 + 2 + 8) * rd.g.plane;   // lam, lamBar, lamC; lamE, geta; u0
     }
     a->total_tiles = first[nruns];
@@ -1214,6 +1330,7 @@ int mhd_adjoint_destroy(mhd_adjoint *a)
     (void)hipFree(a->d_desc);
     (void)hipFree(a->d_first);
     (void)hipFree(a->d_sfirst);
+    (void)hipFree(a->d_tfirst);
     (void)hipFree(a->hist);
     (void)hipFree(a->hists);
     (void)hipFree(a->d_slot_desc);
@@ -1557,10 +1674,9 @@ static int upload_all(mhd_adjoint *a, const double *src, int which, bool device)
                                stream_of(a)));
         stage = a->stage_all;
     }
-    for (int r = 0; r < a->nruns; ++r) {
-        const AdjDesc &d = a->desc[r];
-        AHIP(a, mhd::launch_unpack(stage + a->aos_off[r], which == kU0 ? d.u0 : d.lam, d.g, stream_of(a)));
-    }
+    hipLaunchKernelGGL(aos_all_kernel<false>, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->d_first,
+                       a->nruns, which, const_cast<double *>(stage));
+    AHIP(a, hipGetLastError());
     AHIP(a, hipStreamSynchronize(stream_of(a)));
     return MHD_OK;
 }
@@ -1572,10 +1688,9 @@ static int download_all(mhd_adjoint *a, double *dst, int which, bool device)
     int rc;
     if ((rc = ensure_stage_all(a))) return rc;
     double *stage = device ? dst : a->stage_all;
-    for (int r = 0; r < a->nruns; ++r) {
-        const AdjDesc &d = a->desc[r];
-        AHIP(a, mhd::launch_pack(which == kU ? d.u : d.lam, stage + a->aos_off[r], d.g, stream_of(a)));
-    }
+    hipLaunchKernelGGL(aos_all_kernel<true>, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->d_first,
+                       a->nruns, which, stage);
+    AHIP(a, hipGetLastError());
     if (!device)
         AHIP(a, hipMemcpyAsync(dst, a->stage_all, sizeof(double) * a->aos_off[a->nruns], hipMemcpyDeviceToHost,
                                stream_of(a)));
@@ -1603,12 +1718,9 @@ static int grad_eta_all(mhd_adjoint *a, double *dst, bool device)
     int rc;
     if ((rc = ensure_stage_all(a))) return rc;
     double *stage = device ? dst : a->stage_all;
-    size_t off = 0;
-    for (int r = 0; r < a->nruns; ++r) {
-        const AdjDesc &d = a->desc[r];
-        hipLaunchKernelGGL(pack_plane_kernel, dim3(1024), dim3(256), 0, stream_of(a), d.geta, stage + off, d.g);
-        off += (size_t)d.g.nx * d.g.ny;
-    }
+    const size_t off = a->aos_off[a->nruns] / 8;
+    hipLaunchKernelGGL(geta_all_kernel, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->d_first, a->nruns,
+                       stage);
     AHIP(a, hipGetLastError());
     if (!device)
         AHIP(a, hipMemcpyAsync(dst, a->stage_all, sizeof(double) * off, hipMemcpyDeviceToHost, stream_of(a)));

@@ -349,9 +349,14 @@ def test_whole_batch_transfers_host_and_device():
 def test_face_kernel_forms_agree():
     """The sweep form of the face adjoints (LDS rings, accumulation before
     HBM) gives the per-face form's gradients on a batch of uneven runs, with
-    and without a resistivity plane (sums of the same terms in other orders)."""
-    cases = [_ot(24), _ot(70, eta=3e-3), _harris(80, 50)]
-    cases = [(c, _generic(U, 60 + k, 0.005 if k == 2 else 0.05)) for k, (c, U) in enumerate(cases)]
+    and without a resistivity plane (sums of the same terms in other orders).
+    The runs cover the ways the sweep splits a column's face rows: one
+    partly filled strip (24, 50 rows), a full strip plus 8 rows for the tail
+    kernel (70), a full strip plus the 2 x-face rows and 1 y-face row a
+    64k-row interior leaves (64), and the tail kernel alone (12)."""
+    cases = [_ot(24), _ot(70, eta=3e-3), _harris(80, 50), _ot(12), _harris(40, 64)]
+    harris = [False, False, True, False, True]
+    cases = [(c, _generic(U, 60 + k, 0.005 if harris[k] else 0.05)) for k, (c, U) in enumerate(cases)]
     print("case face-kernel-forms: runs", [(c.nx, c.ny) for c, _ in cases], flush=True)
     out = {}
     for face in ("per_face", "sweep"):
@@ -369,7 +374,7 @@ def test_face_kernel_forms_agree():
     for r in range(len(cases)):
         # the noisy Harris sheet amplifies summation-order differences (see
         # test_adjoint_matches_tangent_linear); Orszag-Tang does not
-        rtol, atol = (1e-9, 1e-12) if r < 2 else (1e-6, 1e-9)
+        rtol, atol = (1e-6, 1e-9) if harris[r] else (1e-9, 1e-12)
         for k in range(2):
             ref, got = out["per_face"][r][k], out["sweep"][r][k]
             assert np.all(np.isfinite(got))
