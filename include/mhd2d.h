@@ -258,6 +258,19 @@ int mhd_batch_set_graph(mhd_batch *b, int on);
  * over the work list (a persistent launch). */
 int mhd_batch_set_waves_per_cu(mhd_batch *b, int waves_per_cu);
 int mhd_batch_get_waves_per_cu(const mhd_batch *b, int *waves_per_cu);
+/* Packed last strips (no counterpart in the reference).  A wavefront
+ * updates 60 interior rows of one column segment; a run's last strip of at
+ * most 28 rows can instead be packed, with 64 / (rows + 4) segments side by
+ * side in one wavefront, swept by a launch of its own after the full
+ * strips'.  AUTO (the default) packs when the full strips alone would take
+ * more than two rounds of resident wavefronts; results are bitwise the same
+ * in every mode.  mhd_batch_get_packed_items: packed items in the current
+ * work list (0: none). */
+#define MHD_STRIP_PACK_AUTO 0
+#define MHD_STRIP_PACK_ALWAYS 1
+#define MHD_STRIP_PACK_NEVER 2
+int mhd_batch_set_strip_packing(mhd_batch *b, int mode);
+int mhd_batch_get_packed_items(const mhd_batch *b, int *items);
 /* Autotuner: times 2*steps batch steps for each candidate sweep length on
  * the batch's current states, then for each launch shape (one wavefront
  * per item, or half or all of the resident wavefront slots per CU) at the

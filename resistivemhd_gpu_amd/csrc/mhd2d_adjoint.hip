@@ -1193,7 +1193,7 @@ int history_steps(mhd_adjoint *a, int n, bool last_to_live)
     for (int k = 0; k < n; ++k) {
         const mhd::RunDesc *A = slot_desc(a, k, 0);
         AHIP(a, mhd::launch_stage_batch(1, b->exact, A, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
-                                        b->blocks_per_xcd, b->cur, s));
+                                        b->blocks_per_xcd, b->cur, s, b->d_tail_run, b->tail_items));
         AHIP(a, mhd::launch_bc_batch(b->exact, A, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
         // stage 2 reads U^n and U* from slot k and writes U^{n+1} into slot
         // k+1 (or the live state); the boundary fill follows it there
@@ -1207,7 +1207,7 @@ int history_steps(mhd_adjoint *a, int n, bool last_to_live)
         }
         if (B) {
             AHIP(a, mhd::launch_stage_batch(2, b->exact, B, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
-                                            b->blocks_per_xcd, b->cur, s));
+                                            b->blocks_per_xcd, b->cur, s, b->d_tail_run, b->tail_items));
             AHIP(a, mhd::launch_bc_batch(b->exact, after, b->nruns, b->max_nx, b->max_ny, false, true, b->cur ^ 1,
                                          s));
             b->cur ^= 1;
@@ -1613,7 +1613,7 @@ int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps)
         hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 1);
         hipLaunchKernelGGL(bc_tlm_y_kernel, gy, blk, 0, s, a->d_desc, 1);
         AHIP(a, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
-                                        b->items_per_xcd, b->blocks_per_xcd, b->cur, s));
+                                        b->items_per_xcd, b->blocks_per_xcd, b->cur, s, b->d_tail_run, b->tail_items));
         AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
         // tangents of stage 2 from (U*, dU*) (B keeps the stage-1 CT value)
         hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
@@ -1624,7 +1624,7 @@ int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps)
         AHIP(a, hipGetLastError());
         // primal stage 2 and its boundary: the state the next step starts from
         AHIP(a, mhd::launch_stage_batch(2, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
-                                        b->items_per_xcd, b->blocks_per_xcd, b->cur, s));
+                                        b->items_per_xcd, b->blocks_per_xcd, b->cur, s, b->d_tail_run, b->tail_items));
         AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, false, true, b->cur ^ 1, s));
         b->cur ^= 1;
     }

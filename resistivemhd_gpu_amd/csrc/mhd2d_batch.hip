@@ -48,7 +48,6 @@ int bfail(mhd_batch *b, int code, const std::string &msg)
         if (e_ != hipSuccess) return bfail(b, MHD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-constexpr int kStrip = mhd::kStripRows;
 constexpr int kG = mhd::kGhost;
 
 int plane_pitch(int ny) { return (ny + 7) & ~7; }
@@ -70,19 +69,48 @@ void set_launch_shape(mhd_batch *b)
                                               : std::max(1, std::min(b->items_per_xcd, cu_per_xcd * b->waves_per_cu));
 }
 
-// Work list for sweep length L: items of run r are [first_item, first_item +
-// nstrips * nsegs), strips fastest as in the single-run kernel.
+// Work lists for sweep length L: run r's full-strip items are [first_item,
+// first_item + nstrips * nsegs), strips fastest as in the single-run kernel;
+// its packed last strip's items are [first_tail, first_tail + tail_items) of
+// the second list.  Last strips are packed only when the full-strip launch
+// would take several rounds of resident wavefronts: packing saves lanes,
+// which shortens a launch only by whole rounds, and costs a launch of its
+// own.
 int build_work_list(mhd_batch *b, int L)
 {
-    std::vector<int> item_run;
+    const long long slots = (long long)(b->num_cu > 0 ? b->num_cu : 256) * mhd::stage_batch_waves_per_cu(b->exact);
+    long long unpacked = 0;
+    for (int r = 0; r < b->nruns; ++r) {
+        Grid g = b->desc[r].g;
+        const int nx_int = g.nx - 2 * kG;
+        mhd::set_strips(g, g.ny - 2 * kG, false);
+        mhd::set_segs(g, nx_int, std::max(1, std::min(L, nx_int)));
+        unpacked += mhd::grid_items(g);
+    }
+    const bool pack = b->strip_packing == MHD_STRIP_PACK_ALWAYS ||
+                      (b->strip_packing == MHD_STRIP_PACK_AUTO && unpacked > 2 * slots);
+    std::vector<int> item_run, tail_run;
     for (int r = 0; r < b->nruns; ++r) {
         Grid &g = b->desc[r].g;
         const int nx_int = g.nx - 2 * kG;
-        g.seg = std::max(1, std::min(L, nx_int));
-        g.nsegs = (nx_int + g.seg - 1) / g.seg;
+        mhd::set_strips(g, g.ny - 2 * kG, pack);
+        mhd::set_segs(g, nx_int, std::max(1, std::min(L, nx_int)));
         b->desc[r].first_item = (int)item_run.size();
         item_run.insert(item_run.end(), (size_t)g.nstrips * g.nsegs, r);
+        b->desc[r].first_tail = (int)tail_run.size();
+        tail_run.insert(tail_run.end(), (size_t)g.tail_items, r);
     }
+    b->tail_items = (int)tail_run.size();
+    if (b->tail_items > b->tail_cap) {
+        BHIP(b, hipStreamSynchronize(b->stream));
+        BHIP(b, hipFree(b->d_tail_run));
+        b->d_tail_run = nullptr;
+        BHIP(b, hipMalloc(&b->d_tail_run, sizeof(int) * (size_t)b->tail_items));
+        b->tail_cap = b->tail_items;
+    }
+    if (b->tail_items)
+        BHIP(b, hipMemcpyAsync(b->d_tail_run, tail_run.data(), sizeof(int) * tail_run.size(), hipMemcpyHostToDevice,
+                               b->stream));
     b->seg = L;
     b->total_items = (int)item_run.size();
     b->xcds = b->num_cu >= 64 ? b->num_cu / 32 : 1;
@@ -114,9 +142,11 @@ int default_seg(const mhd_batch *b)
     for (int L = 8; L <= 512; L += 4) {
         long long items = 0;
         for (int r = 0; r < b->nruns; ++r) {
-            const Grid &g = b->desc[r].g;
+            Grid g = b->desc[r].g;
             const int nx_int = g.nx - 2 * kG;
-            items += (long long)g.nstrips * ((nx_int + L - 1) / L);
+            mhd::set_strips(g, g.ny - 2 * kG, false);
+            mhd::set_segs(g, nx_int, std::max(1, std::min(L, nx_int)));
+            items += mhd::grid_items(g);
         }
         const long long rounds = (items + slots - 1) / slots;
         const double cost = (double)rounds * (std::min(L, b->max_nx) + 4);
@@ -145,10 +175,10 @@ int refresh_dt(mhd_batch *b) { return mhd_batch_refresh_dt(b); }
 int enqueue_step(mhd_batch *b, int cur)
 {
     BHIP(b, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
-                                    b->items_per_xcd, b->blocks_per_xcd, cur, b->stream));
+                                    b->items_per_xcd, b->blocks_per_xcd, cur, b->stream, b->d_tail_run, b->tail_items));
     BHIP(b, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, b->stream));
     BHIP(b, mhd::launch_stage_batch(2, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
-                                    b->items_per_xcd, b->blocks_per_xcd, cur, b->stream));
+                                    b->items_per_xcd, b->blocks_per_xcd, cur, b->stream, b->d_tail_run, b->tail_items));
     BHIP(b, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, false, true, cur ^ 1,
                                  b->stream));
     return MHD_OK;
@@ -262,7 +292,7 @@ int mhd_batch_create(const mhd_config *cfgs, int nruns, mhd_batch **out)
         d.g.ny = c.ny;
         d.g.pitch = plane_pitch(c.ny);
         d.g.plane = plane_size(c.nx, c.ny);
-        d.g.nstrips = (c.ny - 2 * kG + kStrip - 1) / kStrip;
+        mhd::set_strips(d.g, c.ny - 2 * kG, false);   // build_work_list decides on packing
         d.g.xcds = 1;
         d.ph.gamma = c.gamma;
         d.ph.dx = c.dx;
@@ -304,6 +334,7 @@ int mhd_batch_destroy(mhd_batch *b)
     (void)hipFree(b->ctrl);
     (void)hipFree(b->d_desc);
     (void)hipFree(b->d_item_run);
+    (void)hipFree(b->d_tail_run);
     (void)hipFree(b->aos);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -638,6 +669,23 @@ int mhd_batch_get_waves_per_cu(const mhd_batch *b, int *waves)
 {
     if (!b || !waves) return bfail(nullptr, MHD_ERR_INVALID, "NULL argument");
     *waves = b->waves_per_cu;
+    return MHD_OK;
+}
+
+int mhd_batch_set_strip_packing(mhd_batch *b, int mode)
+{
+    if (!b) return bfail(nullptr, MHD_ERR_INVALID, "NULL batch");
+    if (mode != MHD_STRIP_PACK_AUTO && mode != MHD_STRIP_PACK_ALWAYS && mode != MHD_STRIP_PACK_NEVER)
+        return bfail(b, MHD_ERR_INVALID, "mode must be MHD_STRIP_PACK_AUTO, _ALWAYS or _NEVER");
+    BHIP(b, hipStreamSynchronize(b->stream));
+    b->strip_packing = mode;
+    return build_work_list(b, b->seg);
+}
+
+int mhd_batch_get_packed_items(const mhd_batch *b, int *items)
+{
+    if (!b || !items) return bfail(nullptr, MHD_ERR_INVALID, "NULL argument");
+    *items = b->tail_items;
     return MHD_OK;
 }
 

@@ -16,6 +16,9 @@ struct mhd_batch {
     mhd::RunDesc *d_desc = nullptr;
     int *d_item_run = nullptr;
     int item_cap = 0;
+    int *d_tail_run = nullptr;   // packed last strips: item -> run
+    int tail_cap = 0, tail_items = 0;
+    int strip_packing = MHD_STRIP_PACK_AUTO;
     double *pool = nullptr;             // U, U* and the side row of every run
     mhd::Ctrl *ctrl = nullptr;               // one control block per run
     double *aos = nullptr;              // staging for host transfers (largest run)

@@ -9,18 +9,53 @@ namespace mhd {
 constexpr int kGhost = 3;        // ghost layers per side
 constexpr int kStripRows = 60;   // interior rows a wavefront updates (64 lanes - 4 halo rows)
 
+// A last strip of at most this many interior rows is packed: one wavefront
+// sweeps 64 / (rows + 4) segments of it side by side, each in its own lane
+// group, instead of one segment in a mostly idle strip.
+constexpr int kPackedStripRows = 28;
+
 struct Grid {
     int nx, ny;          // cells per axis including ghosts
     int pitch;           // doubles between consecutive x rows of a plane
     long long plane;     // doubles between planes
     int seg;             // columns one wavefront sweeps
-    int nstrips, nsegs;  // work decomposition: strips along y, segments along x
+    int nstrips, nsegs;  // work decomposition: full strips along y (not the packed one), segments along x
+    int tail_g;          // packed last strip: lanes per segment group (its rows + 4 halo rows), 0 if none
+    int tail_items;      // packed last strip: wavefronts, 64 / tail_g segments each
     int xcds;            // L2 domains workgroups are dealt to round-robin
-    int items_per_xcd;   // ceil(nstrips * nsegs / xcds)
+    int items_per_xcd;   // ceil(items / xcds)
 };
 
+// Strips of ny_int interior rows; pack: whether a short last strip may be
+// packed (single-GPU scheme only).
+inline void set_strips(Grid &g, int ny_int, bool pack)
+{
+    const int n = (ny_int + kStripRows - 1) / kStripRows;
+    const int last = ny_int - (n - 1) * kStripRows;
+    g.nstrips = n;
+    g.tail_g = 0;
+    if (pack && last <= kPackedStripRows) {
+        g.nstrips = n - 1;
+        g.tail_g = last + 4;
+    }
+}
+
+// Segments of seg columns over nx_int interior columns.
+inline void set_segs(Grid &g, int nx_int, int seg)
+{
+    g.seg = seg;
+    g.nsegs = (nx_int + seg - 1) / seg;
+    const int per = g.tail_g ? 64 / g.tail_g : 1;
+    g.tail_items = g.tail_g ? (g.nsegs + per - 1) / per : 0;
+}
+
+// Work items (wavefronts) of one stage over the grid: full strips, then the
+// packed last strip.
+inline int grid_items(const Grid &g) { return g.nstrips * g.nsegs + g.tail_items; }
+
 // One run of a batch: its grid, constants, buffers and control block, and
-// the index of its first item in the batch's work list.
+// the index of its first item in the batch's work list (full strips) and in
+// its packed-strip list.
 struct RunDesc {
     Grid g;
     Phys ph;
@@ -29,7 +64,7 @@ struct RunDesc {
     Ctrl *ctrl;
     int bc_x, bc_y;
     int first_item;
-    int pad;
+    int first_tail;
 };
 
 // exact = true: operation-for-operation reference arithmetic (built without
@@ -61,7 +96,8 @@ int stage_waves_per_cu(bool exact);
 // boundary pair of every run's U* (stage_state) or U, optionally reducing
 // the ghost cells' dt into slot.
 hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const int *item_run, int total_items,
-                              int xcds, int items_per_xcd, int blocks_per_xcd, int cur, hipStream_t s);
+                              int xcds, int items_per_xcd, int blocks_per_xcd, int cur, hipStream_t s,
+                              const int *tail_run, int tail_items);
 hipError_t launch_bc_batch(bool exact, const RunDesc *runs, int nruns, int max_nx, int max_ny, bool stage_state,
                            bool with_dt, int slot, hipStream_t s);
 int stage_batch_waves_per_cu(bool exact);

@@ -87,24 +87,48 @@ __device__ __forceinline__ Cell make_cell(double rho, double mx, double my, doub
 //
 // stage_item is one work item (strip sy, segment sx) of one run; the
 // single-run kernel and the batched work-list kernel both call it.
-template <int STAGE, bool EXACT, int SCHEME>
+//
+// TAIL: item of the packed last strip (tail_g = its rows + 4).  Lane
+// q * tail_g + r holds row j0 - 2 + r of segment item * (64 / tail_g) + q:
+// the loop runs over group 0's columns c, and group q's lanes work on
+// column c + dcol, dcol = q * seg (loads clamped to the grid, stores only
+// within the group's own segment).  Each group keeps two halo rows on each
+// side, so the y-neighbour shifts that cross a group boundary only reach
+// rows whose results are discarded, as at the edges of a full strip.
+template <int STAGE, bool EXACT, int SCHEME, bool TAIL = false>
 __device__ __forceinline__ void stage_item(int item, const double *__restrict__ uin, double *uout, const double *uold,
                                            const double *__restrict__ etaf, double *__restrict__ side,
                                            double *__restrict__ aux, Ctrl *__restrict__ ctrl, const Grid &g,
                                            const Phys &ph, int cur)
 {
     constexpr bool MULTI = SCHEME == 1;
+    static_assert(!(TAIL && MULTI), "packed strips belong to the single-GPU scheme");
     const int lane = threadIdx.x;
-    const int sy = item % g.nstrips;
-    const int sx = item / g.nstrips;
+    int sy, sx, row = lane, dcol = 0;
+    bool group_ok = true;
+    if (TAIL) {
+        const int per = 64 / g.tail_g, q = lane / g.tail_g;
+        sy = g.nstrips;
+        sx = item * per;
+        group_ok = q < per && sx + q < g.nsegs;
+        row = q < per ? lane - q * g.tail_g : g.tail_g - 1;
+        dcol = group_ok ? q * g.seg : 0;
+    } else {
+        sy = item % g.nstrips;
+        sx = item / g.nstrips;
+    }
     const int j0 = kGhost + sy * kStripRows;
     const int i0 = kGhost + sx * g.seg;
-    const int iend = min(i0 + g.seg, g.nx - kGhost);
+    // loop end (group 0's segment, in full when TAIL) and, in loop columns,
+    // the end of this lane's own segment
+    const int iend = TAIL ? i0 + g.seg : min(i0 + g.seg, g.nx - kGhost);
+    const int send = TAIL ? (group_ok ? min(i0 + dcol + g.seg, g.nx - kGhost) - dcol : 0) : iend;
     const int jend = min(j0 + kStripRows, g.ny - kGhost);
-    const int jr = j0 - 2 + lane;              // row held by this lane
+    const int jr = j0 - 2 + row;               // row held by this lane
     const int jl = min(jr, g.ny - 1);          // clamped row for loads (rows past ny-1 are unused)
     const bool upd_row = (jr >= j0) && (jr < jend);
     const bool stale_row = (sy == 0) && (jr == kGhost - 1);
+    const bool leader = lane == 0 && item == 0 && (!TAIL || g.nstrips == 0);
 
     // dt: ResistiveMHD2D::calculateDt (P/resistiveMHD_2D.cu:277-291), from
     // the per-cell minimum reduced at the end of the previous step.
@@ -119,7 +143,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
         }
         const long long seq = ctrl->steps - ctrl->seq0;
         if (ctrl->dt_in) dt = ctrl->dt_in[seq];
-        if (item == 0 && lane == 0) {
+        if (leader) {
             if (ctrl->dt_out) ctrl->dt_out[seq] = dt;
             ctrl->dt = dt;
             ctrl->acc[cur ^ 1] = kInfBits;
@@ -128,7 +152,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
         }
     } else {
         dt = ctrl->dt;
-        if (item == 0 && lane == 0) {
+        if (leader) {
             ctrl->time += dt;
             ctrl->steps += 1;
         }
@@ -142,7 +166,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     const double *Rh = PL(uin, 0), *Mx = PL(uin, 1), *My = PL(uin, 2), *Mz = PL(uin, 3);
     const double *Bx = PL(uin, 4), *By = PL(uin, 5), *Bz = PL(uin, 6), *En = PL(uin, 7);
 
-    double bx_prev = Bx[(size_t)(i0 - 3) * g.pitch + jl];
+    double bx_prev = Bx[(size_t)(i0 - 3 + dcol) * g.pitch + jl];
     Cell A{}, B{};                                   // cells c-2 and c-1
     Face7 Lx{};                                      // left state of face c-2+1/2
     Flux8 Fprev{}, Gprev{};                          // F(c-3+1/2), G(c-2)
@@ -155,7 +179,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     double eta_prev = 0.0;   // resistivity of cell c-1
 
     for (int c = i0 - 2; c <= iend + 1; ++c) {
-        const size_t o = (size_t)c * g.pitch + jl;
+        const size_t o = (size_t)(TAIL ? min(c + dcol, g.nx - 1) : c) * g.pitch + jl;
         const double rho = Rh[o], mx = Mx[o], my = My[o], mz = Mz[o];
         const double bx = Bx[o], by = By[o], bz = Bz[o], e = En[o];
         const double bym = By[o - 1];
@@ -360,8 +384,8 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                         PL(aux, 2)[os] = Gprev.f0;
                         PL(aux, 3)[os] = Gprev.f4;
                     }
-                } else if (s < iend && (upd_row || stale_row)) {
-                    const size_t os = (size_t)s * g.pitch + jl;
+                } else if (s < send && (upd_row || stale_row)) {
+                    const size_t os = (size_t)(s + dcol) * g.pitch + jl;
                     double n0, n1, n2, n3, n4, n5, n6, n7;
                     if (STAGE == 1) {
                         // oneStepFirstFunctor (P/resistiveMHD_2D.cu:53-86) and
@@ -384,7 +408,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                             // symmetric-x boundary before the y boundary
                             // overwrites it (S/boundary.cu:56-75).
                             PL(uout, 5)[os] = n5;
-                            side[s] = n5;
+                            side[s + dcol] = n5;
                         }
                     } else {
                         if (upd_row) {
@@ -410,7 +434,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                             if (d < dtloc) dtloc = d;
                             nanflag |= isnan(n7) ? 1 : 0;
                         } else {
-                            PL(uout, 5)[os] = side[s];
+                            PL(uout, 5)[os] = side[s + dcol];
                         }
                     }
                 }
@@ -475,13 +499,51 @@ __device__ __forceinline__ void batch_item(const RunDesc *__restrict__ runs, con
                                 ph, cur);
 }
 
+// The packed last strips of a batch (runs whose grid has tail_g): a launch
+// of its own after the full strips' launch, one wavefront per packed item,
+// so the full-strip kernel keeps its register allocation.  tail_run maps a
+// packed item to its run; the run's packed items start at first_tail.
+template <int STAGE, bool EXACT>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
+void rk_stage_tail_batch_kernel(const RunDesc *__restrict__ runs, const int *__restrict__ tail_run, int total,
+                                int cur)
+{
+    const int item = (int)blockIdx.x;
+    if (item < total) tail_item<STAGE, EXACT>(runs, tail_run, item, cur);
+}
+
+template <int STAGE, bool EXACT>
+__device__ __forceinline__ void tail_item(const RunDesc *__restrict__ runs, const int *__restrict__ tail_run, int item,
+                                          int cur)
+{
+    const RunDesc &d = runs[tail_run[item]];
+    const Grid g = d.g;
+    const Phys ph = d.ph;
+    if (STAGE == 1)
+        stage_item<1, EXACT, 0, true>(item - d.first_tail, d.u, d.us, d.u, d.eta, d.side, nullptr, d.ctrl, g, ph,
+                                      cur);
+    else
+        stage_item<2, EXACT, 0, true>(item - d.first_tail, d.us, d.uo ? d.uo : d.u, d.u, d.eta, d.side, nullptr,
+                                      d.ctrl, g, ph, cur);
+}
+
 template <int STAGE, bool EXACT, bool PERSIST>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(EXACT ? MHD_WAVES_PER_EU_EXACT : MHD_WAVES_PER_EU_FAST)))
 void rk_stage_batch_kernel(const RunDesc *__restrict__ runs, const int *__restrict__ item_run, int total_items,
-                           int xcds, int items_per_xcd, int blocks_per_xcd, int cur)
+                           int xcds, int items_per_xcd, int blocks_per_xcd, int cur, const int *__restrict__ tail_run,
+                           int tail_items)
 {
     if (!PERSIST) {
+        // the packed items follow the full strips' blocks, so they fill the
+        // launch's last round
+        const int main_blocks = xcds * items_per_xcd;
+        if ((int)blockIdx.x >= main_blocks) {
+            if ((int)blockIdx.x - main_blocks < tail_items)
+                tail_item<STAGE, EXACT>(runs, tail_run, (int)blockIdx.x - main_blocks, cur);
+            return;
+        }
         const int item = xcd_item(xcds, items_per_xcd);
         if (item < total_items) batch_item<STAGE, EXACT>(runs, item_run, item, cur);
         return;
@@ -902,9 +964,10 @@ static int grid_for(long long n, int block)
 }
 
 hipError_t launch_stage(int stage, bool exact, int scheme, const double *uin, double *uout, const double *uold,
-                        const double *eta, double *side, double *aux, Ctrl *ctrl, const Grid &g, const Phys &ph,
+                        const double *eta, double *side, double *aux, Ctrl *ctrl, const Grid &grid, const Phys &ph,
                         int cur, hipStream_t s)
 {
+    const Grid &g = grid;
     const int blocks = g.xcds * g.items_per_xcd;
 #define MHD_LAUNCH_STAGE(ST, EX, SC)                                                                            \
     hipLaunchKernelGGL((rk_stage_kernel<ST, EX, SC>), dim3(blocks), dim3(64), 0, s, uin, uout, uold, eta, side, \
@@ -922,17 +985,20 @@ hipError_t launch_stage(int stage, bool exact, int scheme, const double *uin, do
 }
 
 hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const int *item_run, int total_items,
-                              int xcds, int items_per_xcd, int blocks_per_xcd, int cur, hipStream_t s)
+                              int xcds, int items_per_xcd, int blocks_per_xcd, int cur, hipStream_t s,
+                              const int *tail_run, int tail_items)
 {
-    const int blocks = xcds * blocks_per_xcd;
+    const bool persist = blocks_per_xcd < items_per_xcd;
+    const int blocks = xcds * blocks_per_xcd + (persist ? 0 : tail_items);
 #define MHD_LAUNCH_BATCH(ST, EX)                                                                                 \
     do {                                                                                                         \
-        if (blocks_per_xcd < items_per_xcd)                                                                      \
+        if (persist)                                                                                             \
             hipLaunchKernelGGL((rk_stage_batch_kernel<ST, EX, true>), dim3(blocks), dim3(64), 0, s, runs, item_run, \
-                               total_items, xcds, items_per_xcd, blocks_per_xcd, cur);                           \
+                               total_items, xcds, items_per_xcd, blocks_per_xcd, cur, tail_run, tail_items);     \
         else                                                                                                     \
             hipLaunchKernelGGL((rk_stage_batch_kernel<ST, EX, false>), dim3(blocks), dim3(64), 0, s, runs,        \
-                               item_run, total_items, xcds, items_per_xcd, blocks_per_xcd, cur);                 \
+                               item_run, total_items, xcds, items_per_xcd, blocks_per_xcd, cur, tail_run,        \
+                               tail_items);                                                                      \
     } while (0)
     if (stage == 1) {
         if (exact) MHD_LAUNCH_BATCH(1, true); else MHD_LAUNCH_BATCH(1, false);
@@ -940,6 +1006,17 @@ hipError_t launch_stage_batch(int stage, bool exact, const RunDesc *runs, const 
         if (exact) MHD_LAUNCH_BATCH(2, true); else MHD_LAUNCH_BATCH(2, false);
     }
 #undef MHD_LAUNCH_BATCH
+    if (persist && tail_items > 0) {
+#define MHD_LAUNCH_TAIL(ST, EX)                                                                            \
+    hipLaunchKernelGGL((rk_stage_tail_batch_kernel<ST, EX>), dim3(tail_items), dim3(64), 0, s, runs, tail_run, \
+                       tail_items, cur)
+        if (stage == 1) {
+            if (exact) MHD_LAUNCH_TAIL(1, true); else MHD_LAUNCH_TAIL(1, false);
+        } else {
+            if (exact) MHD_LAUNCH_TAIL(2, true); else MHD_LAUNCH_TAIL(2, false);
+        }
+#undef MHD_LAUNCH_TAIL
+    }
     return hipGetLastError();
 }
 

@@ -35,7 +35,8 @@ EXPORTS = (
     "mhd_batch_init_harris", "mhd_batch_step", "mhd_batch_synchronize", "mhd_batch_get_dt", "mhd_batch_get_time",
     "mhd_batch_check_crashed", "mhd_batch_set_numerics", "mhd_batch_set_sweep_length",
     "mhd_batch_get_sweep_length", "mhd_batch_set_graph", "mhd_batch_autotune", "mhd_batch_set_waves_per_cu",
-    "mhd_batch_get_waves_per_cu", "mhd_batch_save", "mhd_batch_load",
+    "mhd_batch_get_waves_per_cu", "mhd_batch_save", "mhd_batch_load", "mhd_batch_set_strip_packing",
+    "mhd_batch_get_packed_items",
     "mhd_adjoint_create", "mhd_adjoint_destroy", "mhd_adjoint_last_error", "mhd_adjoint_set_eta_field",
     "mhd_adjoint_set_state", "mhd_adjoint_set_dts", "mhd_adjoint_get_dts", "mhd_adjoint_forward",
     "mhd_adjoint_get_state", "mhd_adjoint_set_seed", "mhd_adjoint_backward", "mhd_adjoint_get_grad_state",
@@ -155,6 +156,8 @@ def load_library(path=LIB_PATH):
         "mhd_batch_autotune": (I, [P, I, ctypes.POINTER(I)]),
         "mhd_batch_set_waves_per_cu": (I, [P, I]),
         "mhd_batch_get_waves_per_cu": (I, [P, ctypes.POINTER(I)]),
+        "mhd_batch_set_strip_packing": (I, [P, I]),
+        "mhd_batch_get_packed_items": (I, [P, ctypes.POINTER(I)]),
         "mhd_adjoint_create": (I, [C, I, ctypes.POINTER(P)]),
         "mhd_adjoint_destroy": (I, [P]),
         "mhd_adjoint_last_error": (ctypes.c_char_p, [P]),
@@ -581,6 +584,19 @@ class ResistiveMHD2DBatch:
     def waves_per_cu(self):
         v = ctypes.c_int()
         self._ok(self._lib.mhd_batch_get_waves_per_cu(self._h, ctypes.byref(v)))
+        return v.value
+
+    STRIP_PACK_AUTO, STRIP_PACK_ALWAYS, STRIP_PACK_NEVER = 0, 1, 2
+
+    def set_strip_packing(self, mode):
+        """Packing of short last strips (several column segments per
+        wavefront): STRIP_PACK_AUTO (when the launch takes several rounds),
+        _ALWAYS or _NEVER; results are bitwise the same."""
+        self._ok(self._lib.mhd_batch_set_strip_packing(self._h, int(mode)))
+
+    def packed_items(self):
+        v = ctypes.c_int()
+        self._ok(self._lib.mhd_batch_get_packed_items(self._h, ctypes.byref(v)))
         return v.value
 
     def autotune(self, steps=2):

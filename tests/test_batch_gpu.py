@@ -131,6 +131,55 @@ def test_batch_persistent_launch_shapes_agree(numerics):
         assert np.array_equal(ref.getU(r), b.getU(r)), r
 
 
+def _packing_runs():
+    """Last strips of every packable height class: 4 rows (16 segments per
+    wavefront), 16, 28 (two per wavefront), a run that is all last strip
+    (20 rows, symmetric x, whose bottom stale face goes through the packed
+    path), and one that is not packable (40 rows)."""
+    runs = []
+    c = mhd.orszag_tang_config(64, eta=1e-3)                 # 60 + 4
+    runs.append((c, boundary_applied(c, mhd.fill_orszag_tang(c)), None))
+    c = mhd.orszag_tang_config(136, eta=3e-4)                # 120 + 16
+    c.bc_y = mhd.MHD_BC_SYMMETRIC
+    runs.append((c, boundary_applied(c, mhd.fill_orszag_tang(c)), None))
+    c = mhd.harris_config(150, 88)                           # 60 + 28
+    eta = mhd.fill_harris_eta(c)
+    runs.append((c, boundary_applied(c, mhd.fill_harris(c, seed=4, noise_amp=1e-2)), eta))
+    c = mhd.harris_config(97, 20, eta_max=1e-2)              # 20 rows only
+    c.bc_x = mhd.MHD_BC_SYMMETRIC
+    runs.append((c, boundary_applied(c, mhd.fill_harris(c, seed=5, noise_amp=1e-2)), None))
+    c = mhd.orszag_tang_config(100, eta=5e-3)                # 60 + 40: not packed
+    runs.append((c, boundary_applied(c, mhd.fill_orszag_tang(c)), None))
+    return runs
+
+
+@pytest.mark.parametrize("numerics", ["exact", "fast"])
+def test_batch_packed_strips_match_single_handles(numerics):
+    """Packed last strips (several column segments per wavefront, a launch
+    of their own) end bitwise where single handles end, with segments that
+    do and do not fill a packed wavefront and a last segment shorter than
+    the rest; AUTO packs the batched workload's shape and leaves a small
+    batch unpacked."""
+    runs = _packing_runs()
+    steps = 12
+    print(f"case batch-packed numerics={numerics} runs={len(runs)} steps={steps}", flush=True)
+    for L in (7, 16):
+        b = _batch(runs, numerics)
+        b.set_sweep_length(L)
+        b.set_strip_packing(b.STRIP_PACK_ALWAYS)
+        assert b.packed_items() > 0
+        b.steps(steps)
+        for r, (c, U0, eta) in enumerate(runs):
+            s = _single(c, U0, eta, steps, numerics)
+            assert np.array_equal(b.getU(r), s.getU()), (L, r)
+            assert b.dt(r) == s.dt and b.totalTime(r) == s.totalTime, (L, r)
+    b = _batch(runs, numerics)
+    b.set_strip_packing(b.STRIP_PACK_NEVER)
+    assert b.packed_items() == 0
+    b.set_strip_packing(b.STRIP_PACK_AUTO)
+    assert b.packed_items() == 0       # a few hundred items: one round
+
+
 def test_batch_init_and_boundary():
     cfgs = [mhd.orszag_tang_config(48), mhd.harris_config(64, 32)]
     b = mhd.ResistiveMHD2DBatch(cfgs)
