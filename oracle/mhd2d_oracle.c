@@ -1,8 +1,8 @@
 /*
  * mhd2d_oracle.c -- CPU restatement of the reference's single-GPU FP64
  * resistive-MHD RK2 step.  TEST INFRASTRUCTURE ONLY: this file is the
- * checker that tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
- * leg compare the HIP library against.  It is never linked into, called by
+ * reference result that tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg compare the HIP library against.  It is never linked into, called by
  * or shipped as part of the product library.
  *
  * It follows the reference's pass structure literally: array-of-structs
