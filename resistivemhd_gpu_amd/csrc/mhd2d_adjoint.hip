@@ -199,85 +199,109 @@ __global__ __launch_bounds__(256) void zero_kernel(const AdjDesc *runs, const lo
 }
 
 // Transpose of the y boundary fill (periodic: P/boundary.cu:74-141;
-// symmetric: S/boundary.cu:107-205), one thread per column; blockIdx.y = run.
+// symmetric: S/boundary.cu:107-205); blockIdx.y = run.  One ghost cell per
+// lane, as the forward fill: lanes 6c .. 6c+5 of a wavefront take column
+// col0 + c and ghost rows 0, 1, 2, ny-3, ny-2, ny-1 (ten columns per
+// wavefront), each adding its ghost adjoint into the cell it was copied
+// from and clearing it.  Source and ghost rows are disjoint (ny >= 12), so
+// lanes never touch each other's cells except through the atomics on Bx.
+// Symmetric y: the By rebuild chain of a column's bottom (top) ghosts is
+// transposed by its q = 0 (q = 3) lane.
+constexpr int kBctCols = 10;   // columns per wavefront
+
 __global__ __launch_bounds__(256) void bct_y_kernel(const AdjDesc *runs, int which)
 {
     const AdjDesc &d = runs[blockIdx.y];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.g.nx) return;
+    const int lane = threadIdx.x & 63;
+    const int i = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kBctCols + lane / 6;
+    const int q = lane % 6;
+    if (lane >= 6 * kBctCols || i >= d.g.nx) return;
     double *L = buf_ptr(d, which, 0);
     const long long pl = d.g.plane;
     const int ny = d.g.ny;
     const size_t row = (size_t)i * d.g.pitch;
-    const int dst[6] = {0, 1, 2, ny - 3, ny - 2, ny - 1};
+    const int dst = q < 3 ? q : ny - 6 + q;   // 0, 1, 2, ny-3, ny-2, ny-1
     if (d.bc_y == MHD_BC_PERIODIC) {
-        const int src[6] = {ny - 6, ny - 5, ny - 4, 3, 4, 5};
-        for (int v = 0; v < 8; ++v)
-            for (int q = 0; q < 6; ++q) {
-                L[v * pl + row + src[q]] += L[v * pl + row + dst[q]];
-                L[v * pl + row + dst[q]] = 0.0;
-            }
+        const int src = q < 3 ? ny - 6 + q : q;   // ny-6 .. ny-4, 3 .. 5
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            L[v * pl + row + src] += L[v * pl + row + dst];
+            L[v * pl + row + dst] = 0.0;
+        }
         return;
     }
-    const int src[6] = {5, 4, 3, ny - 4, ny - 5, ny - 6};
+    const int src = q < 3 ? 5 - q : ny - 1 - q;   // 5, 4, 3, ny-4, ny-5, ny-6
+#pragma unroll
     for (int v = 0; v < 8; ++v) {
         if (v == 5) continue;
-        for (int q = 0; q < 6; ++q) {
-            const double b = L[v * pl + row + dst[q]];
-            L[v * pl + row + dst[q]] = 0.0;
-            if (v == 4) unsafeAtomicAdd(&L[4 * pl + row + src[q]], b);   // other columns add to Bx too
-            else L[v * pl + row + src[q]] += b;
-        }
+        const double b = L[v * pl + row + dst];
+        L[v * pl + row + dst] = 0.0;
+        if (v == 4) unsafeAtomicAdd(&L[4 * pl + row + src], b);   // the By chains add to Bx too
+        else L[v * pl + row + src] += b;
     }
+    if (q != 0 && q != 3) return;
     double *by = L + 5 * pl + row;
+    double *bx = L + 4 * pl;
+    const size_t prev = (size_t)(i - 1) * d.g.pitch;
     const double c = d.ph.dy / d.ph.dx;
-    double lb0 = by[0], lb1 = by[1], lb2 = by[2];
-    double lt0 = by[ny - 3], lt1 = by[ny - 2], lt2 = by[ny - 1];
-    by[0] = by[1] = by[2] = 0.0;
-    by[ny - 3] = by[ny - 2] = by[ny - 1] = 0.0;
-    // bottom: by2 = by3 + c d3, by1 = by2 + c d3, by0 = by1 + c d4
-    lb1 += lb0;
-    const double ld4 = c * lb0;
-    lb2 += lb1;
-    double ld3 = c * lb1 + c * lb2;
-    by[3] += lb2;
-    // top: by(ny-3) = by(ny-4) - c dA, by(ny-2) = .. - c dB, by(ny-1) = .. - c dC
-    lt1 += lt2;
-    const double ldC = -c * lt2;
-    lt0 += lt1;
-    const double ldB = -c * lt1;
-    by[ny - 4] += lt0;
-    const double ldA = -c * lt0;
-    if (i > 0) {
-        // d_r = Bx(i, r) - Bx(i-1, r)
-        double *bx = L + 4 * pl;
-        const size_t prev = (size_t)(i - 1) * d.g.pitch;
-        const int rows[5] = {3, 4, ny - 4, ny - 5, ny - 6};
-        const double lds[5] = {ld3, ld4, ldA, ldB, ldC};
-        for (int q = 0; q < 5; ++q) {
-            unsafeAtomicAdd(&bx[row + rows[q]], lds[q]);
-            unsafeAtomicAdd(&bx[prev + rows[q]], -lds[q]);
+    if (q == 0) {
+        // bottom: by2 = by3 + c d3, by1 = by2 + c d3, by0 = by1 + c d4
+        double lb0 = by[0], lb1 = by[1], lb2 = by[2];
+        by[0] = by[1] = by[2] = 0.0;
+        lb1 += lb0;
+        const double ld4 = c * lb0;
+        lb2 += lb1;
+        const double ld3 = c * lb1 + c * lb2;
+        by[3] += lb2;
+        if (i > 0) {
+            // d_r = Bx(i, r) - Bx(i-1, r)
+            unsafeAtomicAdd(&bx[row + 3], ld3);
+            unsafeAtomicAdd(&bx[prev + 3], -ld3);
+            unsafeAtomicAdd(&bx[row + 4], ld4);
+            unsafeAtomicAdd(&bx[prev + 4], -ld4);
+        }
+    } else {
+        // top: by(ny-3) = by(ny-4) - c dA, by(ny-2) = .. - c dB, by(ny-1) = .. - c dC
+        double lt0 = by[ny - 3], lt1 = by[ny - 2], lt2 = by[ny - 1];
+        by[ny - 3] = by[ny - 2] = by[ny - 1] = 0.0;
+        lt1 += lt2;
+        const double ldC = -c * lt2;
+        lt0 += lt1;
+        const double ldB = -c * lt1;
+        by[ny - 4] += lt0;
+        const double ldA = -c * lt0;
+        if (i > 0) {
+            const int rows[3] = {ny - 4, ny - 5, ny - 6};
+            const double lds[3] = {ldA, ldB, ldC};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                unsafeAtomicAdd(&bx[row + rows[k]], lds[k]);
+                unsafeAtomicAdd(&bx[prev + rows[k]], -lds[k]);
+            }
         }
     }
 }
 
-// Transpose of the periodic x fill (P/boundary.cu:4-71), one thread per row.
+// Transpose of the periodic x fill (P/boundary.cu:4-71); blockIdx.y = run.
+// One ghost cell per lane, as the forward fill: wavefront w takes ghost
+// column w % 6 (0, 1, 2, nx-3, nx-2, nx-1) and rows (w / 6) * 64 .. + 63.
 __global__ __launch_bounds__(256) void bct_x_kernel(const AdjDesc *runs, int which)
 {
     const AdjDesc &d = runs[blockIdx.y];
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    const int q = w % 6, j = (w / 6) * 64 + (int)(threadIdx.x & 63);
     if (j >= d.g.ny) return;
     double *L = buf_ptr(d, which, 0);
     const long long pl = d.g.plane;
     const int nx = d.g.nx;
     const size_t P = d.g.pitch;
-    const int dst[6] = {0, 1, 2, nx - 3, nx - 2, nx - 1};
-    const int src[6] = {nx - 6, nx - 5, nx - 4, 3, 4, 5};
-    for (int v = 0; v < 8; ++v)
-        for (int q = 0; q < 6; ++q) {
-            L[v * pl + src[q] * P + j] += L[v * pl + dst[q] * P + j];
-            L[v * pl + dst[q] * P + j] = 0.0;
-        }
+    const size_t dst = (size_t)(q < 3 ? q : nx - 6 + q) * P + j;   // columns 0, 1, 2, nx-3, nx-2, nx-1
+    const size_t src = (size_t)(q < 3 ? nx - 6 + q : q) * P + j;   // columns nx-6 .. nx-4, 3 .. 5
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+        L[v * pl + src] += L[v * pl + dst];
+        L[v * pl + dst] = 0.0;
+    }
 }
 
 __device__ __forceinline__ bool non_b(int v) { return v != 4 && v != 5; }
@@ -1091,10 +1115,10 @@ int copy_all(mhd_adjoint *a, int src, int src_slot, int dst, int dst_slot)
 int bc_transpose(mhd_adjoint *a, int which)
 {
     const mhd_batch *b = a->fwd;
-    hipLaunchKernelGGL(bct_y_kernel, dim3((b->max_nx + 255) / 256, a->nruns), dim3(256), 0, stream_of(a), a->d_desc,
-                       which);
-    hipLaunchKernelGGL(bct_x_kernel, dim3((b->max_ny + 255) / 256, a->nruns), dim3(256), 0, stream_of(a), a->d_desc,
-                       which);
+    hipLaunchKernelGGL(bct_y_kernel, dim3((b->max_nx + 4 * kBctCols - 1) / (4 * kBctCols), a->nruns), dim3(256), 0,
+                       stream_of(a), a->d_desc, which);
+    hipLaunchKernelGGL(bct_x_kernel, dim3((6 * ((b->max_ny + 63) / 64) + 3) / 4, a->nruns), dim3(256), 0,
+                       stream_of(a), a->d_desc, which);
     AHIP(a, hipGetLastError());
     return MHD_OK;
 }
