@@ -163,16 +163,19 @@ def test_batch_packed_strips_match_single_handles(numerics):
     runs = _packing_runs()
     steps = 12
     print(f"case batch-packed numerics={numerics} runs={len(runs)} steps={steps}", flush=True)
-    for L in (7, 16):
+    singles = [_single(c, U0, eta, steps, numerics) for c, U0, eta in runs]
+    # (sweep length, persistent waves per CU): a persistent launch runs the
+    # packed items as a launch of their own
+    for L, w in ((7, 0), (16, 0), (7, 3)):
         b = _batch(runs, numerics)
         b.set_sweep_length(L)
+        b.set_waves_per_cu(w)
         b.set_strip_packing(b.STRIP_PACK_ALWAYS)
         assert b.packed_items() > 0
         b.steps(steps)
-        for r, (c, U0, eta) in enumerate(runs):
-            s = _single(c, U0, eta, steps, numerics)
-            assert np.array_equal(b.getU(r), s.getU()), (L, r)
-            assert b.dt(r) == s.dt and b.totalTime(r) == s.totalTime, (L, r)
+        for r, s in enumerate(singles):
+            assert np.array_equal(b.getU(r), s.getU()), (L, w, r)
+            assert b.dt(r) == s.dt and b.totalTime(r) == s.totalTime, (L, w, r)
     b = _batch(runs, numerics)
     b.set_strip_packing(b.STRIP_PACK_NEVER)
     assert b.packed_items() == 0
