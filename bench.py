@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
 """Throughput of the plain FP64 resistive-MHD RK2 step on one MI355X.
 
-Workload (default, --workload harris): Harris current sheet, 4096 x 2048
-interior cells on [-100, 100] x [-50, 50], periodic x, symmetric y,
-Bx = tanh y, rho = sech^2 y + 0.2, p = rho/2, per-cell resistivity
-1e-3 + 5e-3 exp(-(x^2 + y^2)/4), seeded white-noise vy of amplitude 1e-3.
+Workload (default, --workload harris): Harris current sheet, 4096 x 4096
+interior cells on [-100, 100]^2 (the cell size of the 4096 x 2048 case on
+[-100, 100] x [-50, 50], which the default run also times), periodic x,
+symmetric y, Bx = tanh y, rho = sech^2 y + 0.2, p = rho/2, per-cell
+resistivity 1e-3 + 5e-3 exp(-(x^2 + y^2)/4), seeded white-noise vy of
+amplitude 1e-3.
 --workload turbulence: one GPU's subdomain of the MPI library's step,
 8192 x 8192 decaying turbulence (rho = p = 1, eta 1e-4, random-phase modes
 |k| <= 8 with rms v = rms B = 0.5), periodic halo filled on the device.
@@ -49,11 +51,11 @@ def multi_extra_bytes(zones):
     return zones * (2 * (40 + 8) + (64 + 16 + 8) + (16 + 8 + 48 + 16))
 
 
-def measured_traffic(root, numerics, kernel):
+def measured_traffic(root, nx, ny, numerics, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
     1 GiB device copy), or None if there is none."""
-    path = os.path.join(root, "profiles", f"rk_stage_kernel_harris4096x2048_{numerics}_hbm_counters.md")
+    path = os.path.join(root, "profiles", f"rk_stage_kernel_harris{nx}x{ny}_{numerics}_hbm_counters.md")
     if not os.path.exists(path):
         return None
     want = f"traffic_bytes_per_launch {kernel} ="
@@ -74,7 +76,7 @@ def cpu_baseline(nx, ny, steps, workload="harris"):
         o.apply_halo()
         what = "MPI library's step on one periodic subdomain, turbulence"
     else:
-        cfg = mhd.harris_config(nx, ny)
+        cfg = mhd.harris_config(nx, ny, ly=harris_ly(nx, ny))
         o = OracleSim.from_config(cfg)
         o.set_state(mhd.fill_harris(cfg, seed=0, noise_amp=1e-3))
         o.set_eta_field(mhd.fill_harris_eta(cfg))
@@ -368,6 +370,89 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0, degenerate_tol=0.0, fd_h
     }
 
 
+def harris_ly(nx, ny):
+    """y extent of the Harris-sheet workloads: the cell size of the 4096 x
+    2048 case on [-100, 100] x [-50, 50] (dx = dy = 200 / 4096), so the
+    4096^2 grid covers [-100, 100]^2."""
+    return 200.0 * ny / nx
+
+
+def plain_bench(mhd, torch, root, workload, nx, ny, steps, warmup, numerics, sweep):
+    """zone-cycles/s of the plain step on one workload, with the roofline of
+    its dominant stage kernel (HIP events on the library's stream)."""
+    multi = workload == "turbulence"
+    if multi:
+        cfg = mhd.default_config(nx, ny, 0.0, 2.0 * 3.141592653589793, 0.0, 2.0 * 3.141592653589793, eta=1e-4)
+    else:
+        cfg = mhd.harris_config(nx, ny, ly=harris_ly(nx, ny))
+    sim = mhd.ResistiveMHD2D(cfg)
+    sim.set_numerics(numerics)
+    if sweep:
+        sim.set_sweep_length(sweep)
+    if multi:
+        sim.set_scheme("multi")
+        sim.initTurbulence(seed=0, kmax=8, rms_v=0.5, rms_b=0.5)
+        name = f"turbulence_{nx}x{ny}_one_subdomain_of_multi_gpu_step_periodic_halo"
+    else:
+        sim.initHarris(seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3)
+        name = f"harris_sheet_{nx}x{ny}_periodic_x_symmetric_y_eta_field"
+    zones = nx * ny
+
+    sim.steps(warmup)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    sim.profile_enable(True)
+    t0 = time.perf_counter()
+    sim.steps(steps)
+    sim.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms1, ms2, nt = sim.profile_read()
+    sim.profile_enable(False)
+    crashed = sim.checkCalculationIsCrashed()
+    sweep_cols = sim.sweep_length()
+    del sim
+
+    seconds = t1 - t0
+    value = zones * steps / seconds
+    b1, b2 = stage_bytes(zones, not multi, multi)
+    avg1, avg2 = ms1 / nt / 1e3, ms2 / nt / 1e3
+    # dominant kernel = the stage with the larger summed time
+    exact = "true" if numerics == "exact" else "false"
+    scheme = 1 if multi else 0
+    if avg2 >= avg1:
+        kname, kbytes, kavg = f"rk_stage_kernel<2, {exact}, {scheme}>", b2, avg2
+    else:
+        kname, kbytes, kavg = f"rk_stage_kernel<1, {exact}, {scheme}>", b1, avg1
+    traffic = None if multi else measured_traffic(root, nx, ny, numerics, kname)
+    achieved = kbytes / kavg / 1e9
+    return {
+        "value": value,
+        "ms_per_step": seconds / steps * 1e3,
+        "config": {
+            "workload": name,
+            "nx": nx, "ny": ny, "ghost": 3, "sweep_columns": sweep_cols,
+            "numerics": numerics,
+            "parallelism": "single-gpu",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": kname,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": kbytes,
+            "avg_launch_ms": kavg * 1e3,
+            "stage_ms": [ms1 / nt, ms2 / nt],
+        },
+        "step_hbm_frac": value * (b1 + b2 + (multi_extra_bytes(zones) if multi else 0)) / zones
+        / (HBM_PEAK_GBS * 1e9),
+        "crashed": crashed,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -400,6 +485,7 @@ def main():
     ap.add_argument("--cpu-ny", type=int, default=512)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 4096 x 2048 Harris line of the default run")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -442,89 +528,33 @@ def main():
     if multi:
         args.nx = args.nx or 8192
         args.ny = args.ny or 8192
-        cfg = mhd.default_config(args.nx, args.ny, 0.0, 2.0 * 3.141592653589793, 0.0, 2.0 * 3.141592653589793,
-                                 eta=1e-4)
     else:
         args.nx = args.nx or 4096
-        args.ny = args.ny or 2048
-        cfg = mhd.harris_config(args.nx, args.ny)
-    sim = mhd.ResistiveMHD2D(cfg)
-    sim.set_numerics(args.numerics)
-    if args.sweep:
-        sim.set_sweep_length(args.sweep)
-    if multi:
-        sim.set_scheme("multi")
-        sim.initTurbulence(seed=0, kmax=8, rms_v=0.5, rms_b=0.5)
-        workload = f"turbulence_{args.nx}x{args.ny}_one_subdomain_of_multi_gpu_step_periodic_halo"
-    else:
-        sim.initHarris(seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3)
-        workload = f"harris_sheet_{args.nx}x{args.ny}_periodic_x_symmetric_y_eta_field"
-    zones = args.nx * args.ny
-
-    sim.steps(args.warmup)
-    sim.synchronize()
-    torch.cuda.synchronize()
-    sim.profile_enable(True)
-    t0 = time.perf_counter()
-    sim.steps(args.steps)
-    sim.synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms1, ms2, nt = sim.profile_read()
-    sim.profile_enable(False)
-    crashed = sim.checkCalculationIsCrashed()
-
-    seconds = t1 - t0
-    value = zones * args.steps / seconds
-    b1, b2 = stage_bytes(zones, not multi, multi)
-    avg1, avg2 = ms1 / nt / 1e3, ms2 / nt / 1e3
-    # dominant kernel = the stage with the larger summed time
-    exact = "true" if args.numerics == "exact" else "false"
-    scheme = 1 if multi else 0
-    if avg2 >= avg1:
-        kname, kbytes, kavg = f"rk_stage_kernel<2, {exact}, {scheme}>", b2, avg2
-    else:
-        kname, kbytes, kavg = f"rk_stage_kernel<1, {exact}, {scheme}>", b1, avg1
-    traffic = None
-    if not multi and (args.nx, args.ny) == (4096, 2048):
-        traffic = measured_traffic(root, args.numerics, kname)
-    achieved = kbytes / kavg / 1e9
+        args.ny = args.ny or 4096
+    res = plain_bench(mhd, torch, root, args.workload, args.nx, args.ny, args.steps, args.warmup, args.numerics,
+                      args.sweep)
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": res["value"],
         "unit": "zone-cycles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": seconds / args.steps * 1e3,
+        "ms_per_step": res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": workload,
-            "nx": args.nx, "ny": args.ny, "ghost": 3, "sweep_columns": sim.sweep_length(),
-            "numerics": args.numerics,
-            "parallelism": "single-gpu",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kname,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": kbytes,
-            "avg_launch_ms": kavg * 1e3,
-            "stage_ms": [ms1 / nt, ms2 / nt],
-        },
-        "step_hbm_frac": value * (b1 + b2 + (multi_extra_bytes(zones) if multi else 0)) / zones
-        / (HBM_PEAK_GBS * 1e9),
-        "crashed": crashed,
+        "config": res["config"],
+        "roofline": res["roofline"],
+        "step_hbm_frac": res["step_hbm_frac"],
+        "crashed": res["crashed"],
     }
-    del sim
+    if not multi and (args.nx, args.ny) == (4096, 4096) and not args.no_extra:
+        # the headline run of the configured Harris case, 4096 x 2048 cells
+        line["harris_4096x2048"] = plain_bench(mhd, torch, root, "harris", 4096, 2048, args.steps, args.warmup,
+                                               args.numerics, args.sweep)
     if args.calibrate_copy_mib:
         n = args.calibrate_copy_mib * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")

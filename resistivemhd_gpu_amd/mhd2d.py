@@ -11,9 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# MHD2D_LIBRARY points at another build of the same library (launch-shape
-# experiments); by default the in-tree build is used.
-LIB_PATH = os.environ.get("MHD2D_LIBRARY") or os.path.join(HERE, "libmhd2d.so")
+LIB_PATH = os.path.join(HERE, "libmhd2d.so")
 
 MHD_BC_PERIODIC = 0
 MHD_BC_SYMMETRIC = 1
@@ -226,10 +224,11 @@ def orszag_tang_config(n=256, eta=1e-3):
     return default_config(n, n, 0.0, 2.0 * np.pi, 0.0, 2.0 * np.pi, eta=eta)
 
 
-def harris_config(nx=4096, ny=2048, eta_max=6e-3):
-    """Harris sheet on [-100, 100] x [-50, 50], periodic x, symmetric y.
-    eta_max is the dt diffusion limit (the peak of the resistivity field)."""
-    return default_config(nx, ny, -100.0, 200.0, -50.0, 100.0, eta=eta_max,
+def harris_config(nx=4096, ny=2048, eta_max=6e-3, ly=100.0):
+    """Harris sheet on [-100, 100] x [-ly/2, ly/2] (default [-50, 50]),
+    periodic x, symmetric y.  eta_max is the dt diffusion limit (the peak of
+    the resistivity field)."""
+    return default_config(nx, ny, -100.0, 200.0, -0.5 * ly, ly, eta=eta_max,
                           bc_x=MHD_BC_PERIODIC, bc_y=MHD_BC_SYMMETRIC)
 
 
