@@ -59,6 +59,9 @@ struct oracle_sim {
     /* CT members, P/ct.hpp:9-12 */
     Flux *oldFluxF, *oldFluxG;
     double *eZVector;
+    /* face B of UBar right after the stage-1 CT (a snapshot for the tests;
+     * the reference keeps no copy) */
+    double *bXStage1, *bYStage1;
     /* per-cell resistivity sampled at cell (i, j): plays the role of the
      * example's getEta(i*dx, j*dy) (EX/plasmoid/main_plasmoid.cu:159-172) */
     double *etaField;
@@ -710,6 +713,7 @@ static void one_step_rk2(struct oracle_sim *s)
     memcpy(s->oldFluxF, s->fluxF, sizeof(Flux) * N);     /* CT::setOldFlux2D, P/ct.cu:12-19 */
     memcpy(s->oldFluxG, s->fluxG, sizeof(Flux) * N);
     div_b_clean(s, s->bXOld, s->bYOld, UBar);
+    for (int k = 0; k < N; ++k) { s->bXStage1[k] = UBar[k].bX; s->bYStage1[k] = UBar[k].bY; }   /* test snapshot */
 
     apply_bc(s, UBar);
 
@@ -1023,6 +1027,7 @@ struct oracle_sim *oracle_create(int nx, int ny, double dx, double dy, double ga
     s->tmpUForFluxG = calloc(N, sizeof(Cons));
     s->oldFluxF = calloc(N, sizeof(Flux)); s->oldFluxG = calloc(N, sizeof(Flux));
     s->eZVector = calloc(N, sizeof(double));
+    s->bXStage1 = calloc(N, sizeof(double)); s->bYStage1 = calloc(N, sizeof(double));
     s->oNF5 = calloc(N, sizeof(double)); s->oNG4 = calloc(N, sizeof(double)); s->oF5 = calloc(N, sizeof(double));
     s->oG4 = calloc(N, sizeof(double)); s->oNF0 = calloc(N, sizeof(double)); s->oNG0 = calloc(N, sizeof(double));
     s->nNF5 = calloc(N, sizeof(double)); s->nNG4 = calloc(N, sizeof(double)); s->nF5 = calloc(N, sizeof(double));
@@ -1047,7 +1052,7 @@ void oracle_destroy(struct oracle_sim *s)
     free(s->U); free(s->UBar); free(s->fluxF); free(s->fluxG); free(s->dtVector); free(s->bXOld);
     free(s->bYOld); free(s->tmpVector); free(s->solverFlux); free(s->hlldFlux); free(s->dQCenter);
     free(s->dQLeft); free(s->dQRight); free(s->tmpUForFluxG); free(s->oldFluxF); free(s->oldFluxG);
-    free(s->eZVector); free(s->etaField);
+    free(s->eZVector); free(s->etaField); free(s->bXStage1); free(s->bYStage1);
     free(s->oNF5); free(s->oNG4); free(s->oF5); free(s->oG4); free(s->oNF0); free(s->oNG0);
     free(s->nNF5); free(s->nNG4); free(s->nF5); free(s->nG4); free(s->nNF0); free(s->nNG0);
     free(s);
@@ -1055,6 +1060,27 @@ void oracle_destroy(struct oracle_sim *s)
 
 void oracle_set_state(struct oracle_sim *s, const double *aos) { memcpy(s->U, aos, sizeof(Cons) * s->N); }
 void oracle_get_state(const struct oracle_sim *s, double *aos) { memcpy(aos, s->U, sizeof(Cons) * s->N); }
+
+/* Face B (bX, bY planes of nx*ny) of UBar right after the stage-1 CT of the
+ * last single-GPU step (P/resistiveMHD_2D.cu:167-168).  Pins Q7: the second
+ * CT (:190) reuses the stage-1 EMF and the old face B, so the interior face
+ * B of U^{n+1} must equal this bit for bit. */
+void oracle_get_stage1_face_b(const struct oracle_sim *s, double *bx, double *by)
+{
+    memcpy(bx, s->bXStage1, sizeof(double) * s->N);
+    memcpy(by, s->bYStage1, sizeof(double) * s->N);
+}
+
+/* The CT-contact corner EMF of one flux set (m_emf) on a 2 x 2 patch of
+ * cells (i, j), (i, j+1), (i+1, j), (i+1, j+1), each array in that order:
+ * face fluxes NF5 (x-face F.f5), NG4 (y-face G.f4), NF0, NG0 (mass fluxes)
+ * and the cell terms F5, G4 of M/ct.cu:26-65.  Test hook for the upwind
+ * pin against Gardiner & Stone (2005). */
+double oracle_ct_contact_emf(const double *NF5, const double *NG4, const double *F5, const double *G4,
+                             const double *NF0, const double *NG0)
+{
+    return m_emf(0, 2, NF5, NG4, F5, G4, NF0, NG0);
+}
 
 int oracle_set_eta_field(struct oracle_sim *s, const double *eta)
 {

@@ -16,6 +16,9 @@ struct oracle_sim *oracle_create(int nx, int ny, double dx, double dy, double ga
 void oracle_destroy(struct oracle_sim *s);
 void oracle_set_state(struct oracle_sim *s, const double *aos);
 void oracle_get_state(const struct oracle_sim *s, double *aos);
+void oracle_get_stage1_face_b(const struct oracle_sim *s, double *bx, double *by);
+double oracle_ct_contact_emf(const double *NF5, const double *NG4, const double *F5, const double *G4,
+                             const double *NF0, const double *NG0);
 int oracle_set_eta_field(struct oracle_sim *s, const double *eta);
 void oracle_apply_boundary(struct oracle_sim *s);
 void oracle_apply_boundary_axis(struct oracle_sim *s, int axis, int kind);

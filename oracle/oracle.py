@@ -37,6 +37,8 @@ def load():
         "oracle_destroy": (None, [P]),
         "oracle_set_state": (None, [P, D]),
         "oracle_get_state": (None, [P, D]),
+        "oracle_get_stage1_face_b": (None, [P, D, D]),
+        "oracle_ct_contact_emf": (c_d, [D, D, D, D, D, D]),
         "oracle_set_eta_field": (c_i, [P, D]),
         "oracle_apply_boundary": (None, [P]),
         "oracle_apply_boundary_axis": (None, [P, c_i, c_i]),
@@ -99,6 +101,14 @@ class OracleSim:
         out = np.empty((self.nx, self.ny, 8))
         self._lib.oracle_get_state(self._s, _p(out))
         return out
+
+    def stage1_face_b(self):
+        """Face B (bX, bY) of UBar right after the stage-1 CT of the last
+        single-GPU step."""
+        bx = np.empty((self.nx, self.ny))
+        by = np.empty((self.nx, self.ny))
+        self._lib.oracle_get_stage1_face_b(self._s, _p(bx), _p(by))
+        return bx, by
 
     def set_eta_field(self, eta):
         if eta is None:
@@ -175,6 +185,13 @@ def hlld_flux(left, right, gamma=5.0 / 3.0, eps=1e-20):
     out = np.empty(8)
     load().oracle_hlld_flux(_p(L), _p(R), gamma, eps, _p(out))
     return out
+
+
+def ct_contact_emf(NF5, NG4, F5, G4, NF0, NG0):
+    """CT-contact corner EMF of one flux set on a 2 x 2 patch of cells
+    (i, j), (i, j+1), (i+1, j), (i+1, j+1) (each argument: 4 values)."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (NF5, NG4, F5, G4, NF0, NG0)]
+    return load().oracle_ct_contact_emf(*[_p(x) for x in a])
 
 
 def cell_dt(cell, gamma, dx, dy, eps=1e-20):

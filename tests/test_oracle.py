@@ -101,6 +101,55 @@ def test_hlld_mirror_symmetry():
         np.testing.assert_allclose(g, sgn * f, rtol=1e-9, atol=1e-11 * (1 + np.abs(f).max()))
 
 
+def _superfast_pairs(seed, n):
+    """Pairs (L, R) sharing Bx whose flow is faster than the fast speed on
+    both sides (u > max(cf_L, cf_R), so the unclamped HLLD speed S_L > 0)."""
+    rng = np.random.default_rng(seed)
+
+    def cf(q):
+        rho, _, _, _, bx, by, bz, p = q
+        cs2, ca2, va2 = GAMMA * p / rho, (bx * bx + by * by + bz * bz) / rho, bx * bx / rho
+        return np.sqrt(0.5 * (cs2 + ca2 + np.sqrt((cs2 + ca2) ** 2 - 4.0 * cs2 * va2)))
+
+    for _ in range(n):
+        L = np.array([rng.uniform(0.5, 2.0), 0.0, *rng.uniform(-1, 1, 2), *rng.uniform(-1, 1, 3),
+                      rng.uniform(0.2, 2.0)])
+        R = np.array([rng.uniform(0.5, 2.0), 0.0, *rng.uniform(-1, 1, 2), *rng.uniform(-1, 1, 3),
+                      rng.uniform(0.2, 2.0)])
+        R[4] = L[4]
+        c = max(cf(L), cf(R))
+        L[1], R[1] = c + rng.uniform(0.05, 2.0), c + rng.uniform(0.05, 2.0)
+        yield L, R
+
+
+def test_hlld_outer_speed_clamp():
+    """Q2, P/hlld.cu:315-318: the outer speeds are clamped, SL = min(SL, 0),
+    SR = max(SR, 0).  For a flow faster than the fast speed towards +x the
+    unclamped solver would select the outer-left region and return the
+    left state's physical flux F(L) itself.  Clamped, SL = 0 and the flux is
+    the physical flux of the left star state built with a zero wave speed:
+    equal to F(L) only up to rounding (the star state satisfies the jump
+    conditions across a wave of speed 0), and not bit for bit.  Towards -x
+    the clamp gives SR = 0 exactly, which selects the outer-right region
+    alone: the flux is F(R) bit for bit.  Mirroring x -> -x maps one case
+    onto the other, and F of a mirrored state is the mirrored F bit for bit
+    (every term only changes sign), so the mirrored leftward flux is the
+    exact F(L) of the rightward case, which the rightward flux must differ
+    from."""
+    sgn = np.array([-1, 1, -1, -1, 1, -1, -1, -1.0])
+    m = lambda q: q * np.array([1, -1, 1, 1, -1, 1, 1, 1.0])
+    differs = 0
+    n = 200
+    for L, R in _superfast_pairs(21, n):
+        f = orc.hlld_flux(L, R)                 # rightward: left star state at SL = 0
+        g = orc.hlld_flux(m(R), m(L))           # leftward: outer-right region, F(m(L))
+        ref = ideal_mhd_flux(*L)
+        np.testing.assert_allclose(f, ref, rtol=1e-12, atol=1e-13 * np.abs(ref).max())
+        np.testing.assert_allclose(g, sgn * ref, rtol=1e-12, atol=1e-13 * np.abs(ref).max())
+        differs += not np.array_equal(f, sgn * g)
+    assert differs >= 0.9 * n, differs
+
+
 # --------------------------------------------------------------------- dt
 def test_dt_uniform_closed_form():
     for q in _states(3, 50):
@@ -195,6 +244,78 @@ def test_div_b_preserved():
     o.step(10)
     d1 = _divb(o.get_state(), cfg.dx, cfg.dy)
     assert np.abs(d1 - d0).max() <= 1e-10 * np.abs(d0).max()
+
+
+def test_stage2_ct_reuses_stage1_emf():
+    """Q7, P/resistiveMHD_2D.cu:167-168,190 and P/ct.cu:12-19: CT::setOldFlux2D
+    is called once per step and CT::divBClean twice, so the second CT
+    applies the stage-1 corner EMF to the same old face field.  The
+    interior face B after the step must therefore equal the face B of the
+    stage-1 state UBar right after its CT bit for bit."""
+    cfg, o = _ot_state(40)
+    U = o.get_state()
+    rng = np.random.default_rng(8)
+    U[..., 1:4] += 1e-2 * rng.standard_normal(U[..., 1:4].shape)
+    o.set_state(U)
+    o.apply_boundary()
+    o.step(3)
+    Un = o.get_state()
+    bx, by = o.stage1_face_b()
+    inner = (slice(3, -3), slice(3, -3))
+    assert np.array_equal(Un[inner][..., 4], bx[inner])
+    assert np.array_equal(Un[inner][..., 5], by[inner])
+    # and the step did move the field (the check is not vacuous)
+    o.step(1)
+    assert not np.array_equal(o.get_state()[inner][..., 4], Un[inner][..., 4])
+
+
+def _gs05_corner_emf(Ex, Ey, Ec, mx, my):
+    """Corner EMF at (i+1/2, j+1/2) of Gardiner & Stone (2005), J. Comput.
+    Phys. 205, 509, eqs. (41), (45)-(50), written from the paper: the
+    average of the four face EMFs plus the upwinded derivatives of Ez from
+    the face to the corner.  Ex[(a, b)]: Ez at x-face (i+1/2, j+b) for a = 0;
+    Ey[(a, b)]: Ez at y-face (i+a, j+1/2); Ec[(a, b)]: Ez at cell (i+a, j+b);
+    mx[b]: mass flux through x-face (i+1/2, j+b); my[a]: through y-face
+    (i+a, j+1/2).  Grid spacings cancel: (dy/8) * 2/dy = 1/4."""
+    def up(flux, lower, upper):
+        return lower if flux > 0 else (upper if flux < 0 else 0.5 * (lower + upper))
+    # (dE/dy) at (i+1/2, j+1/4) and (i+1/2, j+3/4): from the cell on the
+    # upwind side of the x-face (i for positive mass flux, i+1 for negative)
+    dy_lo = up(mx[0], Ey[(0, 0)] - Ec[(0, 0)], Ey[(1, 0)] - Ec[(1, 0)])
+    dy_hi = up(mx[1], Ec[(0, 1)] - Ey[(0, 0)], Ec[(1, 1)] - Ey[(1, 0)])
+    # (dE/dx) at (i+1/4, j+1/2) and (i+3/4, j+1/2): upwind along y
+    dx_lo = up(my[0], Ex[(0, 0)] - Ec[(0, 0)], Ex[(0, 1)] - Ec[(0, 1)])
+    dx_hi = up(my[1], Ec[(1, 0)] - Ex[(0, 0)], Ec[(1, 1)] - Ex[(0, 1)])
+    avg = 0.25 * (Ex[(0, 0)] + Ex[(0, 1)] + Ey[(0, 0)] + Ey[(1, 0)])
+    return avg + 0.25 * (dy_lo - dy_hi) + 0.25 * (dx_lo - dx_hi)
+
+
+def test_ct_contact_emf_upwinding():
+    """The MPI library's corner EMF (M/ct.cu:147-169) is the CT-contact EMF
+    of Gardiner & Stone (2005): checked against the paper's formula on
+    random patches, with every sign combination of the face mass fluxes
+    (zero included).  Reversing the upwind direction changes the result by
+    O(1) on these patches."""
+    rng = np.random.default_rng(31)
+    cells = [(0, 0), (0, 1), (1, 0), (1, 1)]     # the oracle's patch order
+    for t in range(400):
+        Ex = {(0, 0): rng.normal(), (0, 1): rng.normal()}
+        Ey = {(0, 0): rng.normal(), (1, 0): rng.normal()}
+        Ec = {c: rng.normal() for c in cells}
+        signs = rng.integers(-1, 2, 4) if t % 4 == 0 else rng.choice([-1.0, 1.0], 4)
+        mx = [signs[0] * rng.uniform(0.1, 2), signs[1] * rng.uniform(0.1, 2)]
+        my = [signs[2] * rng.uniform(0.1, 2), signs[3] * rng.uniform(0.1, 2)]
+        # the reference's arrays: x-face F.f5 = -Ez, y-face G.f4 = Ez, cell
+        # terms F5 = -Ez_c and G4 = Ez_c (M/ct.cu:26-65), mass fluxes F.f0, G.f0
+        NF5 = [-Ex[(0, 0)], -Ex[(0, 1)], 0.0, 0.0]
+        NG4 = [Ey[(0, 0)], 0.0, Ey[(1, 0)], 0.0]
+        F5 = [-Ec[c] for c in cells]
+        G4 = [Ec[c] for c in cells]
+        NF0 = [mx[0], mx[1], 0.0, 0.0]
+        NG0 = [my[0], 0.0, my[1], 0.0]
+        got = orc.ct_contact_emf(NF5, NG4, F5, G4, NF0, NG0)
+        want = _gs05_corner_emf(Ex, Ey, Ec, mx, my)
+        assert abs(got - want) <= 1e-14 * (1 + abs(want)), (t, got, want)
 
 
 def test_uniform_state_stays_uniform():
