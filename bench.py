@@ -244,18 +244,13 @@ def adjoint_bench(nruns, steps, run_steps):
     }
 
 
-def gradient_check(nruns, nx, ny, steps, ndirs, seed=0, degenerate_tol=0.0, fd_h=1e-4):
-    """Harris sheet nx x ny, batch of nruns: 'observed' final states from a
-    seeded eta field per run (sum of 6 Gaussians, amplitudes log-uniform in
-    [1e-4, 5e-3]); loss = |(rho, Bx, By)(T) - observed|^2 from eta = 1e-3;
-    gradients with respect to eta(x, y), vx0 and vy0 from the adjoint,
-    checked on run 0 against central differences along ndirs random
-    directions with every dt frozen at its recorded value.  The initial vx
-    carries seeded noise of amplitude 1e-3 like vy, so that MUSCL's limiter
-    is not at its kink (vx = 0 exactly has directional derivatives but no
-    gradient).  degenerate_tol > 0 switches the adjoint and the tangent to
-    the degenerate-limit derivative at HLLD star states whose denominator
-    is below that fraction of rho (S - u)^2."""
+def gradcheck_problem(nx, ny, nruns, seed=0):
+    """The configured gradient check's inputs: the Harris sheet on nx x ny
+    cells, its initial state with seeded vy noise (amplitude 1e-3) and vx
+    noise (so that MUSCL's limiter is off its kink: vx = 0 exactly has
+    directional derivatives but no gradient), and per run a seeded 'true'
+    resistivity (sum of 6 Gaussians, amplitudes log-uniform in
+    [1e-4, 5e-3], plus 1e-4)."""
     import numpy as np
     import resistivemhd_gpu_amd as mhd
     g = mhd.GHOST
@@ -277,15 +272,44 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0, degenerate_tol=0.0, fd_h
     vx = 1e-3 * (2 * rng.uniform(size=U0.shape[:2]) - 1)
     U0[..., 7] += 0.5 * U0[..., 0] * vx * vx
     U0[..., 1] = U0[..., 0] * vx
+    etas = [gaussians(np.random.default_rng(seed + 100 + r)) for r in range(nruns)]
+    return cfg, U0, etas
+
+
+def gradient_check(nruns, nx, ny, steps, ndirs, seed=0, degenerate_tol=0.0, fd_h=1e-4, plain_fd=True,
+                   log=None):
+    """Harris sheet nx x ny, batch of nruns: 'observed' final states from the
+    seeded resistivity of each run (gradcheck_problem); loss = |(rho, Bx,
+    By)(T) - observed|^2 from eta = 1e-3; gradients with respect to eta(x, y),
+    vx0 and vy0 from the adjoint of the whole batch.  Checked on run 0 along
+    ndirs random directions against
+      - the tangent-linear model (ndirs runs of one tangent rollout),
+      - central differences of the decision replay (2 ndirs runs: the
+        perturbed rollouts with every branch decision and dt of the recorded
+        one, the map the adjoint differentiates),
+      - and, reported only, central differences of the rollout itself with
+        dt frozen (plain_fd), whose own spread between steps h and 10 h shows
+        where branch switches within the stencil make the map non-smooth.
+    degenerate_tol > 0 switches the adjoint, the tangent and the replay to
+    the degenerate-limit derivative at HLLD star states whose denominator
+    is below that fraction of rho (S - u)^2."""
+    import numpy as np
+    import resistivemhd_gpu_amd as mhd
+    say = log or (lambda *a: None)
+    g = mhd.GHOST
+    cfg, U0, etas = gradcheck_problem(nx, ny, nruns, seed)
+    rho = U0[..., 0]
+    vx, vy = U0[..., 1] / rho, U0[..., 2] / rho
     cfgs = [cfg] * nruns
     t0 = time.perf_counter()
     obs = mhd.ResistiveMHD2DAdjoint(cfgs)
     for r in range(nruns):
         obs.setU0(r, U0)
-        obs.setEtaField(r, gaussians(np.random.default_rng(seed + 100 + r)))
+        obs.setEtaField(r, etas[r])
     obs.forward(steps)
     target = [obs.getU(r) for r in range(nruns)]
     del obs
+    say(f"observed states: {time.perf_counter() - t0:.1f} s")
     eta0 = np.full((cfg.nx, cfg.ny), 1e-3)
     mask = np.zeros((cfg.nx, cfg.ny, 8))
     mask[g:-g, g:-g, [0, 4, 5]] = 1.0
@@ -302,72 +326,97 @@ def gradient_check(nruns, nx, ny, steps, ndirs, seed=0, degenerate_tol=0.0, fd_h
         a.set_seed(r, 2.0 * mask * (UN[r] - target[r]))
     a.backward()
     t_grad = time.perf_counter() - t1
+    say(f"batch forward + adjoint: {t_grad:.1f} s")
     dts = a.dts(0, steps)
     gU, geta = a.grad_state(0), a.grad_eta(0)
-    rho = U0[..., 0]
     gvx = gU[..., 1] * rho + gU[..., 7] * rho * vx
-    gvy = gU[..., 2] * rho + gU[..., 7] * rho * (U0[..., 2] / rho)
+    gvy = gU[..., 2] * rho + gU[..., 7] * rho * vy
     del a
-
-    one = mhd.ResistiveMHD2DAdjoint([cfg])
-    one.set_degenerate_tol(degenerate_tol)
-
-    def final_state(U, eta):
-        one.setU0(0, U)
-        one.setEtaField(0, eta)
-        one.set_dts(0, dts)
-        one.forward(steps, frozen_dt=True)
-        return one.getU(0)
 
     def with_velocity(dvx, dvy, h):
         U = U0.copy()
-        u = vx + h * dvx
-        v = U0[..., 2] / rho + h * dvy
+        u, v = vx + h * dvx, vy + h * dvy
         U[..., 1], U[..., 2] = rho * u, rho * v
-        U[..., 7] = U0[..., 7] - 0.5 * rho * (vx * vx + (U0[..., 2] / rho) ** 2) + 0.5 * rho * (u * u + v * v)
+        U[..., 7] = U0[..., 7] - 0.5 * rho * (vx * vx + vy * vy) + 0.5 * rho * (u * u + v * v)
         return U
 
-    errs, tl_errs, mags, spreads = [], [], [], []
     drng = np.random.default_rng(seed + 7)
-    for k in range(ndirs):
-        de = drng.normal(size=eta0.shape) * 1e-3
-        dvx = drng.normal(size=rho.shape) * 1e-3
-        dvy = drng.normal(size=rho.shape) * 1e-3
-        ad = float((geta * de).sum() + (gvx * dvx)[g:-g, g:-g].sum() + (gvy * dvy)[g:-g, g:-g].sum())
-        def central(h):
-            Up = final_state(with_velocity(dvx, dvy, h), eta0 + h * de)
-            Um = final_state(with_velocity(dvx, dvy, -h), eta0 - h * de)
-            # L(Up) - L(Um) summed cell by cell as (Up - Um)(Up + Um - 2 T),
-            # so two large sums do not cancel
-            return float((mask * (Up - Um) * (Up + Um - 2.0 * target[0])).sum()) / (2 * h)
-        fd = central(fd_h)
-        # the differences' own spread between steps h and 10 h: where the map
-        # is not smooth at that scale, no derivative can match them
-        spreads.append(abs(central(10 * fd_h) - fd) / abs(fd))
-        errs.append(abs(ad - fd) / abs(fd))
-        # tangent-linear model along the same direction (interior cells; the
-        # boundary fill of U0 is part of the rollout)
+    dirs = [(drng.normal(size=eta0.shape) * 1e-3, drng.normal(size=rho.shape) * 1e-3,
+             drng.normal(size=rho.shape) * 1e-3) for _ in range(ndirs)]
+    ads = [float((geta * de).sum() + (gvx * dvx)[g:-g, g:-g].sum() + (gvy * dvy)[g:-g, g:-g].sum())
+           for de, dvx, dvy in dirs]
+
+    def fd_of(Up, Um, h):
+        # L(Up) - L(Um) summed cell by cell as (Up - Um)(Up + Um - 2 T), so
+        # two large sums do not cancel
+        return float((mask * (Up - Um) * (Up + Um - 2.0 * target[0])).sum()) / (2 * h)
+
+    t2 = time.perf_counter()
+    tl = mhd.ResistiveMHD2DAdjoint([cfg] * ndirs)
+    tl.set_degenerate_tol(degenerate_tol)
+    for k, (de, dvx, dvy) in enumerate(dirs):
         dU0 = np.zeros_like(U0)
         dU0[..., 1] = rho * dvx
         dU0[..., 2] = rho * dvy
-        dU0[..., 7] = rho * (vx * dvx + (U0[..., 2] / rho) * dvy)
-        one.setU0(0, U0)
-        one.setEtaField(0, eta0)
-        one.set_dts(0, dts)
-        one.set_direction(0, dU0, de)
-        one.tangent(steps)
-        tl = float((2.0 * mask * (one.getU(0) - target[0]) * one.get_tangent(0)).sum())
-        tl_errs.append(abs(ad - tl) / abs(tl))
-        mags.append({"adjoint": ad, "tangent": tl, "fd": fd})
-    return {
+        dU0[..., 7] = rho * (vx * dvx + vy * dvy)
+        tl.setU0(k, U0)
+        tl.setEtaField(k, eta0)
+        tl.set_dts(k, dts)
+        tl.set_direction(k, dU0, de)
+    tl.tangent(steps)
+    tls = [float((2.0 * mask * (UN[0] - target[0]) * tl.get_tangent(k)).sum()) for k in range(ndirs)]
+    del tl
+    say(f"tangent of {ndirs} directions: {time.perf_counter() - t2:.1f} s")
+
+    def replay_fd(h):
+        rp = mhd.ResistiveMHD2DAdjoint([cfg] * (2 * ndirs))
+        rp.set_degenerate_tol(degenerate_tol)
+        for k, (de, dvx, dvy) in enumerate(dirs):
+            for q, sg in ((0, 1.0), (1, -1.0)):
+                r = 2 * k + q
+                rp.setU0(r, U0)
+                rp.setEtaField(r, eta0)
+                rp.set_dts(r, dts)
+                rp.set_replay_state(r, with_velocity(dvx, dvy, sg * h), eta0 + sg * h * de)
+        rp.replay(steps)
+        out = [fd_of(rp.get_tangent(2 * k), rp.get_tangent(2 * k + 1), h) for k in range(ndirs)]
+        del rp
+        return out
+
+    t3 = time.perf_counter()
+    rfd = replay_fd(fd_h)
+    say(f"decision replay of {2 * ndirs} perturbed rollouts: {time.perf_counter() - t3:.1f} s")
+    rel = lambda x, y: abs(x - y) / abs(y)
+    res = {
         "workload": f"harris {nx}x{ny}, batch {nruns}, {steps} steps", "directions": ndirs,
         "degenerate_tol": degenerate_tol, "fd_h": fd_h,
-        "fd_rel_err_max": max(errs), "fd_rel_err_median": float(np.median(errs)), "fd_rel_errs": errs,
-        "tangent_rel_err_max": max(tl_errs), "tangent_rel_errs": tl_errs, "values": mags,
-        "fd_step_spread": spreads, "fd_step_spread_median": float(np.median(spreads)),
+        "tangent_rel_errs": [rel(x, y) for x, y in zip(ads, tls)],
+        "replay_fd_rel_errs": [rel(x, y) for x, y in zip(ads, rfd)],
+        "values": [{"adjoint": x, "tangent": y, "replay_fd": z} for x, y, z in zip(ads, tls, rfd)],
         "loss": float((mask * (UN[0] - target[0]) ** 2).sum()),
         "fwd_adj_seconds_batch": t_grad, "setup_seconds": t1 - t0,
     }
+    res["tangent_rel_err_max"] = max(res["tangent_rel_errs"])
+    res["replay_fd_rel_err_max"] = max(res["replay_fd_rel_errs"])
+    if plain_fd:
+        def plain(h):
+            pf = mhd.ResistiveMHD2DAdjoint([cfg] * (2 * ndirs))
+            for k, (de, dvx, dvy) in enumerate(dirs):
+                for q, sg in ((0, 1.0), (1, -1.0)):
+                    r = 2 * k + q
+                    pf.setU0(r, with_velocity(dvx, dvy, sg * h))
+                    pf.setEtaField(r, eta0 + sg * h * de)
+                    pf.set_dts(r, dts)
+            pf.forward(steps, frozen_dt=True)
+            out = [fd_of(pf.getU(2 * k), pf.getU(2 * k + 1), h) for k in range(ndirs)]
+            del pf
+            return out
+        fd, fd10 = plain(fd_h), plain(10 * fd_h)
+        res["fd_rel_errs"] = [rel(x, y) for x, y in zip(ads, fd)]
+        res["fd_rel_err_median"] = float(np.median(res["fd_rel_errs"]))
+        res["fd_step_spread"] = [rel(y, x) for x, y in zip(fd, fd10)]
+        res["fd_step_spread_median"] = float(np.median(res["fd_step_spread"]))
+    return res
 
 
 def harris_ly(nx, ny):
@@ -464,10 +513,10 @@ def main():
     ap.add_argument("--check-runs", type=int, default=32)
     ap.add_argument("--check-steps", type=int, default=2000)
     ap.add_argument("--check-dirs", type=int, default=16)
-    ap.add_argument("--degenerate-tol", type=float, default=1e-8,
+    ap.add_argument("--degenerate-tol", type=float, default=1e-6,
                     help="gradcheck: derivative rule at degenerate HLLD star states (0: the exact derivative, "
                          "which is rounding noise at the Harris far field's faces)")
-    ap.add_argument("--fd-h", type=float, default=1e-4, help="gradcheck: central-difference step")
+    ap.add_argument("--fd-h", type=float, default=1e-5, help="gradcheck: central-difference step")
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=40, help="timed batched steps")
     ap.add_argument("--plain-steps", type=int, default=10, help="timed steps per run of the plain loop")
@@ -511,7 +560,8 @@ def main():
         return
     if args.workload == "gradcheck":
         res = gradient_check(args.check_runs, args.check_nx, args.check_ny, args.check_steps, args.check_dirs,
-                             degenerate_tol=args.degenerate_tol, fd_h=args.fd_h)
+                             degenerate_tol=args.degenerate_tol, fd_h=args.fd_h,
+                             log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps({"metric": "adjoint gradient vs tangent-linear model and central differences",
                           "value": res["tangent_rel_err_max"], "unit": "relative", "higher_is_better": False,
                           "dtype": "f64", "data": "synthetic", "gradcheck": res}), flush=True)

@@ -316,6 +316,20 @@ int mhd_adjoint_get_grad_eta(mhd_adjoint *a, int run, double *host_eta);     /* 
 int mhd_adjoint_set_direction(mhd_adjoint *a, int run, const double *host_dU0, const double *host_deta);
 int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps);
 int mhd_adjoint_get_tangent(mhd_adjoint *a, int run, double *host_dUN);
+/* Decision replay (no counterpart in the reference): the rollout of a
+ * perturbed initial state P0 with the recorded dt and with every branch
+ * decision of every step (minmod limiter, HLLD wave speeds and region, the
+ * signs of Bx, min/max, and with mhd_adjoint_set_degenerate_tol > 0 the
+ * degenerate-limit switch) taken from the rollout of U0.  P0 and the
+ * perturbed resistivity are passed through mhd_adjoint_set_direction in
+ * place of the direction: host_dU0 = P0 (AoS, before the boundary fill),
+ * host_deta = the perturbed eta plane (for a run with an eta plane it is
+ * required: NULL means eta = 0; a run on the uniform cfg.eta keeps it).
+ * mhd_adjoint_get_tangent then returns P_N.  The map P0 -> P_N is smooth,
+ * equals the rollout at P0 = U0, and its derivative there is the tangent's,
+ * so central differences of a loss of P_N check the adjoint without the
+ * jumps that branch switches put into differences of the rollout itself. */
+int mhd_adjoint_replay(mhd_adjoint *a, int nsteps);
 /* Whole-batch transfers (one copy and one synchronisation): every run's AoS
  * state (or eta plane, nx*ny) concatenated in run order.  The _device forms
  * take a device pointer on the handle's GPU (e.g. a torch tensor's storage)

@@ -12,6 +12,10 @@ does.  For every listing three functions are emitted:
   NAME_tlm(in, in_d, par, out, out_d) tangent-linear (one direction)
   NAME_adj(in, par, out_b, in_b)      adjoint: in_b = (d out / d in)^T out_b
 
+  NAME_rep(in, in_p, par, out, out_p) decision replay: out_p is the primal
+                                      at in_p with every branch decision
+                                      taken at in (see replay_lines)
+
 Elementary operations and their derivatives:
   a+b, a-b, a*b, a/b, -a       the usual rules
   sqrt(a)                      1/(2 sqrt a), taken as 0 where sqrt a = 0
@@ -272,6 +276,40 @@ def tangent_ops(ops, d, zero):
     return out
 
 
+def replay_lines(params, inputs, ops):
+    """Primal of the perturbed inputs in_p with the branch decisions of the
+    base inputs in (whose primal is evaluated first): tmin / tmax select the
+    argument the base selected, fabs keeps the base sign, sgn and ind keep
+    their base values, and dlim(x, limit, c) adds the limit's change to x's
+    base value where c holds.  The map in_p -> out_p is smooth, equals the
+    primal at in_p = in (but for the sign of a zero) and its derivative at
+    in is the tangent-linear model's, so central differences of it check the
+    tangent and the adjoint without the jumps the branches put into the
+    primal itself."""
+    cst = set(params)
+    P = lambda a: a if (is_lit(a) or a in cst) else f"{a}_p"
+    out = [f"    const double {v}_p = in_p[{i}];" for i, v in enumerate(inputs)]
+    for dst, op, args in ops:
+        a = args[0] if args else None
+        b = args[1] if len(args) > 1 else None
+        if op in ("add", "sub", "mul", "div", "neg", "copy", "sqrt"):
+            e = C_OP[op].format(*[P(x) for x in args])
+        elif op == "fabs":
+            e = f"({a} > 0.0 ? {P(a)} : ({a} < 0.0 ? -{P(a)} : 0.0))"
+        elif op == "tmin":
+            e = f"(({b}) < ({a}) ? {P(b)} : {P(a)})"
+        elif op == "tmax":
+            e = f"(({a}) < ({b}) ? {P(b)} : {P(a)})"
+        elif op in ("sgn", "ind"):
+            e = dst
+        elif op == "dlim":
+            e = f"({args[2]} != 0.0 ? {a} + ({P(b)} - {b}) : {P(a)})"
+        else:
+            raise ValueError(op)
+        out.append(f"    const double {dst}_p = {e};")
+    return out
+
+
 def reverse_op(dst, op, args, active, ren, first):
     """Adjoint statements of one three-address op; ren maps a primal name to
     the local (recomputed) copy its value is read from.  first holds the
@@ -411,6 +449,13 @@ def emit(name, path):
     lines += ["    {", "    MHD_AD_CONTRACT"]
     lines += adjoint_lines(params, inputs, outputs, ops, guards)
     lines += ["    }"]
+    lines += ["}", "",
+              f"MHD_HD inline void {name}_rep(const double *in, const double *in_p, const double *par, double *out,"
+              f" double *out_p)", "{"]
+    lines += primal_lines(params, inputs, ops)
+    lines += [f"    out[{i}] = {v};" for i, v in enumerate(outputs)]
+    lines += replay_lines(params, inputs, ops)
+    lines += [f"    out_p[{i}] = {v}_p;" for i, v in enumerate(outputs)]
     lines += ["}", ""]
     return lines
 

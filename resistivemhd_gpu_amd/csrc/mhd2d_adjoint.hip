@@ -818,6 +818,42 @@ __global__ __launch_bounds__(256) void face_tlm_kernel(const AdjDesc *runs, cons
     for (int v = 0; v < 8; ++v) out[v * g.plane + o] = dF[v];
 }
 
+// Decision replay of the face fluxes: the flux of the perturbed state P (the
+// tangent's state buffers hold it) and perturbed resistivity (the direction's
+// eta buffer; a run without an eta plane keeps its uniform eta), with every
+// branch decision taken at the primal state.  The rest of the step is
+// linear, so the tangent's EMF, stage and boundary kernels advance P
+// exactly as the forward step advances a state.
+template <int DIR>
+__global__ __launch_bounds__(256) void face_rep_kernel(const AdjDesc *runs, const long long *first, int nruns,
+                                                       int stage)
+{
+    int r, i, j;
+    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    const AdjDesc &d = runs[r];
+    const Grid &g = d.g;
+    if (j >= g.ny) return;
+    const size_t o = (size_t)i * g.pitch + j;
+    double *out = DIR == 0 ? d.dF : d.dG;
+    const bool inside = DIR == 0 ? (i >= 2 && i <= g.nx - 4 && j >= 2 && j <= g.ny - 3)
+                                 : (i >= 2 && i <= g.nx - 3 && j >= 2 && j <= g.ny - 4);
+    if (!inside) {
+#pragma unroll
+        for (int v = 0; v < 8; ++v) out[v * g.plane + o] = 0.0;
+        return;
+    }
+    const Soa S{stage == 2 ? d.us : d.u, g.pitch, g.plane};
+    const Soa SP{stage == 2 ? d.dus : d.du, g.pitch, g.plane};
+    const EtaAcc E{d.eta, g.pitch, d.ph.eta};
+    const EtaAcc EP{d.eta ? d.deta : nullptr, g.pitch, d.ph.eta};
+    const double par[4] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps, d.dtol};
+    const double rpar[2] = {d.ph.dx, d.ph.dy};
+    double F[8], FP[8];
+    mhd_ad::flux_full_rep(S, SP, E, EP, DIR, i, j, par, rpar, F, FP);
+#pragma unroll
+    for (int v = 0; v < 8; ++v) out[v * g.plane + o] = FP[v];
+}
+
 // Tangent of the corner EMF E(i, j) = (G4(i,j) + G4(i+1,j) - F5(i,j) - F5(i,j+1)) / 4.
 __global__ __launch_bounds__(256) void emf_tlm_kernel(const AdjDesc *runs, const long long *first, int nruns)
 {
@@ -858,9 +894,12 @@ __global__ __launch_bounds__(256) void stage_tlm_kernel(const AdjDesc *runs, con
             }
             continue;
         }
-        const double div = dtdx * (d.dF[q] - d.dF[q - g.pitch]) + dtdy * (d.dG[q] - d.dG[q - 1]);
-        if (stage == 1) d.dus[q] = d.du[q] - div;
-        else d.du[q] = 0.5 * (d.du[q] + d.dus[q] - div);
+        // the forward stages' operation order (P/resistiveMHD_2D.cu:67-82,
+        // 103-126), so that the decision replay at the recorded initial
+        // state is the forward rollout bit for bit
+        const double fx = dtdx * (d.dF[q] - d.dF[q - g.pitch]), gy = dtdy * (d.dG[q] - d.dG[q - 1]);
+        if (stage == 1) d.dus[q] = d.du[q] - fx - gy;
+        else d.du[q] = 0.5 * (d.du[q] + d.dus[q] - fx - gy);
     }
 }
 
@@ -1612,7 +1651,16 @@ int mhd_adjoint_set_direction(mhd_adjoint *a, int run, const double *host_dU0, c
     return MHD_OK;
 }
 
-int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps)
+// The tangent-linear rollout (replay false) or the decision replay of a
+// perturbed rollout (replay true): the same linear cell, CT and boundary
+// kernels around the tangent or the replayed face fluxes.
+static int run_tlm(mhd_adjoint *a, int nsteps, bool replay);
+
+int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps) { return run_tlm(a, nsteps, false); }
+
+int mhd_adjoint_replay(mhd_adjoint *a, int nsteps) { return run_tlm(a, nsteps, true); }
+
+static int run_tlm(mhd_adjoint *a, int nsteps, bool replay)
 {
     if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
     if (nsteps < 1 || nsteps > per_run_steps(a)) return afail(a, MHD_ERR_STATE, "needs a recorded or set dt per step");
@@ -1630,8 +1678,13 @@ int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps)
     if ((rc = set_replay(a, true, false, 0))) return rc;
     for (int st = 0; st < nsteps; ++st) {
         // tangents of stage 1 from (U, dU), primal U* = BC(stage 1 of U)
-        hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-        hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+        if (replay) {
+            hipLaunchKernelGGL(face_rep_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+            hipLaunchKernelGGL(face_rep_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+        } else {
+            hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+            hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+        }
         hipLaunchKernelGGL(emf_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
         hipLaunchKernelGGL(stage_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1, st);
         hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 1);
@@ -1640,8 +1693,13 @@ int mhd_adjoint_tangent(mhd_adjoint *a, int nsteps)
                                         b->items_per_xcd, b->blocks_per_xcd, b->cur, s, b->d_tail_run, b->tail_items));
         AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
         // tangents of stage 2 from (U*, dU*) (B keeps the stage-1 CT value)
-        hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
-        hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+        if (replay) {
+            hipLaunchKernelGGL(face_rep_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+            hipLaunchKernelGGL(face_rep_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+        } else {
+            hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+            hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+        }
         hipLaunchKernelGGL(stage_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2, st);
         hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 2);
         hipLaunchKernelGGL(bc_tlm_y_kernel, gy, blk, 0, s, a->d_desc, 2);

@@ -241,4 +241,35 @@ MHD_HD inline void flux_full_tlm(const S &U, const S &dU, const E &eta, const E 
     }
 }
 
+// Decision replay of flux_full: FP is the flux of the perturbed state P
+// (and perturbed resistivity etaP) with every branch decision (minmod,
+// HLLD speeds and region, signs, the degenerate-limit switch) taken at the
+// base state U; F is the flux of U.  Central differences of the rollout
+// built from it check the tangent and the adjoint without the jumps of the
+// branches themselves.
+template <class S, class E>
+MHD_HD inline void flux_full_rep(const S &U, const S &P, const E &eta, const E &etaP, int dir, int i, int j,
+                                 const double *par, const double *rpar, double *F, double *FP)
+{
+    double in[32], in_p[32], ff[7], ff_p[7], rin[14], rin_p[14], d[3], d_p[3];
+    gather_face(U, dir, i, j, in);
+    gather_face(P, dir, i, j, in_p);
+    face_flux_rep(in, in_p, par, ff, ff_p);
+    gather_resist(U, eta, dir, i, j, rin);
+    gather_resist(P, etaP, dir, i, j, rin_p);
+    if (dir == 0) {
+        resist_x_rep(rin, rin_p, rpar, d, d_p);
+        F[0] = ff[0]; F[1] = ff[1]; F[2] = ff[2]; F[3] = ff[3]; F[4] = 0.0;
+        F[5] = ff[4] - d[0]; F[6] = ff[5] + d[1]; F[7] = ff[6] + d[2];
+        FP[0] = ff_p[0]; FP[1] = ff_p[1]; FP[2] = ff_p[2]; FP[3] = ff_p[3]; FP[4] = 0.0;
+        FP[5] = ff_p[4] - d_p[0]; FP[6] = ff_p[5] + d_p[1]; FP[7] = ff_p[6] + d_p[2];
+    } else {
+        resist_y_rep(rin, rin_p, rpar, d, d_p);
+        F[0] = ff[0]; F[1] = ff[3]; F[2] = ff[1]; F[3] = ff[2];
+        F[4] = ff[5] + d[0]; F[5] = 0.0; F[6] = ff[4] - d[1]; F[7] = ff[6] + d[2];
+        FP[0] = ff_p[0]; FP[1] = ff_p[3]; FP[2] = ff_p[1]; FP[3] = ff_p[2];
+        FP[4] = ff_p[5] + d_p[0]; FP[5] = 0.0; FP[6] = ff_p[4] - d_p[1]; FP[7] = ff_p[6] + d_p[2];
+    }
+}
+
 }  // namespace mhd_ad

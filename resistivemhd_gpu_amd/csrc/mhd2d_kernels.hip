@@ -8,9 +8,10 @@
 // Lane l holds row j0-2+l.  Everything along x (MUSCL window, the x-face
 // flux, EMF and flux of the previous column) stays in registers; y
 // neighbours are exchanged with cross-lane shuffles.  Per column the wave
-// loads one 64-row slice of the 8 planes (coalesced), computes the x-face
-// HLLD + resistive flux, the y-face flux of the previous column, the corner
-// EMF, and writes the updated cell two columns behind the load front.
+// reads one 64-row slice of the 8 planes and the resistivity, staged in LDS
+// one column ahead by direct-to-LDS loads, computes the x-face HLLD +
+// resistive flux, the y-face flux of the previous column, the corner EMF,
+// and writes the updated cell two columns behind the load front.
 //
 // Reference (paths relative to its root,
 // P = ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_periodicXY,
@@ -24,6 +25,16 @@
 namespace mhd {
 
 #define PL(ptr, v) ((ptr) + (size_t)(v) * g.plane)
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+// Rows of one staged column: global rows j0-3 .. j0+62 of a strip (66 rows,
+// 33 16-byte row pairs; j0 - 3 = 60 sy is even, so every pair is 16-byte
+// aligned).  Lane l's row j0-2+l sits at index l+1 and the row below it
+// (the By face under the cell) at index l.
+constexpr int kStageRows = 66;
+constexpr int kStagePlanes = 9;   // 8 state planes and the resistivity plane
 
 // x-frame primitive variables of one cell (P/calculate_half_Q.cu:4-38 with
 // the centred B of shiftUToCenterForCT, P/resistiveMHD_2D.cu:313-368).
@@ -166,6 +177,26 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     const double *Rh = PL(uin, 0), *Mx = PL(uin, 1), *My = PL(uin, 2), *Mz = PL(uin, 3);
     const double *Bx = PL(uin, 4), *By = PL(uin, 5), *Bz = PL(uin, 6), *En = PL(uin, 7);
 
+    // Full strips stage each column in LDS one column ahead with direct
+    // loads (global_load_lds), so a column's HBM latency overlaps the
+    // previous column's arithmetic instead of stalling the wavefront.
+    constexpr bool DMA = !TAIL;
+    __shared__ __attribute__((aligned(16))) double ring[2][kStagePlanes][kStageRows];
+    const int r0 = j0 - 3;
+    const int rpair = min(r0 + 2 * lane, (g.ny - 2) & ~1);
+    auto stage_col = [&](int col) {
+        if (lane < kStageRows / 2) {
+            const size_t off = (size_t)col * g.pitch + rpair;
+            double(*slot)[kStageRows] = ring[col & 1];
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(uin + (size_t)v * g.plane + off),
+                                                 (lds_void_t *)slot[v], 16, 0, 0);
+            if (etaf) __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + off), (lds_void_t *)slot[8], 16, 0, 0);
+        }
+    };
+    if (DMA) stage_col(i0 - 2);
+
     double bx_prev = Bx[(size_t)(i0 - 3 + dcol) * g.pitch + jl];
     Cell A{}, B{};                                   // cells c-2 and c-1
     Face7 Lx{};                                      // left state of face c-2+1/2
@@ -179,11 +210,23 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     double eta_prev = 0.0;   // resistivity of cell c-1
 
     for (int c = i0 - 2; c <= iend + 1; ++c) {
-        const size_t o = (size_t)(TAIL ? min(c + dcol, g.nx - 1) : c) * g.pitch + jl;
-        const double rho = Rh[o], mx = Mx[o], my = My[o], mz = Mz[o];
-        const double bx = Bx[o], by = By[o], bz = Bz[o], e = En[o];
-        const double bym = By[o - 1];
-        const double eta_c = etaf ? etaf[o] : ph.eta;
+        double rho, mx, my, mz, bx, by, bz, e, bym, eta_c;
+        if (DMA) {
+            // s_waitcnt vmcnt(0): this column's staging loads have landed
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+            const double(*slot)[kStageRows] = ring[c & 1];
+            rho = slot[0][lane + 1]; mx = slot[1][lane + 1]; my = slot[2][lane + 1]; mz = slot[3][lane + 1];
+            bx = slot[4][lane + 1]; by = slot[5][lane + 1]; bz = slot[6][lane + 1]; e = slot[7][lane + 1];
+            bym = slot[5][lane];
+            eta_c = etaf ? slot[8][lane + 1] : ph.eta;
+            if (c + 1 <= iend + 1) stage_col(c + 1);
+        } else {
+            const size_t o = (size_t)(TAIL ? min(c + dcol, g.nx - 1) : c) * g.pitch + jl;
+            rho = Rh[o]; mx = Mx[o]; my = My[o]; mz = Mz[o];
+            bx = Bx[o]; by = By[o]; bz = Bz[o]; e = En[o];
+            bym = By[o - 1];
+            eta_c = etaf ? etaf[o] : ph.eta;
+        }
         const double bxc = 0.5 * (bx + bx_prev);
         bx_prev = bx;
         const double byc = 0.5 * (by + bym);

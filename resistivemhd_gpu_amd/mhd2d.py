@@ -43,7 +43,7 @@ EXPORTS = (
     "mhd_adjoint_get_grad_states", "mhd_save_subdomain", "mhd_load_subdomain", "mhd_adjoint_get_grad_etas",
     "mhd_adjoint_set_states_device", "mhd_adjoint_get_states_device", "mhd_adjoint_set_seeds_device",
     "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device", "mhd_adjoint_set_face_kernel",
-    "mhd_adjoint_autotune", "mhd_adjoint_set_degenerate_tol",
+    "mhd_adjoint_autotune", "mhd_adjoint_set_degenerate_tol", "mhd_adjoint_replay",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -172,6 +172,7 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_synchronize": (I, [P]),
         "mhd_adjoint_set_direction": (I, [P, I, D, D]),
         "mhd_adjoint_tangent": (I, [P, I]),
+        "mhd_adjoint_replay": (I, [P, I]),
         "mhd_adjoint_get_tangent": (I, [P, I, D]),
         "mhd_adjoint_set_states": (I, [P, D]),
         "mhd_adjoint_get_states": (I, [P, D]),
@@ -624,6 +625,7 @@ class ResistiveMHD2DAdjoint:
             raise MHDError(f"libmhd2d error {rc}: {msg.decode() if msg else ''}")
         self._h = h
         self.nsteps = 0
+        self._eta_set = {}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -651,9 +653,11 @@ class ResistiveMHD2DAdjoint:
     def setEtaField(self, run, eta):
         if eta is None:
             self._ok(self._lib.mhd_adjoint_set_eta_field(self._h, run, None))
+            self._eta_set[run] = False
             return
         eta = np.ascontiguousarray(eta, dtype=np.float64)
         self._ok(self._lib.mhd_adjoint_set_eta_field(self._h, run, _dptr(eta)))
+        self._eta_set[run] = True
 
     def setU0(self, run, U0):
         U0 = np.ascontiguousarray(U0, dtype=np.float64)
@@ -709,6 +713,21 @@ class ResistiveMHD2DAdjoint:
 
     def tangent(self, nsteps=None):
         self._ok(self._lib.mhd_adjoint_tangent(self._h, int(nsteps or self.nsteps)))
+
+    def replay(self, nsteps=None):
+        """Decision replay: the rollout of the perturbed state set by
+        set_replay_state with every branch decision and dt of the recorded
+        rollout of U0 (read the result with get_tangent)."""
+        self._ok(self._lib.mhd_adjoint_replay(self._h, int(nsteps or self.nsteps)))
+
+    def set_replay_state(self, run, P0, eta_p=None):
+        """Perturbed initial state (before its boundary fill) and perturbed
+        resistivity of the decision replay.  A run with an eta field needs
+        eta_p (the whole perturbed field); a run on the uniform cfg.eta keeps
+        it."""
+        if self._eta_set.get(run) and eta_p is None:
+            raise ValueError("this run has an eta field: pass the perturbed field as eta_p")
+        self.set_direction(run, P0, eta_p)
 
     def get_tangent(self, run):
         out = np.empty(self.shape(run))

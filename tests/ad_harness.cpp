@@ -69,6 +69,20 @@ void h_flux_full_tlm(const double *U, const double *dU, const double *eta, const
     mhd_ad::flux_full_tlm(S, dS, E, dE, dir, i, j, par, rpar, F, dF);
 }
 
+void h_face_flux_rep(const double *in, const double *in_p, const double *par, double *out, double *out_p)
+{
+    mhd_ad::face_flux_rep(in, in_p, par, out, out_p);
+}
+
+void h_flux_full_rep(const double *U, const double *P, const double *eta, const double *etaP, double eta0, int nx,
+                     int ny, int dir, int i, int j, const double *par, const double *rpar, double *F, double *FP)
+{
+    (void)nx;
+    const Aos S{U, ny}, SP{P, ny};
+    const Eta E{eta, ny, eta0}, EP{etaP, ny, eta0};
+    mhd_ad::flux_full_rep(S, SP, E, EP, dir, i, j, par, rpar, F, FP);
+}
+
 // Adjoint of the face flux: adds into lamUc (centred-state adjoint, AoS)
 // and geta.
 void h_flux_full_adj(const double *U, const double *eta, double eta0, int nx, int ny, int dir, int i, int j,

@@ -378,3 +378,65 @@ def test_degenerate_rule_adjoint_is_transpose_of_tangent(ad):
             ad.h_face_flux_adj(p(x), p(on), p(w), p(xb))
             lhs, rhs = out_d @ w, v @ xb
             assert abs(lhs - rhs) <= 1e-12 * max(1.0, abs(lhs), np.abs(out_d).max() * np.abs(w).max()), (lhs, rhs)
+
+
+def test_replay_equals_primal_at_the_base_inputs(ad):
+    """The decision replay evaluated at the base inputs is the primal."""
+    rng = np.random.default_rng(51)
+    par = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20, 1e-8])
+    out, out_p, ref = np.empty(7), np.empty(7), np.empty(7)
+    for x in [degenerate_face()] + [random_face_inputs(rng) for _ in range(50)]:
+        ad.h_face_flux_rep(p(x), p(x), p(par), p(out), p(out_p))
+        ad.h_face_flux(p(x), p(par), p(ref))
+        assert np.array_equal(out, ref) and np.array_equal(out_p, ref)
+
+
+@pytest.mark.parametrize("dtol", [0.0, 1e-8])
+def test_replay_differences_match_tangent_on_every_face(ad, dtol):
+    """Central differences of the decision replay (branches frozen at the
+    base inputs) agree with the tangent on every draw, with no exception for
+    draws that straddle a region switch or a limiter kink (those are what
+    break plain differences, test_face_flux_tangent_matches_finite_differences),
+    and at the degenerate face with the limit rule on."""
+    rng = np.random.default_rng(53)
+    par = np.array([2.0 / 3.0, 5.0 / 3.0, 1e-20, dtol])
+    xs = [random_face_inputs(rng) for _ in range(300)]
+    if dtol > 0.0:
+        xs.append(degenerate_face())
+    out, out_d, fp, fm, tmp = (np.empty(7) for _ in range(5))
+    worst = 0.0
+    for x in xs:
+        dx = rng.normal(size=32) * np.maximum(np.abs(x), 1e-3)
+        ad.h_face_flux_tlm(p(x), p(dx), p(par), p(out), p(out_d))
+        h = 1e-6
+        ad.h_face_flux_rep(p(x), p(x + h * dx), p(par), p(tmp), p(fp))
+        ad.h_face_flux_rep(p(x), p(x - h * dx), p(par), p(tmp), p(fm))
+        fd = (fp - fm) / (2 * h)
+        worst = max(worst, np.abs(fd - out_d).max() / max(1.0, np.abs(out_d).max()))
+    assert worst < 1e-6, worst
+
+
+@pytest.mark.parametrize("d", [0, 1])
+def test_assembled_replay_differences_match_tangent(ad, d):
+    """The same for the assembled face (gather, rotation, resistive stencil
+    with a perturbed eta, centring)."""
+    cfg, U = noisy_ot(14, 7)
+    rng = np.random.default_rng(60 + d)
+    eta = rng.uniform(1e-3, 1e-2, (cfg.nx, cfg.ny))
+    par, rpar = par_of(cfg)
+    ri, rj = face_ranges(cfg, d)
+    F, dF, FP, FM, T = (np.empty(8) for _ in range(5))
+    worst = 0.0
+    for i in list(ri)[::2]:
+        for j in list(rj)[::2]:
+            dU = rng.normal(size=U.shape) * 0.1
+            deta = rng.normal(size=eta.shape) * 1e-3
+            ad.h_flux_full_tlm(p(U), p(dU), p(eta), p(deta), ctypes.c_double(0.0), cfg.nx, cfg.ny, d, i, j, p(par),
+                               p(rpar), p(F), p(dF))
+            h = 1e-6
+            ad.h_flux_full_rep(p(U), p(U + h * dU), p(eta), p(eta + h * deta), ctypes.c_double(0.0), cfg.nx, cfg.ny,
+                               d, i, j, p(par), p(rpar), p(T), p(FP))
+            ad.h_flux_full_rep(p(U), p(U - h * dU), p(eta), p(eta - h * deta), ctypes.c_double(0.0), cfg.nx, cfg.ny,
+                               d, i, j, p(par), p(rpar), p(T), p(FM))
+            worst = max(worst, np.abs((FP - FM) / (2 * h) - dF).max() / max(1.0, np.abs(dF).max()))
+    assert worst < 1e-6, worst

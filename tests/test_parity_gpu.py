@@ -143,3 +143,29 @@ def test_harris_4096x2048_configured_length():
         assert div < 1e-12 * bscale, chunk
         assert sim.totalTime > t
         t = sim.totalTime
+
+
+def _golden_hashes():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "full_size_hashes.json")
+    return json.load(open(path))
+
+
+@pytest.mark.parametrize("name", ["harris_4096x2048", "harris_4096x4096"])
+def test_full_size_headline_bitwise_against_oracle_hash(name):
+    """The headline workloads at full size (Harris sheet 4096 x 2048 and
+    4096 x 4096, 100 steps, exact numerics): the state is bitwise the
+    oracle's, compared through the SHA-256 the oracle's run wrote
+    (tests/golden/make_full_size_hashes.py), with dt and time."""
+    from tests.golden.make_full_size_hashes import state_hash
+    ref = _golden_hashes()[name]
+    print(f"case {name}: {ref['steps']} steps against the oracle's hash", flush=True)
+    cfg = mhd.harris_config(ref["nx"], ref["ny"], ly=ref["ly"])
+    sim = mhd.ResistiveMHD2D(cfg)
+    sim.initHarris(seed=ref["seed"], noise_amp=ref["noise_amp"], eta0=ref["eta0"], eta1=ref["eta1"])
+    assert state_hash(sim.getU()) == ref["initial_state_sha256"]
+    sim.steps(ref["steps"])
+    assert not sim.checkCalculationIsCrashed()
+    assert state_hash(sim.getU()) == ref["state_sha256"]
+    assert sim.dt == ref["dt"] and sim.totalTime == ref["time"]
