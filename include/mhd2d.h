@@ -165,6 +165,30 @@ int mhd_init_orszag_tang(mhd_handle *h);
 int mhd_init_harris(mhd_handle *h, unsigned long long seed, double noise_amp, double eta0, double eta1);
 int mhd_init_turbulence(mhd_handle *h, unsigned long long seed, int kmax, double rms_v, double rms_b);
 
+/* Allocation-free step.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  nsteps RK2 steps of
+ * mhd_step_rk2 (exact numerics, MHD_SCHEME_SINGLE, the boundaries of
+ * cfg->bc_x / bc_y) of a state the caller owns on the current device,
+ * started from its CFL dt, enqueued on `stream` (a hipStream_t; NULL: the
+ * null stream) with no host synchronisation.  dev_state: 8 planes (rho,
+ * rho u, rho v, rho w, Bx, By, Bz, e, face B as in the AoS layout), plane
+ * v at dev_state + v * plane, cell (i, j) of a plane at i * pitch + j,
+ * ghosts included; advanced in place.  dev_eta: the per-cell resistivity
+ * in the layout of one plane, or NULL for the uniform cfg->eta.  pitch and
+ * plane must be even with pitch >= ny and plane >= nx * pitch, and both
+ * buffers 16-byte aligned.  dev_info (device, 3 doubles, or NULL): dt of
+ * the last step, the time the steps advanced, 1.0 if an energy became NaN.
+ * It allocates nothing: U*, the control block and the symmetric-x side row
+ * live in a fixed scratch array in the library, 32 MiB on each GPU the
+ * library runs on, held until the process ends.  The limit is plane <=
+ * 2^19 doubles (about 724 x 724 cells with ghosts) and nx <= 8192; larger
+ * grids return MHD_ERR_INVALID (use mhd_create / mhd_step_rk2).  Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one GPU
+ * (concurrent streams or host threads) race. */
+int mhd_step_rk2_noalloc(const mhd_config *cfg, double *dev_state, long long pitch, long long plane,
+                         const double *dev_eta, int nsteps, void *stream, double *dev_info);
+
 /* Step scheme.
  *  - MHD_SCHEME_SINGLE (default): the single-GPU library's oneStepRK2
  *    (P/resistiveMHD_2D.cu:133-195, boundaries per cfg.bc_x / bc_y).

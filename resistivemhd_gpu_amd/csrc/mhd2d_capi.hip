@@ -22,8 +22,10 @@
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -64,12 +66,19 @@ struct mhd_handle {
 
 namespace {
 
-thread_local std::string g_last_error;
+// Last error of calls without a handle: a fixed buffer, so that setting it
+// allocates nothing (mhd_step_rk2_noalloc reports through it).
+thread_local char g_last_error[512];
+
+void set_last_error(const char *msg)
+{
+    std::snprintf(g_last_error, sizeof g_last_error, "%s", msg);
+}
 
 int fail(mhd_handle *h, int code, const std::string &msg)
 {
     if (h) h->err = msg;
-    g_last_error = msg;
+    set_last_error(msg.c_str());
     return code;
 }
 
@@ -106,26 +115,37 @@ int choose_seg(int nx_int, int nstrips, long long slots)
     return best;
 }
 
-void layout(mhd_handle *h, int seg)
+// Work decomposition of one grid (nx, ny with ghosts) of a given plane
+// layout: strips, segments of seg columns (0: from the occupancy, given the
+// resident stage wavefronts per CU) and the XCD-contiguous item mapping.
+void layout_grid(Grid &g, int nx, int ny, int pitch, long long plane, int num_cu, int waves_per_cu, int seg)
 {
-    Grid &g = h->g;
-    g.nx = h->cfg.nx;
-    g.ny = h->cfg.ny;
-    g.pitch = (h->cfg.ny + 7) & ~7;
-    g.plane = ((long long)g.nx * g.pitch + 63) & ~63LL;
-    const int nx_int = h->cfg.nx - 2 * mhd::kGhost;
-    g.nstrips = (h->cfg.ny - 2 * mhd::kGhost + mhd::kStripRows - 1) / mhd::kStripRows;
+    g.nx = nx;
+    g.ny = ny;
+    g.pitch = pitch;
+    g.plane = plane;
+    const int nx_int = nx - 2 * mhd::kGhost;
+    g.nstrips = (ny - 2 * mhd::kGhost + mhd::kStripRows - 1) / mhd::kStripRows;
+    g.tail_g = 0;
+    g.tail_items = 0;
     if (seg <= 0) {
-        const long long slots = (long long)(h->num_cu > 0 ? h->num_cu : 256) * mhd::stage_waves_per_cu(h->exact);
+        const long long slots = (long long)(num_cu > 0 ? num_cu : 256) * waves_per_cu;
         seg = choose_seg(nx_int, g.nstrips, slots);
     }
     g.seg = seg < nx_int ? seg : nx_int;
     g.nsegs = (nx_int + g.seg - 1) / g.seg;
     // MI355X: 256 CUs in 8 XCDs of 32
-    g.xcds = h->num_cu >= 64 ? h->num_cu / 32 : 1;
+    g.xcds = num_cu >= 64 ? num_cu / 32 : 1;
     if (g.xcds < 1) g.xcds = 1;
     const int items = g.nstrips * g.nsegs;
     g.items_per_xcd = (items + g.xcds - 1) / g.xcds;
+}
+
+void layout(mhd_handle *h, int seg)
+{
+    const int pitch = (h->cfg.ny + 7) & ~7;
+    const long long plane = ((long long)h->cfg.nx * pitch + 63) & ~63LL;
+    layout_grid(h->g, h->cfg.nx, h->cfg.ny, pitch, plane, h->num_cu, mhd::stage_waves_per_cu(h->exact), seg);
 }
 
 void set_phys(mhd_handle *h)
@@ -138,19 +158,28 @@ void set_phys(mhd_handle *h)
     h->ph.eta = h->cfg.eta;
 }
 
+// nullptr if the configuration is valid, else what is wrong with it.
+const char *config_error(const mhd_config *c)
+{
+    if (!c) return "config is NULL";
+    if (c->nx < 4 * MHD_GHOST || c->ny < 4 * MHD_GHOST) return "nx and ny must be >= 12 (6 interior cells)";
+    if ((long long)c->nx * c->ny > (1LL << 31)) return "nx*ny must fit in 2^31 cells";
+    if (!(c->dx > 0.0) || !(c->dy > 0.0)) return "dx and dy must be > 0";
+    if (!(c->gamma > 1.0)) return "gamma must be > 1";
+    if (!(c->cfl > 0.0)) return "cfl must be > 0";
+    if (!(c->eta >= 0.0)) return "eta must be >= 0";
+    if (!(c->eps >= 0.0)) return "eps must be >= 0";
+    if ((c->bc_x != MHD_BC_PERIODIC && c->bc_x != MHD_BC_SYMMETRIC) ||
+        (c->bc_y != MHD_BC_PERIODIC && c->bc_y != MHD_BC_SYMMETRIC))
+        return "bc_x/bc_y must be MHD_BC_PERIODIC or MHD_BC_SYMMETRIC";
+    return nullptr;
+}
+
 int check_config(const mhd_config *c, std::string &why)
 {
-    if (!c) { why = "config is NULL"; return 0; }
-    if (c->nx < 4 * MHD_GHOST || c->ny < 4 * MHD_GHOST) { why = "nx and ny must be >= 12 (6 interior cells)"; return 0; }
-    if ((long long)c->nx * c->ny > (1LL << 31)) { why = "nx*ny must fit in 2^31 cells"; return 0; }
-    if (!(c->dx > 0.0) || !(c->dy > 0.0)) { why = "dx and dy must be > 0"; return 0; }
-    if (!(c->gamma > 1.0)) { why = "gamma must be > 1"; return 0; }
-    if (!(c->cfl > 0.0)) { why = "cfl must be > 0"; return 0; }
-    if (!(c->eta >= 0.0)) { why = "eta must be >= 0"; return 0; }
-    if (!(c->eps >= 0.0)) { why = "eps must be >= 0"; return 0; }
-    if ((c->bc_x != MHD_BC_PERIODIC && c->bc_x != MHD_BC_SYMMETRIC) ||
-        (c->bc_y != MHD_BC_PERIODIC && c->bc_y != MHD_BC_SYMMETRIC)) { why = "bc_x/bc_y must be MHD_BC_PERIODIC or MHD_BC_SYMMETRIC"; return 0; }
-    return 1;
+    const char *e = config_error(c);
+    if (e) why = e;
+    return e == nullptr;
 }
 
 int ensure_aos(mhd_handle *h)
@@ -185,13 +214,148 @@ int read_ctrl(mhd_handle *h, Ctrl *out)
     return MHD_OK;
 }
 
+// ----------------------------------------------- the allocation-free step
+// Scratch of mhd_step_rk2_noalloc on each GPU: U* (8 planes of at most
+// kNoAllocPlane doubles), the side row of the symmetric-x boundary face
+// (at most kNoAllocSide doubles) and the control block.  The array lives in
+// the library's code object, so every GPU the library runs on holds it
+// (32 MiB) until the process ends; its address on a device comes from
+// hipGetSymbolAddress with that device current (host code never uses the
+// array's name as an address).
+constexpr long long kNoAllocPlane = 1LL << 19;
+constexpr long long kNoAllocSide = 1LL << 13;
+constexpr long long kNoAllocCtrl = 64;   // doubles reserved for the control block
+constexpr long long kNoAllocDoubles = 8 * kNoAllocPlane + kNoAllocSide + kNoAllocCtrl;
+static_assert(sizeof(Ctrl) <= sizeof(double) * kNoAllocCtrl, "control block does not fit its scratch slot");
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+    double *ptr = nullptr;   // this device's copy of scratch
+    int num_cu = 0;
+    int waves_per_cu = 0;
+};
+NoAllocDevice g_noalloc[kMaxDevices];
+std::mutex g_noalloc_mutex;
+
+// The control block of a fresh run: no dt yet, no frozen-dt replay.
+__global__ void noalloc_ctrl_init_kernel(Ctrl *c)
+{
+    if (threadIdx.x == 0) {
+        c->acc[0] = c->acc[1] = mhd::kInfBits;
+        c->dt = 0.0;
+        c->time = 0.0;
+        c->steps = 0;
+        c->crashed = 0;
+        c->first_nan[0] = c->first_nan[1] = 0;
+        c->pad = 0;
+        c->dt_in = nullptr;
+        c->dt_out = nullptr;
+        c->seq0 = 0;
+    }
+}
+
+// dt of the last step, the time the steps advanced and the crash flag.
+__global__ void noalloc_info_kernel(const Ctrl *c, double *info)
+{
+    if (threadIdx.x == 0) {
+        info[0] = c->dt;
+        info[1] = c->time;
+        info[2] = c->crashed ? 1.0 : 0.0;
+    }
+}
+
+int noalloc_fail(int code, const char *msg)
+{
+    set_last_error(msg);
+    return code;
+}
+
 }  // namespace
 
 extern "C" {
 
+int mhd_step_rk2_noalloc(const mhd_config *cfg, double *dev_state, long long pitch, long long plane,
+                         const double *dev_eta, int nsteps, void *stream, double *dev_info)
+{
+    char msg[256];
+    if (const char *bad = config_error(cfg)) {
+        std::snprintf(msg, sizeof msg, "mhd_step_rk2_noalloc: %s", bad);
+        return noalloc_fail(MHD_ERR_INVALID, msg);
+    }
+    if (!dev_state) return noalloc_fail(MHD_ERR_INVALID, "mhd_step_rk2_noalloc: dev_state is NULL");
+    if (nsteps < 0) return noalloc_fail(MHD_ERR_INVALID, "mhd_step_rk2_noalloc: nsteps must be >= 0");
+    if (pitch < cfg->ny || (pitch & 1) || plane < (long long)cfg->nx * pitch || (plane & 1) ||
+        ((uintptr_t)dev_state & 15) || ((uintptr_t)dev_eta & 15))
+        return noalloc_fail(MHD_ERR_INVALID, "mhd_step_rk2_noalloc: needs even pitch >= ny, even plane >= nx * pitch "
+                                             "and 16-byte aligned buffers");
+    if (plane > kNoAllocPlane || cfg->nx > kNoAllocSide) {
+        std::snprintf(msg, sizeof msg,
+                      "mhd_step_rk2_noalloc: a plane of %lld doubles (nx %d) exceeds the scratch maximum of %lld "
+                      "doubles per plane and nx <= %lld; use mhd_create / mhd_step_rk2",
+                      plane, cfg->nx, kNoAllocPlane, kNoAllocSide);
+        return noalloc_fail(MHD_ERR_INVALID, msg);
+    }
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return noalloc_fail(MHD_ERR_HIP, "mhd_step_rk2_noalloc: hipGetDevice failed");
+    if (dev < 0 || dev >= kMaxDevices) return noalloc_fail(MHD_ERR_INVALID, "mhd_step_rk2_noalloc: device index too large");
+    NoAllocDevice d;
+    {
+        std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+        NoAllocDevice &slot = g_noalloc[dev];
+        if (!slot.ptr) {
+            void *p = nullptr;
+            if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess || !p) {
+                std::snprintf(msg, sizeof msg, "mhd_step_rk2_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+                return noalloc_fail(MHD_ERR_HIP, msg);
+            }
+            int cu = 0;
+            if ((e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+                return noalloc_fail(MHD_ERR_HIP, "mhd_step_rk2_noalloc: hipDeviceGetAttribute failed");
+            slot.num_cu = cu;
+            slot.waves_per_cu = mhd::stage_waves_per_cu(true);
+            slot.ptr = static_cast<double *>(p);
+        }
+        d = slot;
+    }
+    Grid g{};
+    layout_grid(g, cfg->nx, cfg->ny, (int)pitch, plane, d.num_cu, d.waves_per_cu, 0);
+    Phys ph{cfg->gamma, cfg->dx, cfg->dy, cfg->eps, cfg->cfl, cfg->eta};
+    double *us = d.ptr;
+    double *side = d.ptr + 8 * plane;
+    Ctrl *ctrl = reinterpret_cast<Ctrl *>(d.ptr + 8 * kNoAllocPlane + kNoAllocSide);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const double *eta = dev_eta;
+    // the same launches as mhd_step_rk2 (exact numerics, single-GPU scheme),
+    // with U* and the control block in the scratch
+    hipLaunchKernelGGL(noalloc_ctrl_init_kernel, dim3(1), dim3(64), 0, s, ctrl);
+    int cur = 0;
+    if ((e = mhd::launch_dt_all(true, false, dev_state, ctrl, cur, g, ph, s)) != hipSuccess) goto hip_error;
+    for (int n = 0; n < nsteps; ++n) {
+        if ((e = mhd::launch_stage(1, true, MHD_SCHEME_SINGLE, dev_state, us, dev_state, eta, side, nullptr, ctrl, g,
+                                   ph, cur, s)) != hipSuccess ||
+            (e = mhd::launch_bc_x(cfg->bc_x, true, us, nullptr, 0, g, ph, s)) != hipSuccess ||
+            (e = mhd::launch_bc_y(cfg->bc_y, true, us, nullptr, 0, g, ph, s)) != hipSuccess ||
+            (e = mhd::launch_stage(2, true, MHD_SCHEME_SINGLE, us, dev_state, dev_state, eta, side, nullptr, ctrl, g,
+                                   ph, cur, s)) != hipSuccess ||
+            (e = mhd::launch_bc_x(cfg->bc_x, true, dev_state, ctrl, cur ^ 1, g, ph, s)) != hipSuccess ||
+            (e = mhd::launch_bc_y(cfg->bc_y, true, dev_state, ctrl, cur ^ 1, g, ph, s)) != hipSuccess)
+            goto hip_error;
+        cur ^= 1;
+    }
+    if (dev_info) hipLaunchKernelGGL(noalloc_info_kernel, dim3(1), dim3(64), 0, s, ctrl, dev_info);
+    if ((e = hipGetLastError()) != hipSuccess) goto hip_error;
+    return MHD_OK;
+hip_error:
+    std::snprintf(msg, sizeof msg, "mhd_step_rk2_noalloc: kernel launch: %s", hipGetErrorString(e));
+    return noalloc_fail(MHD_ERR_HIP, msg);
+}
+
 const char *mhd_last_error(const mhd_handle *h)
 {
-    return h ? h->err.c_str() : g_last_error.c_str();
+    return h ? h->err.c_str() : g_last_error;
 }
 
 int mhd_create(const mhd_config *cfg, mhd_handle **out)

@@ -43,7 +43,7 @@ EXPORTS = (
     "mhd_adjoint_get_grad_states", "mhd_save_subdomain", "mhd_load_subdomain", "mhd_adjoint_get_grad_etas",
     "mhd_adjoint_set_states_device", "mhd_adjoint_get_states_device", "mhd_adjoint_set_seeds_device",
     "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device", "mhd_adjoint_set_face_kernel",
-    "mhd_adjoint_autotune", "mhd_adjoint_set_degenerate_tol", "mhd_adjoint_replay",
+    "mhd_adjoint_autotune", "mhd_adjoint_set_degenerate_tol", "mhd_adjoint_replay", "mhd_step_rk2_noalloc",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -173,6 +173,7 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_set_direction": (I, [P, I, D, D]),
         "mhd_adjoint_tangent": (I, [P, I]),
         "mhd_adjoint_replay": (I, [P, I]),
+        "mhd_step_rk2_noalloc": (I, [C, P, ctypes.c_longlong, ctypes.c_longlong, P, I, P, P]),
         "mhd_adjoint_get_tangent": (I, [P, I, D]),
         "mhd_adjoint_set_states": (I, [P, D]),
         "mhd_adjoint_get_states": (I, [P, D]),
