@@ -364,6 +364,7 @@ int mhd_adjoint_set_seeds(mhd_adjoint *a, const double *host_all);         /* al
 int mhd_adjoint_get_grad_states(mhd_adjoint *a, double *host_all);         /* all dL/dU0 */
 int mhd_adjoint_get_grad_etas(mhd_adjoint *a, double *host_all);           /* all dL/deta */
 int mhd_adjoint_set_states_device(mhd_adjoint *a, const double *dev_all);
+int mhd_adjoint_set_eta_fields_device(mhd_adjoint *a, const double *dev_all);   /* every run's eta plane */
 int mhd_adjoint_get_states_device(mhd_adjoint *a, double *dev_all);
 int mhd_adjoint_set_seeds_device(mhd_adjoint *a, const double *dev_all);
 int mhd_adjoint_get_grad_states_device(mhd_adjoint *a, double *dev_all);

@@ -1436,13 +1436,21 @@ int mhd_adjoint_set_degenerate_tol(mhd_adjoint *a, double tol)
     return upload_desc(a);
 }
 
+// The checkpoints and history no longer belong to the inputs: backward needs
+// a new forward pass first.
+static void invalidate_rollout(mhd_adjoint *a)
+{
+    a->forward_done = false;
+    a->last_in_hist = false;
+}
+
 int mhd_adjoint_set_eta_field(mhd_adjoint *a, int run, const double *host_eta)
 {
     int rc = check_run(a, run);
     if (rc) return rc;
     if ((rc = mhd_batch_set_eta_field(a->fwd, run, host_eta))) return afail(a, rc, a->fwd->err);
     a->desc[run].eta = a->fwd->desc[run].eta;
-    a->forward_done = false;   // the recorded rollout used the previous field
+    invalidate_rollout(a);   // the recorded rollout used the previous field
     return upload_desc(a);
 }
 
@@ -1456,6 +1464,7 @@ int mhd_adjoint_set_state(mhd_adjoint *a, int run, const double *host_aos)
     AHIP(a, hipMemcpyAsync(a->fwd->aos, host_aos, bytes, hipMemcpyHostToDevice, stream_of(a)));
     AHIP(a, mhd::launch_unpack(a->fwd->aos, d.u0, d.g, stream_of(a)));
     AHIP(a, hipStreamSynchronize(stream_of(a)));
+    invalidate_rollout(a);   // the recorded rollout started from the previous U0
     return MHD_OK;
 }
 
@@ -1466,6 +1475,7 @@ int mhd_adjoint_set_dts(mhd_adjoint *a, int run, const double *host_dts, int nst
     if (!host_dts || nsteps < 1) return afail(a, MHD_ERR_INVALID, "need nsteps >= 1 dt values");
     if ((rc = ensure_dts(a, nsteps))) return rc;
     AHIP(a, hipMemcpy(a->desc[run].dts, host_dts, sizeof(double) * nsteps, hipMemcpyHostToDevice));
+    invalidate_rollout(a);   // the recorded rollout's checkpoints used other dt values
     return MHD_OK;
 }
 
@@ -1760,6 +1770,7 @@ static int upload_all(mhd_adjoint *a, const double *src, int which, bool device)
                        a->nruns, which, const_cast<double *>(stage));
     AHIP(a, hipGetLastError());
     AHIP(a, hipStreamSynchronize(stream_of(a)));
+    if (which == kU0) invalidate_rollout(a);   // new initial states: the recorded rollout is stale
     return MHD_OK;
 }
 
@@ -1808,6 +1819,28 @@ static int grad_eta_all(mhd_adjoint *a, double *dst, bool device)
         AHIP(a, hipMemcpyAsync(dst, a->stage_all, sizeof(double) * off, hipMemcpyDeviceToHost, stream_of(a)));
     AHIP(a, hipStreamSynchronize(stream_of(a)));
     return MHD_OK;
+}
+
+int mhd_adjoint_set_eta_fields_device(mhd_adjoint *a, const double *dev_all)
+{
+    if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
+    if (!dev_all) return afail(a, MHD_ERR_INVALID, "NULL buffer");
+    mhd_batch *b = a->fwd;
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    size_t off = 0;
+    for (int r = 0; r < a->nruns; ++r) {
+        mhd::RunDesc &d = b->desc[r];
+        if (!b->eta[r]) AHIP(a, hipMalloc(&b->eta[r], sizeof(double) * (size_t)d.g.plane));
+        AHIP(a, mhd::launch_scatter_plane(dev_all + off, b->eta[r], d.g, stream_of(a)));
+        d.eta = b->eta[r];
+        a->desc[r].eta = d.eta;
+        off += (size_t)d.g.nx * d.g.ny;
+    }
+    AHIP(a, hipMemcpyAsync(b->d_desc, b->desc.data(), sizeof(mhd::RunDesc) * (size_t)a->nruns, hipMemcpyHostToDevice,
+                           stream_of(a)));
+    AHIP(a, hipStreamSynchronize(stream_of(a)));
+    invalidate_rollout(a);   // the recorded rollout used the previous fields
+    return upload_desc(a);
 }
 
 int mhd_adjoint_get_grad_etas(mhd_adjoint *a, double *host_all) { return grad_eta_all(a, host_all, false); }

@@ -44,6 +44,7 @@ EXPORTS = (
     "mhd_adjoint_set_states_device", "mhd_adjoint_get_states_device", "mhd_adjoint_set_seeds_device",
     "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device", "mhd_adjoint_set_face_kernel",
     "mhd_adjoint_autotune", "mhd_adjoint_set_degenerate_tol", "mhd_adjoint_replay", "mhd_step_rk2_noalloc",
+    "mhd_adjoint_set_eta_fields_device",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -174,6 +175,7 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_tangent": (I, [P, I]),
         "mhd_adjoint_replay": (I, [P, I]),
         "mhd_step_rk2_noalloc": (I, [C, P, ctypes.c_longlong, ctypes.c_longlong, P, I, P, P]),
+        "mhd_adjoint_set_eta_fields_device": (I, [P, P]),
         "mhd_adjoint_get_tangent": (I, [P, I, D]),
         "mhd_adjoint_set_states": (I, [P, D]),
         "mhd_adjoint_get_states": (I, [P, D]),
@@ -815,6 +817,14 @@ class ResistiveMHD2DAdjoint:
             raise ValueError(f"{t.numel()} elements, expected {numel}")
         torch.cuda.current_stream().synchronize()
         return ctypes.c_void_p(t.data_ptr())
+
+    def setEtaField_all_device(self, t):
+        """Every run's eta plane (nx * ny per run, run order) from a flat
+        float64 tensor on the GPU."""
+        n = sum(c.nx * c.ny for c in self.cfgs)
+        self._ok(self._lib.mhd_adjoint_set_eta_fields_device(self._h, self._dev(t, n)))
+        for r in range(len(self.cfgs)):
+            self._eta_set[r] = True
 
     def setU0_all_device(self, t):
         self._ok(self._lib.mhd_adjoint_set_states_device(self._h, self._dev(t, self._total())))

@@ -28,18 +28,26 @@ class MHDRollout(torch.autograd.Function):
         B = U0.shape[0]
         if len(cfgs) != B or eta.shape[0] != B:
             raise ValueError("one configuration and one eta field per run")
+        for r, c in enumerate(cfgs):
+            if tuple(U0.shape[1:]) != (c.nx, c.ny, 8) or tuple(eta.shape[1:]) != (c.nx, c.ny):
+                raise ValueError(f"run {r}: U0 {tuple(U0.shape[1:])} and eta {tuple(eta.shape[1:])} do not match "
+                                 f"the grid ({c.nx}, {c.ny})")
+        if U0.is_cuda != eta.is_cuda:
+            raise ValueError("U0 and eta must be on the same device")
         adj = ResistiveMHD2DAdjoint(cfgs)
         adj.set_degenerate_tol(degenerate_tol)
-        etan = eta.detach().to("cpu").numpy()
-        for r in range(B):
-            adj.setEtaField(r, etan[r])
         if U0.is_cuda:
-            # states move between torch's memory and the library's on the GPU
+            # states and resistivities move between torch's memory and the
+            # library's on the GPU, with no host round trip
+            adj.setEtaField_all_device(eta.detach().contiguous().reshape(-1))
             adj.setU0_all_device(U0.detach().contiguous().reshape(-1))
             adj.forward(int(nsteps), int(stride))
             UN = torch.empty_like(U0)
             adj.getU_all_device(UN.view(-1))
         else:
+            etan = eta.detach().contiguous().numpy()
+            for r in range(B):
+                adj.setEtaField(r, etan[r])
             U0n = U0.detach().contiguous().numpy()
             adj.setU0_all([U0n[r] for r in range(B)])
             adj.forward(int(nsteps), int(stride))

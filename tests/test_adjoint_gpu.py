@@ -55,6 +55,10 @@ def _loss(cfg, U0, eta, W, nsteps, dts):
     return float((W * a.getU(0)).sum())
 
 
+def _rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def _velocity_direction(U0, seed):
     """dU0 of a velocity perturbation (dvx, dvy) at fixed rho, p and B."""
     rng = np.random.default_rng(seed)
@@ -262,9 +266,17 @@ def test_batched_adjoint_and_stride_agree():
             s.backward()
             gU, ge = s.grad_state(0), s.grad_eta(0)
             # face contributions are summed with atomics: equal up to the
-            # rounding of their order, amplified where contributions cancel
-            np.testing.assert_allclose(gU, grads[r][0], rtol=1e-6, atol=1e-9 * np.abs(gU).max())
-            np.testing.assert_allclose(ge, grads[r][1], rtol=1e-6, atol=1e-9 * np.abs(ge).max())
+            # rounding of their order.  The well-conditioned Orszag-Tang runs
+            # hold that to 1e-11 in relative L2 (an index or scatter slip
+            # would be O(1)); the noisy Harris sheet has Jacobian entries
+            # that cancel by orders of magnitude near degenerate HLLD star
+            # states, which amplifies the reordering to ~1e-9 there.
+            bound = 1e-8 if r == 2 else 1e-11
+            assert _rel_l2(gU, grads[r][0]) <= bound, (r, stride, _rel_l2(gU, grads[r][0]))
+            assert _rel_l2(ge, grads[r][1]) <= bound, (r, stride, _rel_l2(ge, grads[r][1]))
+            if r == 2:
+                np.testing.assert_allclose(gU, grads[r][0], rtol=1e-6, atol=1e-9 * np.abs(gU).max())
+                np.testing.assert_allclose(ge, grads[r][1], rtol=1e-6, atol=1e-9 * np.abs(ge).max())
 
 
 @pytest.mark.parametrize("case", ["ot", "harris-degenerate"])
@@ -336,8 +348,11 @@ def test_whole_batch_transfers_host_and_device():
         assert np.array_equal(host_U[r], b.getU(r))
         assert np.array_equal(gU[o:o + n].reshape(U0s[r].shape), host_gU[r])
         assert np.array_equal(ge[oe:oe + c.nx * c.ny].reshape(c.nx, c.ny), host_ge[r])
-        np.testing.assert_allclose(host_gU[r], b.grad_state(r), rtol=1e-6, atol=1e-9 * np.abs(host_gU[r]).max())
-        np.testing.assert_allclose(host_ge[r], b.grad_eta(r), rtol=1e-6, atol=1e-9 * np.abs(host_ge[r]).max())
+        # per-run and whole-batch uploads give the same rollout; the
+        # gradients agree to the rounding of the atomic sums' order
+        bound = 1e-11 if r == 0 else 1e-8
+        assert _rel_l2(host_gU[r], b.grad_state(r)) <= bound, r
+        assert _rel_l2(host_ge[r], b.grad_eta(r)) <= bound, r
         o += n
         oe += c.nx * c.ny
     with pytest.raises(ValueError):
@@ -379,3 +394,46 @@ def test_face_kernel_forms_agree():
             ref, got = out["per_face"][r][k], out["sweep"][r][k]
             assert np.all(np.isfinite(got))
             np.testing.assert_allclose(got, ref, rtol=rtol, atol=atol * np.abs(ref).max())
+
+
+def test_eta_fields_device_upload_and_stale_rollout():
+    """Every run's eta plane uploaded from the GPU gives the rollout and the
+    gradients of the per-run host uploads; after new initial states, eta
+    fields or dt values, backward refuses to run on the stale rollout."""
+    import torch
+    cases = [_ot(24), _harris(32, 20)]
+    cfgs = [c for c, _ in cases]
+    etas = [_eta_field(c, 7 + r) for r, c in enumerate(cfgs)]
+    Ws = [_weights(c, 60 + r) for r, c in enumerate(cfgs)]
+    res = []
+    for device in (False, True):
+        a = mhd.ResistiveMHD2DAdjoint(cfgs)
+        if device:
+            a.setEtaField_all_device(torch.tensor(np.concatenate([e.ravel() for e in etas]), device="cuda"))
+        else:
+            for r in range(len(cfgs)):
+                a.setEtaField(r, etas[r])
+        for r, (c, U0) in enumerate(cases):
+            a.setU0(r, U0)
+        a.forward(5)
+        for r in range(len(cfgs)):
+            a.set_seed(r, Ws[r])
+        a.backward()
+        res.append([(a.getU(r), a.grad_state(r), a.grad_eta(r)) for r in range(len(cfgs))])
+        if device:
+            a.setU0(0, cases[0][1])
+            with pytest.raises(mhd.MHDError):
+                a.backward()
+            a.forward(5)
+            a.backward()
+            a.set_dts(1, a.dts(1, 5))
+            with pytest.raises(mhd.MHDError):
+                a.backward()
+            a.forward(5)
+            a.setEtaField(1, etas[1])
+            with pytest.raises(mhd.MHDError):
+                a.backward()
+    for r in range(len(cfgs)):
+        assert np.array_equal(res[0][r][0], res[1][r][0])
+        for q in (1, 2):
+            assert _rel_l2(res[1][r][q], res[0][r][q]) <= (1e-11 if r == 0 else 1e-8)
