@@ -15,7 +15,8 @@ host synchronisation inside it.
 
 Prints one JSON line: zone-cycles/s (interior cells x RK2 steps / s), the
 roofline of the dominant stage kernel (HIP events on the library's stream)
-and the CPU oracle timed on a bounded sample of the same setup.
+and the CPU baseline: the Thrust twin of the reference's pass structure on
+the host's cores, timed on a bounded sample of the same setup.
 """
 import argparse
 import json
@@ -65,33 +66,60 @@ def measured_traffic(root, nx, ny, numerics, kernel):
     return None
 
 
-def cpu_baseline(nx, ny, steps, workload="harris"):
-    from oracle.oracle import SCHEME_MULTI_SHARE, OracleSim, num_threads
-    import resistivemhd_gpu_amd as mhd
-    if workload == "turbulence":
-        cfg = mhd.default_config(nx, ny, 0.0, 2.0 * 3.141592653589793, 0.0, 2.0 * 3.141592653589793, eta=1e-4)
-        o = OracleSim.from_config(cfg)
-        o.set_scheme(SCHEME_MULTI_SHARE)
-        o.set_state(mhd.fill_turbulence(cfg, seed=0))
-        o.apply_halo()
-        what = "MPI library's step on one periodic subdomain, turbulence"
-    else:
-        cfg = mhd.harris_config(nx, ny, ly=harris_ly(nx, ny))
-        o = OracleSim.from_config(cfg)
-        o.set_state(mhd.fill_harris(cfg, seed=0, noise_amp=1e-3))
-        o.set_eta_field(mhd.fill_harris_eta(cfg))
-        o.apply_boundary()
-        what = "Harris sheet"
-    o.step(1)  # first touch of the intermediates
+def host_cpu():
+    """CPU model, sockets and cores of this host (lscpu)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "cpus": info.get("CPU(s)")}
+
+
+def _twin_rate(mhd, nx, ny, warm, steps, threads):
+    from oracle.oracle import TwinSim
+    cfg = mhd.harris_config(nx, ny, ly=harris_ly(nx, ny))
+    TwinSim.set_threads(threads)
+    t = TwinSim.from_config(cfg)
+    t.set_state(mhd.fill_harris(cfg, seed=0, noise_amp=1e-3))
+    t.set_eta_field(mhd.fill_harris_eta(cfg))
+    t.apply_boundary()
+    t.step(warm)   # first touch of the intermediates
     t0 = time.perf_counter()
-    o.step(steps)
-    t = time.perf_counter() - t0
+    crashed = t.step(steps)
+    sec = time.perf_counter() - t0
+    del t
+    return nx * ny * steps / sec, sec, crashed
+
+
+def cpu_baseline(nx, ny, steps, threads=0, nx1=1024, ny1=1024, steps1=2):
+    """The Thrust twin of the reference's pass structure (rocThrust's OpenMP
+    backend, oracle/thrust_twin.cpp; its states equal the oracle's bit for
+    bit) on the Harris-sheet workload: all host threads on the GPU line's
+    grid for a short step count, and one thread on a smaller grid of the
+    same cell size."""
+    import resistivemhd_gpu_amd as mhd
+    from oracle.oracle import TwinSim
+    threads = threads or TwinSim.max_threads()
+    rate, sec, crashed = _twin_rate(mhd, nx, ny, 1, steps, threads)
+    rate1, sec1, _ = _twin_rate(mhd, nx1, ny1, 1, steps1, 1)
     return {
-        "value": nx * ny * steps / t,
+        "value": rate,
         "unit": "zone-cycles/s",
-        "cores": num_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": f"CPU oracle (C restatement, OpenMP), {what} {nx}x{ny} interior, {steps} RK2 steps, {t:.1f} s",
+        "sample": (f"Thrust twin of the reference's pass structure (rocThrust OpenMP backend, {threads} threads), "
+                   f"Harris sheet {nx}x{ny}, {steps} RK2 steps after 1 warm-up step, {sec:.1f} s"),
+        "one_thread": {"value": rate1, "unit": "zone-cycles/s",
+                       "sample": f"same twin on 1 thread, Harris sheet {nx1}x{ny1}, {steps1} steps, {sec1:.1f} s"},
+        "host": host_cpu(),
+        "crashed": crashed,
     }
 
 
@@ -530,10 +558,8 @@ def main():
     ap.add_argument("--sweep", type=int, default=0, help="columns per wavefront sweep (0 = library default)")
     ap.add_argument("--numerics", choices=["exact", "fast"], default="exact",
                     help="exact: bitwise reference arithmetic (library default); fast: rounding-level differences")
-    ap.add_argument("--cpu-nx", type=int, default=1024)
-    ap.add_argument("--cpu-ny", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="timed steps of the CPU baseline on the GPU line's grid")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the 4096 x 2048 Harris line of the default run")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (reference point for HBM byte counters)")
@@ -618,11 +644,8 @@ def main():
         line["batched"] = batch_bench(args.batch_runs, args.batch_steps, args.plain_steps, 500, args.numerics)
     if not multi and not args.no_adjoint and not args.no_cpu and args.numerics == "exact":
         line["batched_fwd_adjoint"] = adjoint_bench(args.batch_runs, args.adj_steps, 500)
-    if not args.no_cpu:
-        if multi:
-            line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_nx, args.cpu_steps, "turbulence")
-        else:
-            line["cpu_baseline"] = cpu_baseline(args.cpu_nx, args.cpu_ny, args.cpu_steps)
+    if not args.no_cpu and not multi:
+        line["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_steps)
     print(json.dumps(line), flush=True)
 
 

@@ -10,6 +10,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+TWIN_PATH = os.path.join(HERE, "libthrust_twin.so")
 BC_PERIODIC, BC_SYMMETRIC = 0, 1
 SCHEME_SINGLE, SCHEME_MULTI_SHARE = 0, 1
 
@@ -202,3 +203,103 @@ def cell_dt(cell, gamma, dx, dy, eps=1e-20):
 def num_threads():
     """OpenMP threads the oracle's loops run on."""
     return load().oracle_num_threads()
+
+
+_twin = None
+
+
+def load_twin():
+    """libthrust_twin.so: the reference's pass structure on rocThrust's
+    OpenMP backend (the benchmark's CPU baseline)."""
+    global _twin
+    if _twin is not None:
+        return _twin
+    if not os.path.exists(TWIN_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "libthrust_twin.so"], check=True)
+    lib = ctypes.CDLL(TWIN_PATH)
+    D = ctypes.POINTER(ctypes.c_double)
+    P = ctypes.c_void_p
+    c_d, c_i = ctypes.c_double, ctypes.c_int
+    sig = {
+        "twin_create": (P, [c_i, c_i, c_d, c_d, c_d, c_d, c_d, c_d, c_i, c_i]),
+        "twin_destroy": (None, [P]),
+        "twin_set_state": (None, [P, D]),
+        "twin_get_state": (None, [P, D]),
+        "twin_set_eta_field": (None, [P, D]),
+        "twin_apply_boundary": (None, [P]),
+        "twin_step": (c_i, [P, c_i]),
+        "twin_get_dt": (c_d, [P]),
+        "twin_get_time": (c_d, [P]),
+        "twin_set_threads": (None, [c_i]),
+        "twin_max_threads": (c_i, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _twin = lib
+    return lib
+
+
+class TwinSim:
+    """The Thrust twin of the reference's single-GPU step (same array shapes
+    as OracleSim)."""
+
+    def __init__(self, nx, ny, dx, dy, gamma=5.0 / 3.0, cfl=0.7, eps=1e-20, eta=0.0,
+                 bc_x=BC_PERIODIC, bc_y=BC_PERIODIC):
+        self._lib = load_twin()
+        self.nx, self.ny = nx, ny
+        self._s = self._lib.twin_create(nx, ny, dx, dy, gamma, cfl, eps, eta, bc_x, bc_y)
+        if not self._s:
+            raise MemoryError("twin_create failed")
+
+    @classmethod
+    def from_config(cls, cfg):
+        return cls(cfg.nx, cfg.ny, cfg.dx, cfg.dy, cfg.gamma, cfg.cfl, cfg.eps, cfg.eta, cfg.bc_x, cfg.bc_y)
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            self._lib.twin_destroy(self._s)
+            self._s = None
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        assert U.shape == (self.nx, self.ny, 8)
+        self._lib.twin_set_state(self._s, _p(U))
+
+    def get_state(self):
+        out = np.empty((self.nx, self.ny, 8))
+        self._lib.twin_get_state(self._s, _p(out))
+        return out
+
+    def set_eta_field(self, eta):
+        if eta is None:
+            self._lib.twin_set_eta_field(self._s, None)
+            return
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        assert eta.shape == (self.nx, self.ny)
+        self._lib.twin_set_eta_field(self._s, _p(eta))
+
+    def apply_boundary(self):
+        self._lib.twin_apply_boundary(self._s)
+
+    def step(self, n=1):
+        """n steps of the examples' loop (step, then the NaN check); True if
+        a step crashed."""
+        return bool(self._lib.twin_step(self._s, int(n)))
+
+    @property
+    def dt(self):
+        return self._lib.twin_get_dt(self._s)
+
+    @property
+    def time(self):
+        return self._lib.twin_get_time(self._s)
+
+    @staticmethod
+    def set_threads(n):
+        load_twin().twin_set_threads(int(n))
+
+    @staticmethod
+    def max_threads():
+        return load_twin().twin_max_threads()
