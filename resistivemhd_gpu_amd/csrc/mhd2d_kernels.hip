@@ -11,7 +11,13 @@
 // reads one 64-row slice of the 8 planes and the resistivity, staged in LDS
 // one column ahead by direct-to-LDS loads, computes the x-face HLLD +
 // resistive flux, the y-face flux of the previous column, the corner EMF,
-// and writes the updated cell two columns behind the load front.
+// and writes the updated cell two columns behind the load front.  The
+// update's own operands come from LDS too: stage 1 keeps four staged
+// columns, so the cell two columns back is still there; stage 2 stages
+// that cell's U* and U^n while the y-face flux is computed.  The next
+// column's loads are issued after the x-face flux: a spill reload waits for
+// every older vector memory load, so loads issued at the top of the column
+// would be waited for a few hundred instructions later.
 //
 // Reference (paths relative to its root,
 // P = ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_periodicXY,
@@ -180,19 +186,43 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     // Full strips stage each column in LDS one column ahead with direct
     // loads (global_load_lds), so a column's HBM latency overlaps the
     // previous column's arithmetic instead of stalling the wavefront.
+    // LDS per wavefront: 19,008 B in stage 1 and 16,896 B in stage 2, so
+    // eight wavefronts (two per SIMD, the VGPR limit) fit a CU's 160 KiB.
     constexpr bool DMA = !TAIL;
-    __shared__ __attribute__((aligned(16))) double ring[2][kStagePlanes][kStageRows];
+    // stage 1 keeps four columns, so that the update of column c-2 reads
+    // its conserved state from LDS
+    constexpr int kSlots = (STAGE == 1 && !MULTI) ? 4 : 2;
+    __shared__ __attribute__((aligned(16))) double ring[kSlots][kStagePlanes][kStageRows];
     const int r0 = j0 - 3;
     const int rpair = min(r0 + 2 * lane, (g.ny - 2) & ~1);
     auto stage_col = [&](int col) {
         if (lane < kStageRows / 2) {
             const size_t off = (size_t)col * g.pitch + rpair;
-            double(*slot)[kStageRows] = ring[col & 1];
+            double(*slot)[kStageRows] = ring[col & (kSlots - 1)];
 #pragma unroll
             for (int v = 0; v < 8; ++v)
                 __builtin_amdgcn_global_load_lds((glb_void_t *)(uin + (size_t)v * g.plane + off),
                                                  (lds_void_t *)slot[v], 16, 0, 0);
             if (etaf) __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + off), (lds_void_t *)slot[8], 16, 0, 0);
+        }
+    };
+    // stage 2: U* and U^n of the cell being updated, staged before the
+    // y-face flux of the same column
+    constexpr bool UPD_DMA = DMA && STAGE == 2 && !MULTI;
+    __shared__ __attribute__((aligned(16))) double upd[UPD_DMA ? 14 : 1][kStageRows];
+    auto stage_upd = [&](int col) {
+        if (lane < kStageRows / 2) {
+            const size_t off = (size_t)col * g.pitch + rpair;
+#pragma unroll
+            for (int v = 0; v < 8; ++v)
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(uin + (size_t)v * g.plane + off),
+                                                 (lds_void_t *)upd[v], 16, 0, 0);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int v = k < 4 ? k : k + 2;
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(uold + (size_t)v * g.plane + off),
+                                                 (lds_void_t *)upd[8 + k], 16, 0, 0);
+            }
         }
     };
     if (DMA) stage_col(i0 - 2);
@@ -214,12 +244,14 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
         if (DMA) {
             // s_waitcnt vmcnt(0): this column's staging loads have landed
             __builtin_amdgcn_s_waitcnt(0x0F70);
-            const double(*slot)[kStageRows] = ring[c & 1];
+            const double(*slot)[kStageRows] = ring[c & (kSlots - 1)];
             rho = slot[0][lane + 1]; mx = slot[1][lane + 1]; my = slot[2][lane + 1]; mz = slot[3][lane + 1];
             bx = slot[4][lane + 1]; by = slot[5][lane + 1]; bz = slot[6][lane + 1]; e = slot[7][lane + 1];
             bym = slot[5][lane];
             eta_c = etaf ? slot[8][lane + 1] : ph.eta;
-            if (c + 1 <= iend + 1) stage_col(c + 1);
+            // warm-up columns stage the next column at once; the others after
+            // their x-face flux
+            if (c < i0) stage_col(c + 1);
         } else {
             const size_t o = (size_t)(TAIL ? min(c + dcol, g.nx - 1) : c) * g.pitch + jl;
             rho = Rh[o]; mx = Mx[o]; my = My[o]; mz = Mz[o];
@@ -309,6 +341,9 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             Lx.p = B.px + 0.5 * sp;
             rJY = eJY; rJZ = eJZ; rJYBZ = eJYBZ; rJZBY = eJZBY;
             rTA = tA; rTB = tB;
+
+            if (DMA && c + 1 <= iend + 1) stage_col(c + 1);
+            if (UPD_DMA && c >= i0 + 2 && c - 2 < send) stage_upd(c - 2);
 
             // y-face flux of column c-1 at faces j+1/2 (HLLD::calculateFluxG,
             // P/hlld.cu:209-233): rotated frame (u,v,w)->(v,w,u),
@@ -433,8 +468,15 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     if (STAGE == 1) {
                         // oneStepFirstFunctor (P/resistiveMHD_2D.cu:53-86) and
                         // CT_kernel (P/ct.cu:44-59)
-                        const double U0 = Rh[os], U1 = Mx[os], U2 = My[os], U3 = Mz[os];
-                        const double U4 = Bx[os], U5 = By[os], U6 = Bz[os], U7 = En[os];
+                        double U0, U1, U2, U3, U4, U5, U6, U7;
+                        if (DMA) {
+                            const double(*us)[kStageRows] = ring[s & (kSlots - 1)];
+                            U0 = us[0][lane + 1]; U1 = us[1][lane + 1]; U2 = us[2][lane + 1]; U3 = us[3][lane + 1];
+                            U4 = us[4][lane + 1]; U5 = us[5][lane + 1]; U6 = us[6][lane + 1]; U7 = us[7][lane + 1];
+                        } else {
+                            U0 = Rh[os]; U1 = Mx[os]; U2 = My[os]; U3 = Mz[os];
+                            U4 = Bx[os]; U5 = By[os]; U6 = Bz[os]; U7 = En[os];
+                        }
                         n0 = U0 - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m);
                         n1 = U1 - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m);
                         n2 = U2 - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m);
@@ -457,10 +499,21 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                         if (upd_row) {
                             // oneStepSecondFunctor (P/resistiveMHD_2D.cu:88-130);
                             // B is the stage-1 CT value (same EMF, same old B).
-                            const double S0 = Rh[os], S1 = Mx[os], S2 = My[os], S3 = Mz[os];
-                            const double S4 = Bx[os], S5 = By[os], S6 = Bz[os], S7 = En[os];
-                            const double O0 = PL(uold, 0)[os], O1 = PL(uold, 1)[os], O2 = PL(uold, 2)[os];
-                            const double O3 = PL(uold, 3)[os], O6 = PL(uold, 6)[os], O7 = PL(uold, 7)[os];
+                            double S0, S1, S2, S3, S4, S5, S6, S7, O0, O1, O2, O3, O6, O7;
+                            if (UPD_DMA) {
+                                // s_waitcnt vmcnt(0): the cell's staging loads have landed
+                                __builtin_amdgcn_s_waitcnt(0x0F70);
+                                const int r = lane + 1;
+                                S0 = upd[0][r]; S1 = upd[1][r]; S2 = upd[2][r]; S3 = upd[3][r];
+                                S4 = upd[4][r]; S5 = upd[5][r]; S6 = upd[6][r]; S7 = upd[7][r];
+                                O0 = upd[8][r]; O1 = upd[9][r]; O2 = upd[10][r]; O3 = upd[11][r];
+                                O6 = upd[12][r]; O7 = upd[13][r];
+                            } else {
+                                S0 = Rh[os]; S1 = Mx[os]; S2 = My[os]; S3 = Mz[os];
+                                S4 = Bx[os]; S5 = By[os]; S6 = Bz[os]; S7 = En[os];
+                                O0 = PL(uold, 0)[os]; O1 = PL(uold, 1)[os]; O2 = PL(uold, 2)[os];
+                                O3 = PL(uold, 3)[os]; O6 = PL(uold, 6)[os]; O7 = PL(uold, 7)[os];
+                            }
                             n0 = 0.5 * (O0 + S0 - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m));
                             n1 = 0.5 * (O1 + S1 - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m));
                             n2 = 0.5 * (O2 + S2 - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m));
