@@ -12,12 +12,13 @@
 // one column ahead by direct-to-LDS loads, computes the x-face HLLD +
 // resistive flux, the y-face flux of the previous column, the corner EMF,
 // and writes the updated cell two columns behind the load front.  The
-// update's own operands come from LDS too: stage 1 keeps four staged
-// columns, so the cell two columns back is still there; stage 2 stages
-// that cell's U* and U^n while the y-face flux is computed.  The next
-// column's loads are issued after the x-face flux: a spill reload waits for
-// every older vector memory load, so loads issued at the top of the column
-// would be waited for a few hundred instructions later.
+// update's own operands come from LDS too: the cell's stage input (and in
+// stage 2 U^n) is staged while the y-face flux is computed, and the
+// previous column's face fluxes wait in per-lane LDS slots instead of
+// registers held across both HLLD solves.  The next column's loads are
+// issued after the x-face flux: a spill reload waits for every older vector
+// memory load, so loads issued at the top of the column would be waited for
+// a few hundred instructions later.
 //
 // Reference (paths relative to its root,
 // P = ResistiveMHD_singleGPU_thrust/lib_ResistiveMHD_2D_GPU_periodicXY,
@@ -186,18 +187,21 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     // Full strips stage each column in LDS one column ahead with direct
     // loads (global_load_lds), so a column's HBM latency overlaps the
     // previous column's arithmetic instead of stalling the wavefront.
-    // LDS per wavefront: 19,008 B in stage 1 and 16,896 B in stage 2, so
+    // LDS per wavefront: 20,384 B in stage 1 and 19,968 B in stage 2, so
     // eight wavefronts (two per SIMD, the VGPR limit) fit a CU's 160 KiB.
     constexpr bool DMA = !TAIL;
     // stage 1 keeps four columns, so that the update of column c-2 reads
     // its conserved state from LDS
-    constexpr int kSlots = (STAGE == 1 && !MULTI) ? 4 : 2;
+    constexpr int kSlots = 2;
     __shared__ __attribute__((aligned(16))) double ring[kSlots][kStagePlanes][kStageRows];
     const int r0 = j0 - 3;
     const int rpair = min(r0 + 2 * lane, (g.ny - 2) & ~1);
     auto stage_col = [&](int col) {
         if (lane < kStageRows / 2) {
-            const size_t off = (size_t)col * g.pitch + rpair;
+            // opaque, so the lane offset is not folded into one loop-invariant
+            // 64-bit pointer per plane (VGPR pairs the loop would have to keep)
+            size_t off = (size_t)col * g.pitch + rpair;
+            asm volatile("" : "+v"(off));
             double(*slot)[kStageRows] = ring[col & (kSlots - 1)];
 #pragma unroll
             for (int v = 0; v < 8; ++v)
@@ -206,19 +210,28 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             if (etaf) __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + off), (lds_void_t *)slot[8], 16, 0, 0);
         }
     };
-    // stage 2: U* and U^n of the cell being updated, staged before the
-    // y-face flux of the same column
-    constexpr bool UPD_DMA = DMA && STAGE == 2 && !MULTI;
-    __shared__ __attribute__((aligned(16))) double upd[UPD_DMA ? 14 : 1][kStageRows];
+    // the stage input (and in stage 2 U^n) of the cell being updated,
+    // staged before the y-face flux of the same column
+    constexpr bool UPD_DMA = DMA && !MULTI;
+    __shared__ __attribute__((aligned(16))) double upd[UPD_DMA ? (STAGE == 2 ? 14 : 8) : 1][kStageRows];
+    // the previous column's face fluxes (the components the update reads),
+    // one slot per lane, instead of registers held across both HLLD solves
+    constexpr bool FLDS = DMA && !MULTI;
+    constexpr bool GLDS = DMA && !MULTI && STAGE == 1;
+    __shared__ double fb[FLDS ? 6 : 1][64];
+    __shared__ double gb[GLDS ? 7 : 1][64];
     auto stage_upd = [&](int col) {
         if (lane < kStageRows / 2) {
-            const size_t off = (size_t)col * g.pitch + rpair;
+            // opaque, so the lane offset is not folded into one loop-invariant
+            // 64-bit pointer per plane (VGPR pairs the loop would have to keep)
+            size_t off = (size_t)col * g.pitch + rpair;
+            asm volatile("" : "+v"(off));
 #pragma unroll
             for (int v = 0; v < 8; ++v)
                 __builtin_amdgcn_global_load_lds((glb_void_t *)(uin + (size_t)v * g.plane + off),
                                                  (lds_void_t *)upd[v], 16, 0, 0);
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
+            for (int k = 0; k < (STAGE == 2 ? 6 : 0); ++k) {
                 const int v = k < 4 ? k : k + 2;
                 __builtin_amdgcn_global_load_lds((glb_void_t *)(uold + (size_t)v * g.plane + off),
                                                  (lds_void_t *)upd[8 + k], 16, 0, 0);
@@ -401,19 +414,29 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             double E = 0.0;
             if (!MULTI && STAGE == 1 && c >= i0 + 1) {
                 const double f5up = lane_next(F.f5);
-                E = 0.25 * (Gprev.f4 + G.f4 + (-1.0 * F.f5) + (-1.0 * f5up));
+                const double gp4 = GLDS ? gb[4][lane] : Gprev.f4;
+                E = 0.25 * (gp4 + G.f4 + (-1.0 * F.f5) + (-1.0 * f5up));
             }
 
             if (c >= i0 + 2) {
                 const int s = c - 2;
-                const double g0m = lane_prev(Gprev.f0), g1m = lane_prev(Gprev.f1);
-                const double g2m = lane_prev(Gprev.f2), g3m = lane_prev(Gprev.f3);
-                const double g6m = lane_prev(Gprev.f6), g7m = lane_prev(Gprev.f7);
+                Flux8 Fp = Fprev, Gp = Gprev;
+                if (FLDS) {
+                    Fp.f0 = fb[0][lane]; Fp.f1 = fb[1][lane]; Fp.f2 = fb[2][lane];
+                    Fp.f3 = fb[3][lane]; Fp.f6 = fb[4][lane]; Fp.f7 = fb[5][lane];
+                }
+                if (GLDS) {
+                    Gp.f0 = gb[0][lane]; Gp.f1 = gb[1][lane]; Gp.f2 = gb[2][lane]; Gp.f3 = gb[3][lane];
+                    Gp.f4 = gb[4][lane]; Gp.f6 = gb[5][lane]; Gp.f7 = gb[6][lane];
+                }
+                const double g0m = lane_prev(Gp.f0), g1m = lane_prev(Gp.f1);
+                const double g2m = lane_prev(Gp.f2), g3m = lane_prev(Gp.f3);
+                const double g6m = lane_prev(Gp.f6), g7m = lane_prev(Gp.f7);
                 const double Em = lane_prev(E);
                 if (MULTI) {
                     // face-smoothed B of the stage input (shift + back,
                     // M/resistiveMHD2D.cu:597-690): cells s = A and s+1 = B
-                    const double g4m = lane_prev(Gprev.f4), g5m = lane_prev(Gprev.f5);
+                    const double g4m = lane_prev(Gp.f4), g5m = lane_prev(Gp.f5);
                     const double byc_up = lane_next(A.byc);
                     const double sbx = 0.5 * (A.bxc + B.bxc);
                     const double sby = 0.5 * (A.byc + byc_up);
@@ -435,32 +458,32 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                         double n0, n1, n2, n3, n4, n5, n6, n7;
                         if (STAGE == 1) {
                             // oneStepFirst_kernel, M/resistiveMHD2D.cu:64-103
-                            n0 = Rh[os] - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m);
-                            n1 = Mx[os] - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m);
-                            n2 = My[os] - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m);
-                            n3 = Mz[os] - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m);
-                            n4 = sbx - dtdx * (F.f4 - Fprev.f4) - dtdy * (Gprev.f4 - g4m);
-                            n5 = sby - dtdx * (F.f5 - Fprev.f5) - dtdy * (Gprev.f5 - g5m);
-                            n6 = Bz[os] - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m);
-                            n7 = En[os] - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m);
+                            n0 = Rh[os] - dtdx * (F.f0 - Fp.f0) - dtdy * (Gp.f0 - g0m);
+                            n1 = Mx[os] - dtdx * (F.f1 - Fp.f1) - dtdy * (Gp.f1 - g1m);
+                            n2 = My[os] - dtdx * (F.f2 - Fp.f2) - dtdy * (Gp.f2 - g2m);
+                            n3 = Mz[os] - dtdx * (F.f3 - Fp.f3) - dtdy * (Gp.f3 - g3m);
+                            n4 = sbx - dtdx * (F.f4 - Fp.f4) - dtdy * (Gp.f4 - g4m);
+                            n5 = sby - dtdx * (F.f5 - Fp.f5) - dtdy * (Gp.f5 - g5m);
+                            n6 = Bz[os] - dtdx * (F.f6 - Fp.f6) - dtdy * (Gp.f6 - g6m);
+                            n7 = En[os] - dtdx * (F.f7 - Fp.f7) - dtdy * (Gp.f7 - g7m);
                         } else {
                             // oneStepSecond_kernel, M/resistiveMHD2D.cu:106-144
-                            n0 = 0.5 * (PL(uold, 0)[os] + Rh[os] - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m));
-                            n1 = 0.5 * (PL(uold, 1)[os] + Mx[os] - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m));
-                            n2 = 0.5 * (PL(uold, 2)[os] + My[os] - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m));
-                            n3 = 0.5 * (PL(uold, 3)[os] + Mz[os] - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m));
-                            n4 = 0.5 * (obx + sbx - dtdx * (F.f4 - Fprev.f4) - dtdy * (Gprev.f4 - g4m));
-                            n5 = 0.5 * (oby + sby - dtdx * (F.f5 - Fprev.f5) - dtdy * (Gprev.f5 - g5m));
-                            n6 = 0.5 * (PL(uold, 6)[os] + Bz[os] - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m));
-                            n7 = 0.5 * (PL(uold, 7)[os] + En[os] - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m));
+                            n0 = 0.5 * (PL(uold, 0)[os] + Rh[os] - dtdx * (F.f0 - Fp.f0) - dtdy * (Gp.f0 - g0m));
+                            n1 = 0.5 * (PL(uold, 1)[os] + Mx[os] - dtdx * (F.f1 - Fp.f1) - dtdy * (Gp.f1 - g1m));
+                            n2 = 0.5 * (PL(uold, 2)[os] + My[os] - dtdx * (F.f2 - Fp.f2) - dtdy * (Gp.f2 - g2m));
+                            n3 = 0.5 * (PL(uold, 3)[os] + Mz[os] - dtdx * (F.f3 - Fp.f3) - dtdy * (Gp.f3 - g3m));
+                            n4 = 0.5 * (obx + sbx - dtdx * (F.f4 - Fp.f4) - dtdy * (Gp.f4 - g4m));
+                            n5 = 0.5 * (oby + sby - dtdx * (F.f5 - Fp.f5) - dtdy * (Gp.f5 - g5m));
+                            n6 = 0.5 * (PL(uold, 6)[os] + Bz[os] - dtdx * (F.f6 - Fp.f6) - dtdy * (Gp.f6 - g6m));
+                            n7 = 0.5 * (PL(uold, 7)[os] + En[os] - dtdx * (F.f7 - Fp.f7) - dtdy * (Gp.f7 - g7m));
                         }
                         PL(uout, 0)[os] = n0; PL(uout, 1)[os] = n1; PL(uout, 2)[os] = n2; PL(uout, 3)[os] = n3;
                         PL(uout, 4)[os] = n4; PL(uout, 5)[os] = n5; PL(uout, 6)[os] = n6; PL(uout, 7)[os] = n7;
                         // face fluxes of the CT-contact EMF (M/ct.cu:58-63)
                         PL(aux, 0)[os] = F.f0;
                         PL(aux, 1)[os] = F.f5;
-                        PL(aux, 2)[os] = Gprev.f0;
-                        PL(aux, 3)[os] = Gprev.f4;
+                        PL(aux, 2)[os] = Gp.f0;
+                        PL(aux, 3)[os] = Gp.f4;
                     }
                 } else if (s < send && (upd_row || stale_row)) {
                     const size_t os = (size_t)(s + dcol) * g.pitch + jl;
@@ -469,22 +492,24 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                         // oneStepFirstFunctor (P/resistiveMHD_2D.cu:53-86) and
                         // CT_kernel (P/ct.cu:44-59)
                         double U0, U1, U2, U3, U4, U5, U6, U7;
-                        if (DMA) {
-                            const double(*us)[kStageRows] = ring[s & (kSlots - 1)];
-                            U0 = us[0][lane + 1]; U1 = us[1][lane + 1]; U2 = us[2][lane + 1]; U3 = us[3][lane + 1];
-                            U4 = us[4][lane + 1]; U5 = us[5][lane + 1]; U6 = us[6][lane + 1]; U7 = us[7][lane + 1];
+                        if (UPD_DMA) {
+                            // s_waitcnt vmcnt(0): the cell's staging loads have landed
+                            __builtin_amdgcn_s_waitcnt(0x0F70);
+                            const int r = lane + 1;
+                            U0 = upd[0][r]; U1 = upd[1][r]; U2 = upd[2][r]; U3 = upd[3][r];
+                            U4 = upd[4][r]; U5 = upd[5][r]; U6 = upd[6][r]; U7 = upd[7][r];
                         } else {
                             U0 = Rh[os]; U1 = Mx[os]; U2 = My[os]; U3 = Mz[os];
                             U4 = Bx[os]; U5 = By[os]; U6 = Bz[os]; U7 = En[os];
                         }
-                        n0 = U0 - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m);
-                        n1 = U1 - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m);
-                        n2 = U2 - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m);
-                        n3 = U3 - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m);
+                        n0 = U0 - dtdx * (F.f0 - Fp.f0) - dtdy * (Gp.f0 - g0m);
+                        n1 = U1 - dtdx * (F.f1 - Fp.f1) - dtdy * (Gp.f1 - g1m);
+                        n2 = U2 - dtdx * (F.f2 - Fp.f2) - dtdy * (Gp.f2 - g2m);
+                        n3 = U3 - dtdx * (F.f3 - Fp.f3) - dtdy * (Gp.f3 - g3m);
                         n4 = U4 - dtdy * (E - Em);
                         n5 = U5 + dtdx * (E - Eprev);
-                        n6 = U6 - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m);
-                        n7 = U7 - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m);
+                        n6 = U6 - dtdx * (F.f6 - Fp.f6) - dtdy * (Gp.f6 - g6m);
+                        n7 = U7 - dtdx * (F.f7 - Fp.f7) - dtdy * (Gp.f7 - g7m);
                         if (upd_row) {
                             PL(uout, 0)[os] = n0; PL(uout, 1)[os] = n1; PL(uout, 2)[os] = n2; PL(uout, 3)[os] = n3;
                             PL(uout, 4)[os] = n4; PL(uout, 5)[os] = n5; PL(uout, 6)[os] = n6; PL(uout, 7)[os] = n7;
@@ -514,14 +539,14 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                                 O0 = PL(uold, 0)[os]; O1 = PL(uold, 1)[os]; O2 = PL(uold, 2)[os];
                                 O3 = PL(uold, 3)[os]; O6 = PL(uold, 6)[os]; O7 = PL(uold, 7)[os];
                             }
-                            n0 = 0.5 * (O0 + S0 - dtdx * (F.f0 - Fprev.f0) - dtdy * (Gprev.f0 - g0m));
-                            n1 = 0.5 * (O1 + S1 - dtdx * (F.f1 - Fprev.f1) - dtdy * (Gprev.f1 - g1m));
-                            n2 = 0.5 * (O2 + S2 - dtdx * (F.f2 - Fprev.f2) - dtdy * (Gprev.f2 - g2m));
-                            n3 = 0.5 * (O3 + S3 - dtdx * (F.f3 - Fprev.f3) - dtdy * (Gprev.f3 - g3m));
+                            n0 = 0.5 * (O0 + S0 - dtdx * (F.f0 - Fp.f0) - dtdy * (Gp.f0 - g0m));
+                            n1 = 0.5 * (O1 + S1 - dtdx * (F.f1 - Fp.f1) - dtdy * (Gp.f1 - g1m));
+                            n2 = 0.5 * (O2 + S2 - dtdx * (F.f2 - Fp.f2) - dtdy * (Gp.f2 - g2m));
+                            n3 = 0.5 * (O3 + S3 - dtdx * (F.f3 - Fp.f3) - dtdy * (Gp.f3 - g3m));
                             n4 = S4;
                             n5 = S5;
-                            n6 = 0.5 * (O6 + S6 - dtdx * (F.f6 - Fprev.f6) - dtdy * (Gprev.f6 - g6m));
-                            n7 = 0.5 * (O7 + S7 - dtdx * (F.f7 - Fprev.f7) - dtdy * (Gprev.f7 - g7m));
+                            n6 = 0.5 * (O6 + S6 - dtdx * (F.f6 - Fp.f6) - dtdy * (Gp.f6 - g6m));
+                            n7 = 0.5 * (O7 + S7 - dtdx * (F.f7 - Fp.f7) - dtdy * (Gp.f7 - g7m));
                             PL(uout, 0)[os] = n0; PL(uout, 1)[os] = n1; PL(uout, 2)[os] = n2; PL(uout, 3)[os] = n3;
                             PL(uout, 4)[os] = n4; PL(uout, 5)[os] = n5; PL(uout, 6)[os] = n6; PL(uout, 7)[os] = n7;
                             // fused epilogue: per-cell dt of the new state for
@@ -535,8 +560,18 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     }
                 }
             }
-            Fprev = F;
-            Gprev = G;
+            if (FLDS) {
+                fb[0][lane] = F.f0; fb[1][lane] = F.f1; fb[2][lane] = F.f2;
+                fb[3][lane] = F.f3; fb[4][lane] = F.f6; fb[5][lane] = F.f7;
+            } else {
+                Fprev = F;
+            }
+            if (GLDS) {
+                gb[0][lane] = G.f0; gb[1][lane] = G.f1; gb[2][lane] = G.f2; gb[3][lane] = G.f3;
+                gb[4][lane] = G.f4; gb[5][lane] = G.f6; gb[6][lane] = G.f7;
+            } else {
+                Gprev = G;
+            }
             Eprev = E;
         }
         A = B;
