@@ -530,6 +530,56 @@ def plain_bench(mhd, torch, root, workload, nx, ny, steps, warmup, numerics, swe
     }
 
 
+def fast_vs_exact(mhd, steps=100):
+    """Fast arithmetic against exact (bitwise the oracle) after `steps` steps
+    of the plain-method case (Orszag-Tang 256^2, eta 1e-3) and of the
+    configured Harris case (4096 x 2048, eta field): relative L2 of the
+    interior state per conserved variable, relative differences of dt and
+    time, and the faces of the final exact state whose HLLD region differs
+    under fast arithmetic (mhd_face_regions)."""
+    import numpy as np
+    names = ("rho", "rhoU", "rhoV", "rhoW", "bX", "bY", "bZ", "e")
+    g = mhd.GHOST
+    cases = [
+        ("orszag_tang_256_eta1e-3", mhd.orszag_tang_config(256, eta=1e-3), lambda sim: sim.initOrszagTang()),
+        ("harris_4096x2048_eta_field", mhd.harris_config(4096, 2048, ly=harris_ly(4096, 2048)),
+         lambda sim: sim.initHarris(seed=0, noise_amp=1e-3, eta0=1e-3, eta1=5e-3)),
+    ]
+    out = {}
+    for tag, cfg, init in cases:
+        sims = {}
+        for mode in ("exact", "fast"):
+            sim = mhd.ResistiveMHD2D(cfg)
+            sim.set_numerics(mode)
+            init(sim)
+            sim.steps(steps)
+            sim.synchronize()
+            sims[mode] = sim
+        Ue = sims["exact"].getU()[g:-g, g:-g]
+        Uf = sims["fast"].getU()[g:-g, g:-g]
+        num = np.sqrt(((Uf - Ue) ** 2).reshape(-1, 8).sum(axis=0))
+        den = np.sqrt((Ue ** 2).reshape(-1, 8).sum(axis=0))
+        rel = num / np.where(den == 0.0, 1.0, den)
+        rx, ry = sims["exact"].face_regions("exact")
+        fx, fy = sims["exact"].face_regions("fast")
+        ex, ey, qx, qy = rx[2:-3, 3:-3], ry[3:-3, 2:-3], fx[2:-3, 3:-3], fy[3:-3, 2:-3]
+        de, df = sims["exact"].dt, sims["fast"].dt
+        te, tf = sims["exact"].totalTime, sims["fast"].totalTime
+        out[tag] = {
+            "steps": steps,
+            "rel_l2": {n: float(r) for n, r in zip(names, rel)},
+            "rel_l2_max": float(rel.max()),
+            "dt_rel_diff": abs(df - de) / de,
+            "time_rel_diff": abs(tf - te) / te,
+            "faces": int(ex.size + ey.size),
+            "region_flips_x": int((ex != qx).sum()),
+            "region_flips_y": int((ey != qy).sum()),
+            "crashed": [bool(sims[m].checkCalculationIsCrashed()) for m in ("exact", "fast")],
+        }
+        del sims
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -631,6 +681,12 @@ def main():
         # the headline run of the configured Harris case, 4096 x 2048 cells
         line["harris_4096x2048"] = plain_bench(mhd, torch, root, "harris", 4096, 2048, args.steps, args.warmup,
                                                args.numerics, args.sweep)
+    if not multi and (args.nx, args.ny) == (4096, 4096) and not args.no_extra:
+        # fast arithmetic: its throughput on the headline grid and how far it
+        # is from exact (the oracle's arithmetic) on the two reference cases
+        fast = plain_bench(mhd, torch, root, "harris", 4096, 4096, args.steps, args.warmup, "fast", args.sweep)
+        line["fast_vs_exact"] = {"fast_4096x4096": {k: fast[k] for k in ("value", "ms_per_step", "roofline")},
+                                 **fast_vs_exact(mhd)}
     if args.calibrate_copy_mib:
         n = args.calibrate_copy_mib * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")

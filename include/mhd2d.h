@@ -212,7 +212,10 @@ int mhd_get_scheme(const mhd_handle *h, int *scheme);
  * code path is what MHD_NUMERICS_EXACT reproduces):
  *  - MHD_NUMERICS_EXACT (default): the reference's expressions operation for
  *    operation, with IEEE division and square root and no contraction;
- *    results are bitwise equal to the reference arithmetic on the same inputs.
+ *    results are bitwise equal to an uncontracted IEEE double restatement of
+ *    those expressions (the CPU oracle) on the same inputs.  A reference
+ *    build whose compiler contracts a*b+c into FMAs (nvcc does by default)
+ *    rounds differently, so it agrees with EXACT to rounding only.
  *  - MHD_NUMERICS_FAST: shared reciprocals, squared wave speeds, FMA
  *    contraction.  It differs from EXACT by rounding, but the reference's HLLD
  *    takes the physical flux of each star state, which jumps where a
@@ -222,6 +225,16 @@ int mhd_get_scheme(const mhd_handle *h, int *scheme);
 enum mhd_numerics { MHD_NUMERICS_FAST = 0, MHD_NUMERICS_EXACT = 1 };
 int mhd_set_numerics(mhd_handle *h, int mode);
 int mhd_get_numerics(const mhd_handle *h, int *mode);
+
+/* Diagnostic (no counterpart in the reference): the HLLD region of every
+ * face of the current state under the arithmetic `mode`, from the stage
+ * kernel's reconstruction.  host_rx[j + i*ny] is the face (i+1/2, j),
+ * host_ry[j + i*ny] the face (i, j+1/2), each nx*ny bytes; bits 0-5 are the
+ * indicators of P/hlld.cu:64-69 (outer-left, single-star left, double-star
+ * left, outer-right, single-star right, double-star right), 0xFF marks faces
+ * no update reads.  Counting faces whose bits differ between the two modes
+ * measures where FAST's rounding changes the HLLD branch. */
+int mhd_face_regions(mhd_handle *h, int mode, unsigned char *host_rx, unsigned char *host_ry);
 
 /* Launch-shape control (segment length along x of one wave's sweep; 0 picks
  * the built-in default).  Exposed for the autotuner and the tests. */

@@ -820,6 +820,58 @@ void oracle_get_stage1_face_b(const struct oracle_sim *s, double *bx, double *by
     memcpy(by, s->bYStage1, sizeof(double) * s->N);
 }
 
+/* HLLD region indicators (P/hlld.cu:64-69) of every face of the current
+ * state, as bits: outer-left, single-star left, double-star left,
+ * outer-right, single-star right, double-star right.  The faces are those
+ * the next stage solves: the state is centred (shift_to_center) and
+ * reconstructed as in hlld_flux_x / hlld_flux_y.  rx[k]: face between
+ * cells k and k+ny; ry[k]: face between k and k+1 (0 where the reference's
+ * ranges leave a face unset).  Test hook for the numerics comparison. */
+static unsigned char region_bits(const HlldParams *P)
+{
+    return (unsigned char)((P->SL > 0.0) | ((P->SL <= 0.0) && (0.0 < P->S1L)) << 1
+                           | ((P->S1L <= 0.0) && (0.0 < P->SM)) << 2 | (P->SR <= 0.0) << 3
+                           | ((P->S1R <= 0.0) && (0.0 < P->SR)) << 4 | ((P->SM <= 0.0) && (0.0 < P->S1R)) << 5);
+}
+
+void oracle_face_regions(struct oracle_sim *s, unsigned char *rx, unsigned char *ry)
+{
+    int N = s->N, ny = s->ny;
+    Cons *c = (Cons *)malloc(sizeof(Cons) * N);
+    memcpy(c, s->U, sizeof(Cons) * N);
+    shift_to_center(s, c);
+    memset(rx, 0, N);
+    memset(ry, 0, N);
+    for (int k = 0; k < N - ny; ++k)
+        s->dQCenter[k] = basic_param(&c[k], &c[k + ny], s->gamma);
+    for (int k = ny; k < N - ny; ++k)
+        s->dQLeft[k] = left_param(&s->dQCenter[k - ny], &s->dQCenter[k], &s->dQCenter[k + ny], s->eps);
+    for (int k = 0; k < N - 2 * ny; ++k)
+        s->dQRight[k] = right_param(&s->dQCenter[k], &s->dQCenter[k + ny], &s->dQCenter[k + 2 * ny], s->eps);
+    for (int k = ny; k < N - 2 * ny; ++k) {
+        HlldParams P = hlld_params(&s->dQLeft[k], &s->dQRight[k], s->gamma, s->eps);
+        rx[k] = region_bits(&P);
+    }
+    Cons *t = s->tmpUForFluxG;
+    for (int k = 0; k < N; ++k) {
+        t[k].rho = c[k].rho;
+        t[k].rhoU = c[k].rhoV; t[k].rhoV = c[k].rhoW; t[k].rhoW = c[k].rhoU;
+        t[k].bX = c[k].bY; t[k].bY = c[k].bZ; t[k].bZ = c[k].bX;
+        t[k].e = c[k].e;
+    }
+    for (int k = 0; k < N - 1; ++k)
+        s->dQCenter[k] = basic_param(&t[k], &t[k + 1], s->gamma);
+    for (int k = 1; k < N - 1; ++k)
+        s->dQLeft[k] = left_param(&s->dQCenter[k - 1], &s->dQCenter[k], &s->dQCenter[k + 1], s->eps);
+    for (int k = 0; k < N - 2; ++k)
+        s->dQRight[k] = right_param(&s->dQCenter[k], &s->dQCenter[k + 1], &s->dQCenter[k + 2], s->eps);
+    for (int k = 1; k < N - 2; ++k) {
+        HlldParams P = hlld_params(&s->dQLeft[k], &s->dQRight[k], s->gamma, s->eps);
+        ry[k] = region_bits(&P);
+    }
+    free(c);
+}
+
 /* The CT-contact corner EMF of one flux set (m_emf) on a 2 x 2 patch of
  * cells (i, j), (i, j+1), (i+1, j), (i+1, j+1), each array in that order:
  * face fluxes NF5 (x-face F.f5), NG4 (y-face G.f4), NF0, NG0 (mass fluxes)

@@ -17,6 +17,7 @@ void oracle_destroy(struct oracle_sim *s);
 void oracle_set_state(struct oracle_sim *s, const double *aos);
 void oracle_get_state(const struct oracle_sim *s, double *aos);
 void oracle_get_stage1_face_b(const struct oracle_sim *s, double *bx, double *by);
+void oracle_face_regions(struct oracle_sim *s, unsigned char *rx, unsigned char *ry);
 double oracle_ct_contact_emf(const double *NF5, const double *NG4, const double *F5, const double *G4,
                              const double *NF0, const double *NG0);
 int oracle_set_eta_field(struct oracle_sim *s, const double *eta);

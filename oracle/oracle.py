@@ -11,10 +11,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 TWIN_PATH = os.path.join(HERE, "libthrust_twin.so")
+# the same restatement compiled with FMA contraction (a measurement of what
+# contraction alone changes; never the reference arithmetic)
+FMA_PATH = os.path.join(HERE, "liboracle_fma.so")
 BC_PERIODIC, BC_SYMMETRIC = 0, 1
 SCHEME_SINGLE, SCHEME_MULTI_SHARE = 0, 1
 
-_lib = None
+_libs = {}
 
 
 def build():
@@ -23,13 +26,12 @@ def build():
     return LIB_PATH
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        build()
-    lib = ctypes.CDLL(LIB_PATH)
+def load(path=LIB_PATH):
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        subprocess.run(["make", "-s", "-C", HERE, os.path.basename(path)], check=True)
+    lib = ctypes.CDLL(path)
     D = ctypes.POINTER(ctypes.c_double)
     P = ctypes.c_void_p
     c_d, c_i = ctypes.c_double, ctypes.c_int
@@ -39,6 +41,7 @@ def load():
         "oracle_set_state": (None, [P, D]),
         "oracle_get_state": (None, [P, D]),
         "oracle_get_stage1_face_b": (None, [P, D, D]),
+        "oracle_face_regions": (None, [P, ctypes.c_void_p, ctypes.c_void_p]),
         "oracle_ct_contact_emf": (c_d, [D, D, D, D, D, D]),
         "oracle_set_eta_field": (c_i, [P, D]),
         "oracle_apply_boundary": (None, [P]),
@@ -63,7 +66,7 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -76,8 +79,8 @@ class OracleSim:
     HIP binding: (nx, ny, 8) float64, ghosts included)."""
 
     def __init__(self, nx, ny, dx, dy, gamma=5.0 / 3.0, cfl=0.7, eps=1e-20, eta=0.0,
-                 bc_x=BC_PERIODIC, bc_y=BC_PERIODIC):
-        self._lib = load()
+                 bc_x=BC_PERIODIC, bc_y=BC_PERIODIC, lib_path=LIB_PATH):
+        self._lib = load(lib_path)
         self.nx, self.ny = nx, ny
         self.gamma, self.eps, self.dx, self.dy = gamma, eps, dx, dy
         self._s = self._lib.oracle_create(nx, ny, dx, dy, gamma, cfl, eps, eta, bc_x, bc_y)
@@ -85,8 +88,9 @@ class OracleSim:
             raise ValueError("oracle_create failed (nx, ny must be >= 12)")
 
     @classmethod
-    def from_config(cls, cfg):
-        return cls(cfg.nx, cfg.ny, cfg.dx, cfg.dy, cfg.gamma, cfg.cfl, cfg.eps, cfg.eta, cfg.bc_x, cfg.bc_y)
+    def from_config(cls, cfg, lib_path=LIB_PATH):
+        return cls(cfg.nx, cfg.ny, cfg.dx, cfg.dy, cfg.gamma, cfg.cfl, cfg.eps, cfg.eta, cfg.bc_x, cfg.bc_y,
+                   lib_path=lib_path)
 
     def __del__(self):
         if getattr(self, "_s", None):
@@ -110,6 +114,16 @@ class OracleSim:
         by = np.empty((self.nx, self.ny))
         self._lib.oracle_get_stage1_face_b(self._s, _p(bx), _p(by))
         return bx, by
+
+    def face_regions(self):
+        """HLLD region bits of every face of the current state, (rx, ry) as
+        (nx, ny) uint8 arrays: rx[i, j] the face (i+1/2, j), ry[i, j] the
+        face (i, j+1/2); bits outer-left, single-star left, double-star left,
+        outer-right, single-star right, double-star right."""
+        rx = np.zeros((self.nx, self.ny), dtype=np.uint8)
+        ry = np.zeros((self.nx, self.ny), dtype=np.uint8)
+        self._lib.oracle_face_regions(self._s, rx.ctypes.data, ry.ctypes.data)
+        return rx, ry
 
     def set_eta_field(self, eta):
         if eta is None:

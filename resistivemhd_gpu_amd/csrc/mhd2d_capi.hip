@@ -847,6 +847,23 @@ int mhd_set_numerics(mhd_handle *h, int mode)
     return MHD_OK;
 }
 
+int mhd_face_regions(mhd_handle *h, int mode, unsigned char *host_rx, unsigned char *host_ry)
+{
+    if (!h || !host_rx || !host_ry) return fail(h, MHD_ERR_INVALID, "NULL argument");
+    if (mode != MHD_NUMERICS_FAST && mode != MHD_NUMERICS_EXACT)
+        return fail(h, MHD_ERR_INVALID, "mode must be MHD_NUMERICS_FAST or MHD_NUMERICS_EXACT");
+    const size_t n = (size_t)h->cfg.nx * h->cfg.ny;
+    unsigned char *d = nullptr;
+    if (hipMalloc(&d, 2 * n) != hipSuccess) return fail(h, MHD_ERR_NOMEM, "face region buffers");
+    hipError_t e = mhd::launch_face_regions(mode == MHD_NUMERICS_EXACT, h->U, d, d + n, h->g, h->ph, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_rx, d, n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(host_ry, d + n, n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d);
+    if (e != hipSuccess) return hip_fail(h, e, "mhd_face_regions");
+    return MHD_OK;
+}
+
 int mhd_get_numerics(const mhd_handle *h, int *mode)
 {
     if (!h || !mode) return fail(nullptr, MHD_ERR_INVALID, "NULL argument");

@@ -147,7 +147,19 @@ struct Face7 {
 // HLLD flux at one interface with normal field bX (both sides share it):
 // parameters P/hlld.cu:256-433, candidates P/hlld.cu:469-549, selection as
 // the sum of six indicator-weighted candidates P/hlld.cu:35-107.
-__device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, double bX, const Phys &ph)
+// Region indicators of an interface as bits: outer-left, single-star left,
+// double-star left, outer-right, single-star right, double-star right
+// (P/hlld.cu:64-69).
+__device__ __forceinline__ int region_bits(double SL, double S1L, double SM, double S1R, double SR)
+{
+    return (int)(SL > 0.0) | (int)((SL <= 0.0) && (0.0 < S1L)) << 1 | (int)((S1L <= 0.0) && (0.0 < SM)) << 2
+         | (int)(SR <= 0.0) << 3 | (int)((S1R <= 0.0) && (0.0 < SR)) << 4 | (int)((SM <= 0.0) && (0.0 < S1R)) << 5;
+}
+
+// region: if not null, receives region_bits of the interface (a diagnostic;
+// the stage kernels pass null and the store folds away).
+__device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, double bX, const Phys &ph,
+                                            int *region = nullptr)
 {
     const double gamma = ph.gamma, eps = ph.eps;
     const double rhoL = Lq.rho, uL = Lq.u, vL = Lq.v, wL = Lq.w, bXL = bX, bYL = Lq.by, bZL = Lq.bz, pL = Lq.p;
@@ -237,6 +249,7 @@ __device__ __forceinline__ Flux8 hlld_exact(const Face7 &Lq, const Face7 &Rq, do
     const bool in_oR = (SR <= 0.0);
     const bool in_mR = (S1R <= 0.0) && (0.0 < SR);
     const bool in_iR = (SM <= 0.0) && (0.0 < S1R);
+    if (region) *region = region_bits(SL, S1L, SM, S1R, SR);
 
     const double sq1L = sqrt(rho1L), sq1R = sqrt(rho1R);
     const double sbx = sgn(bX1L);
@@ -371,7 +384,7 @@ __device__ __forceinline__ double sqrt_fast(double x)
 }
 
 __device__ __forceinline__ Flux8 hlld_fast(const Face7 &L, const Face7 &R, double bx, const Phys &ph,
-                                           double inv_gm1)
+                                           double inv_gm1, int *region = nullptr)
 {
 #pragma clang fp contract(fast)
     const double gamma = ph.gamma, eps = ph.eps;
@@ -422,6 +435,7 @@ __device__ __forceinline__ Flux8 hlld_fast(const Face7 &L, const Face7 &R, doubl
     const double abx = fabs(bx);
     const double S1L = SM - abx * isqL;
     const double S1R = SM + abx * isqR;
+    if (region) *region = region_bits(SL, S1L, SM, S1R, SR);
 
     // double-star states
     const double s = sgn(bx);
@@ -476,10 +490,11 @@ __device__ __forceinline__ double cell_dt_fast(double rho, double mx, double my,
 }
 
 template <bool EXACT>
-__device__ __forceinline__ Flux8 hlld(const Face7 &L, const Face7 &R, double bx, const Phys &ph, double inv_gm1)
+__device__ __forceinline__ Flux8 hlld(const Face7 &L, const Face7 &R, double bx, const Phys &ph, double inv_gm1,
+                                      int *region = nullptr)
 {
-    if (EXACT) return hlld_exact(L, R, bx, ph);
-    return hlld_fast(L, R, bx, ph, inv_gm1);
+    if (EXACT) return hlld_exact(L, R, bx, ph, region);
+    return hlld_fast(L, R, bx, ph, inv_gm1, region);
 }
 
 template <bool EXACT>

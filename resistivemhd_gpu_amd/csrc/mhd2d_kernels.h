@@ -107,5 +107,7 @@ hipError_t launch_dt_all_batch(bool exact, const RunDesc *runs, int nruns, int m
 hipError_t launch_unpack(const double *aos, double *u, const Grid &g, hipStream_t s);
 hipError_t launch_pack(const double *u, double *aos, const Grid &g, hipStream_t s);
 hipError_t launch_scatter_plane(const double *src, double *dst, const Grid &g, hipStream_t s);
+hipError_t launch_face_regions(bool exact, const double *u, unsigned char *rx, unsigned char *ry, const Grid &g,
+                               const Phys &ph, hipStream_t s);
 
 }  // namespace mhd

@@ -1085,6 +1085,83 @@ __global__ __launch_bounds__(256) void scatter_plane_kernel(const double *__rest
     }
 }
 
+// Diagnostic: region_bits of every face the update uses, in one arithmetic,
+// from the stage kernel's reconstruction (cells with centred B, minmod
+// slopes, the face's normal field the mean of the two cells' centred
+// components).  rx[j + i*ny]: face (i+1/2, j); ry[j + i*ny]: face
+// (i, j+1/2); 0xFF where no update reads the face.
+template <bool EXACT>
+__device__ __forceinline__ Cell cell_at(const double *u, const Grid &g, const Phys &ph, int i, int j)
+{
+    const size_t o = (size_t)i * g.pitch + j;
+    const double bxc = 0.5 * (PL(u, 4)[o] + PL(u, 4)[o - g.pitch]);
+    const double byc = 0.5 * (PL(u, 5)[o] + PL(u, 5)[o - 1]);
+    return make_cell<EXACT>(PL(u, 0)[o], PL(u, 1)[o], PL(u, 2)[o], PL(u, 3)[o], bxc, byc, PL(u, 6)[o],
+                            PL(u, 7)[o], ph);
+}
+
+// Left and right states of the face between cells a and b, with m before a
+// and p after b, in the frame given by the field order.
+__device__ __forceinline__ void face_states(const double (&m)[7], const double (&a)[7], const double (&b)[7],
+                                            const double (&p)[7], double eps, Face7 &L, Face7 &R)
+{
+    double l[7], r[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) {
+        l[v] = a[v] + 0.5 * minmod(a[v] - m[v], b[v] - a[v], eps);
+        r[v] = b[v] - 0.5 * minmod(b[v] - a[v], p[v] - b[v], eps);
+    }
+    L = Face7{l[0], l[1], l[2], l[3], l[4], l[5], l[6]};
+    R = Face7{r[0], r[1], r[2], r[3], r[4], r[5], r[6]};
+}
+
+template <bool EXACT>
+__global__ __launch_bounds__(256) void face_region_kernel(const double *__restrict__ u, Grid g, Phys ph,
+                                                          unsigned char *__restrict__ rx,
+                                                          unsigned char *__restrict__ ry)
+{
+    const double inv_gm1 = 1.0 / (ph.gamma - 1.0);
+    const long long total = (long long)g.nx * g.ny;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / g.ny), j = (int)(k % g.ny);
+        int cx = 0xFF, cy = 0xFF;
+        if (i >= kGhost - 1 && i <= g.nx - kGhost - 1 && j >= kGhost && j < g.ny - kGhost) {
+            Cell c[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c[q] = cell_at<EXACT>(u, g, ph, i - 1 + q, j);
+            double f[4][7];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double t[7] = {c[q].rho, c[q].u, c[q].v, c[q].w, c[q].byc, c[q].bz, c[q].px};
+#pragma unroll
+                for (int v = 0; v < 7; ++v) f[q][v] = t[v];
+            }
+            Face7 L, R;
+            face_states(f[0], f[1], f[2], f[3], ph.eps, L, R);
+            hlld<EXACT>(L, R, 0.5 * (c[1].bxc + c[2].bxc), ph, inv_gm1, &cx);
+        }
+        if (i >= kGhost && i < g.nx - kGhost && j >= kGhost - 1 && j <= g.ny - kGhost - 1) {
+            Cell c[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c[q] = cell_at<EXACT>(u, g, ph, i, j - 1 + q);
+            double f[4][7];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                // rotated frame (u,v,w)->(v,w,u), (Bx,By,Bz)->(By,Bz,Bx)
+                const double t[7] = {c[q].rho, c[q].v, c[q].w, c[q].u, c[q].bz, c[q].bxc, c[q].py};
+#pragma unroll
+                for (int v = 0; v < 7; ++v) f[q][v] = t[v];
+            }
+            Face7 L, R;
+            face_states(f[0], f[1], f[2], f[3], ph.eps, L, R);
+            hlld<EXACT>(L, R, 0.5 * (c[1].byc + c[2].byc), ph, inv_gm1, &cy);
+        }
+        rx[k] = (unsigned char)cx;
+        ry[k] = (unsigned char)cy;
+    }
+}
+
 // ------------------------------------------------------------- launchers
 static int grid_for(long long n, int block)
 {
@@ -1289,6 +1366,15 @@ hipError_t launch_unpack(const double *aos, double *u, const Grid &g, hipStream_
 hipError_t launch_pack(const double *u, double *aos, const Grid &g, hipStream_t s)
 {
     hipLaunchKernelGGL(pack_kernel, dim3(grid_for((long long)g.nx * g.ny, 256)), dim3(256), 0, s, u, aos, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_face_regions(bool exact, const double *u, unsigned char *rx, unsigned char *ry, const Grid &g,
+                               const Phys &ph, hipStream_t s)
+{
+    const dim3 grid(grid_for((long long)g.nx * g.ny, 256));
+    if (exact) hipLaunchKernelGGL(face_region_kernel<true>, grid, dim3(256), 0, s, u, g, ph, rx, ry);
+    else hipLaunchKernelGGL(face_region_kernel<false>, grid, dim3(256), 0, s, u, g, ph, rx, ry);
     return hipGetLastError();
 }
 

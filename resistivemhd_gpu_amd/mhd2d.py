@@ -26,7 +26,7 @@ EXPORTS = (
     "mhd_get_time", "mhd_set_time", "mhd_check_crashed", "mhd_save", "mhd_load", "mhd_synchronize",
     "mhd_fill_orszag_tang", "mhd_fill_harris", "mhd_fill_harris_eta", "mhd_init_orszag_tang",
     "mhd_init_harris", "mhd_set_sweep_length", "mhd_get_sweep_length", "mhd_profile_enable",
-    "mhd_profile_read", "mhd_set_numerics", "mhd_get_numerics", "mhd_fill_turbulence",
+    "mhd_profile_read", "mhd_set_numerics", "mhd_get_numerics", "mhd_face_regions", "mhd_fill_turbulence",
     "mhd_init_turbulence", "mhd_set_scheme", "mhd_get_scheme",
     "mhd_batch_create", "mhd_batch_destroy", "mhd_batch_last_error", "mhd_batch_size", "mhd_batch_set_state",
     "mhd_batch_get_state", "mhd_batch_set_eta_field", "mhd_batch_apply_boundary", "mhd_batch_init_orszag_tang",
@@ -127,6 +127,7 @@ def load_library(path=LIB_PATH):
         "mhd_profile_read": (I, [P, D, D, ctypes.POINTER(I)]),
         "mhd_set_numerics": (I, [P, I]),
         "mhd_get_numerics": (I, [P, ctypes.POINTER(I)]),
+        "mhd_face_regions": (I, [P, I, P, P]),
         "mhd_fill_turbulence": (I, [C, ctypes.c_ulonglong, I, ctypes.c_double, ctypes.c_double, D]),
         "mhd_init_turbulence": (I, [P, ctypes.c_ulonglong, I, ctypes.c_double, ctypes.c_double]),
         "mhd_set_scheme": (I, [P, I]),
@@ -423,6 +424,18 @@ class ResistiveMHD2D:
         v = ctypes.c_int()
         self._ok(self._lib.mhd_get_numerics(self._h, ctypes.byref(v)))
         return "exact" if v.value == MHD_NUMERICS_EXACT else "fast"
+
+    def face_regions(self, mode):
+        """HLLD region bits of every face of the current state under the
+        arithmetic `mode` ("exact"/"fast"): (rx, ry) as (nx, ny) uint8 arrays,
+        rx[i, j] the face (i+1/2, j), ry[i, j] the face (i, j+1/2); 0xFF where
+        no update reads the face."""
+        if isinstance(mode, str):
+            mode = {"fast": MHD_NUMERICS_FAST, "exact": MHD_NUMERICS_EXACT}[mode]
+        rx = np.empty((self.cfg.nx, self.cfg.ny), dtype=np.uint8)
+        ry = np.empty((self.cfg.nx, self.cfg.ny), dtype=np.uint8)
+        self._ok(self._lib.mhd_face_regions(self._h, int(mode), rx.ctypes.data, ry.ctypes.data))
+        return rx, ry
 
     def set_sweep_length(self, columns):
         self._ok(self._lib.mhd_set_sweep_length(self._h, int(columns)))

@@ -151,6 +151,60 @@ def test_hlld_outer_speed_clamp():
 
 
 # --------------------------------------------------------------------- dt
+# HLLD region bits: outer-left, single-star left, double-star left,
+# outer-right, single-star right, double-star right
+R_OL, R_ML, R_IL, R_OR, R_MR, R_IR = 1, 2, 4, 8, 16, 32
+
+# (normal velocity, expected bits) for a uniform state whose normal Alfven
+# speed |Bn|/sqrt(rho) is ca and fast speed cf: with L = R the HLLD speeds
+# are SM = u_n, S1L = u_n - ca, S1R = u_n + ca, SL = min(u_n - cf, 0),
+# SR = max(u_n + cf, 0) (the clamp P/hlld.cu:315-318), so the region is set
+# by where 0 falls among them.  The clamp makes a super-fast flow to the
+# right single-star left (SL = 0 <= 0 < S1L) and one to the left outer-right
+# only (SR = 0).
+REGION_CASES = [
+    (0.0, R_IR), ("+ca/2", R_IL), ("(ca+cf)/2", R_ML), ("2cf", R_ML),
+    ("-ca/2", R_IR), ("-(ca+cf)/2", R_MR), ("-2cf", R_OR),
+]
+
+
+def _speed(tag, ca, cf):
+    return {0.0: 0.0, "+ca/2": 0.5 * ca, "(ca+cf)/2": 0.5 * (ca + cf), "2cf": 2.0 * cf,
+            "-ca/2": -0.5 * ca, "-(ca+cf)/2": -0.5 * (ca + cf), "-2cf": -2.0 * cf}[tag]
+
+
+def region_case_states(nx=16, ny=14, rho=1.0, p=0.6, bx=0.8, by=0.5, bz=0.3):
+    """Uniform states for each REGION_CASES row, with the expected x-face and
+    y-face bits (shared by the GPU diagnostic's test)."""
+    cs2, b2 = GAMMA * p / rho, (bx * bx + by * by + bz * bz) / rho
+
+    def cf(bn):
+        va2 = bn * bn / rho
+        return np.sqrt(0.5 * (cs2 + b2 + np.sqrt((cs2 + b2) ** 2 - 4.0 * cs2 * va2)))
+
+    cax, cay = abs(bx) / np.sqrt(rho), abs(by) / np.sqrt(rho)
+    out = []
+    for (tx, ex), (ty, ey) in zip(REGION_CASES, REGION_CASES[::-1]):
+        u, v = _speed(tx, cax, cf(bx)), _speed(ty, cay, cf(by))
+        out.append((_uniform_state(nx, ny, rho=rho, u=u, v=v, w=0.1, bx=bx, by=by, bz=bz, p=p), ex, ey))
+    return out
+
+
+def face_interiors(rx, ry):
+    """The faces the update reads: x-faces (i+1/2, j) for i in [2, nx-4],
+    j in [3, ny-4]; y-faces (i, j+1/2) for i in [3, nx-4], j in [2, ny-4]."""
+    return rx[2:-3, 3:-3], ry[3:-3, 2:-3]
+
+
+def test_face_regions_uniform_closed_forms():
+    for U, ex, ey in region_case_states():
+        o = OracleSim(U.shape[0], U.shape[1], 0.1, 0.1, GAMMA, 0.7, EPS, 1e-3)
+        o.set_state(U)
+        fx, fy = face_interiors(*o.face_regions())
+        assert np.all(fx == ex), (ex, np.unique(fx))
+        assert np.all(fy == ey), (ey, np.unique(fy))
+
+
 def test_dt_uniform_closed_form():
     for q in _states(3, 50):
         rho, u, v, w, bx, by, bz, p = q
