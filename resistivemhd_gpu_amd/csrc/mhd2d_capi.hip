@@ -859,7 +859,7 @@ int mhd_face_regions(mhd_handle *h, int mode, unsigned char *host_rx, unsigned c
     if (e == hipSuccess) e = hipMemcpyAsync(host_rx, d, n, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(host_ry, d + n, n, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    hipFree(d);
+    (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(h, e, "mhd_face_regions");
     return MHD_OK;
 }

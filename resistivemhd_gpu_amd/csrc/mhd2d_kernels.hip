@@ -186,14 +186,13 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
 
     // Full strips stage each column in LDS one column ahead with direct
     // loads (global_load_lds), so a column's HBM latency overlaps the
-    // previous column's arithmetic instead of stalling the wavefront.
-    // LDS per wavefront: 20,384 B in stage 1 and 19,968 B in stage 2, so
+    // previous column's arithmetic instead of stalling the wavefront.  One
+    // slot serves: a column's slot is read only at the top of its iteration,
+    // and the next column's loads are issued after those reads are done.
+    // LDS per wavefront: 15,632 B in stage 1 and 18,800 B in stage 2, so
     // eight wavefronts (two per SIMD, the VGPR limit) fit a CU's 160 KiB.
     constexpr bool DMA = !TAIL;
-    // stage 1 keeps four columns, so that the update of column c-2 reads
-    // its conserved state from LDS
-    constexpr int kSlots = 2;
-    __shared__ __attribute__((aligned(16))) double ring[kSlots][kStagePlanes][kStageRows];
+    __shared__ __attribute__((aligned(16))) double slot[kStagePlanes][kStageRows];
     const int r0 = j0 - 3;
     const int rpair = min(r0 + 2 * lane, (g.ny - 2) & ~1);
     auto stage_col = [&](int col) {
@@ -202,7 +201,6 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             // 64-bit pointer per plane (VGPR pairs the loop would have to keep)
             size_t off = (size_t)col * g.pitch + rpair;
             asm volatile("" : "+v"(off));
-            double(*slot)[kStageRows] = ring[col & (kSlots - 1)];
 #pragma unroll
             for (int v = 0; v < 8; ++v)
                 __builtin_amdgcn_global_load_lds((glb_void_t *)(uin + (size_t)v * g.plane + off),
@@ -217,7 +215,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     // the previous column's face fluxes (the components the update reads),
     // one slot per lane, instead of registers held across both HLLD solves
     constexpr bool FLDS = DMA && !MULTI;
-    constexpr bool GLDS = DMA && !MULTI && STAGE == 1;
+    constexpr bool GLDS = DMA && !MULTI;
     __shared__ double fb[FLDS ? 6 : 1][64];
     __shared__ double gb[GLDS ? 7 : 1][64];
     auto stage_upd = [&](int col) {
@@ -257,14 +255,18 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
         if (DMA) {
             // s_waitcnt vmcnt(0): this column's staging loads have landed
             __builtin_amdgcn_s_waitcnt(0x0F70);
-            const double(*slot)[kStageRows] = ring[c & (kSlots - 1)];
             rho = slot[0][lane + 1]; mx = slot[1][lane + 1]; my = slot[2][lane + 1]; mz = slot[3][lane + 1];
             bx = slot[4][lane + 1]; by = slot[5][lane + 1]; bz = slot[6][lane + 1]; e = slot[7][lane + 1];
             bym = slot[5][lane];
             eta_c = etaf ? slot[8][lane + 1] : ph.eta;
             // warm-up columns stage the next column at once; the others after
             // their x-face flux
-            if (c < i0) stage_col(c + 1);
+            if (c < i0) {
+                // s_waitcnt lgkmcnt(0): the slot's reads are done before the
+                // next column's loads may land in it
+                __builtin_amdgcn_s_waitcnt(0xC07F);
+                stage_col(c + 1);
+            }
         } else {
             const size_t o = (size_t)(TAIL ? min(c + dcol, g.nx - 1) : c) * g.pitch + jl;
             rho = Rh[o]; mx = Mx[o]; my = My[o]; mz = Mz[o];
@@ -355,7 +357,10 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             rJY = eJY; rJZ = eJZ; rJYBZ = eJYBZ; rJZBY = eJZBY;
             rTA = tA; rTB = tB;
 
-            if (DMA && c + 1 <= iend + 1) stage_col(c + 1);
+            if (DMA && c + 1 <= iend + 1) {
+                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0), as above
+                stage_col(c + 1);
+            }
             if (UPD_DMA && c >= i0 + 2 && c - 2 < send) stage_upd(c - 2);
 
             // y-face flux of column c-1 at faces j+1/2 (HLLD::calculateFluxG,
