@@ -331,8 +331,10 @@ int mhd_batch_autotune(mhd_batch *b, int steps, int *columns);
  * ghosts included), with every step's dt held at its recorded value
  * ("frozen dt").  With frozen_dt = 1 the forward replays dt values set by
  * mhd_adjoint_set_dts instead of the CFL ones (for finite-difference checks
- * with the same dt sequence).  Exact numerics only; periodic x with periodic
- * or symmetric y (MHD_ERR_INVALID otherwise).  States use the AoS layout of
+ * with the same dt sequence).  Exact numerics only; periodic or symmetric
+ * boundaries in each direction (the symmetric-x fill's ghost Bx rebuild,
+ * S/boundary.cu:56-88, is differentiated through the bottom boundary face
+ * By that the CT leaves for it).  States use the AoS layout of
  * mhd_set_state. */
 typedef struct mhd_adjoint mhd_adjoint;
 int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out);

@@ -28,9 +28,13 @@
 // from a primal recomputed with IEEE arithmetic on every branch input, so
 // the adjoint differentiates the branch the forward step took.
 // Contributions of faces to cells are summed with FP64 atomics (the
-// gradient's last bits depend on their order).  Boundaries: periodic x with periodic or symmetric
-// y (the symmetric-x fill reads a stale CT face value and is not
-// differentiated).
+// gradient's last bits depend on their order).  Boundaries: periodic or
+// symmetric in each direction.  The symmetric-x fill rebuilds the ghost Bx
+// from By of the bottom boundary face row 2 as the stage's CT left it, before
+// the y fill overwrites that row: the forward keeps the CT value there (and
+// stage 2 copies it into U^{n+1}), and the CT adjoint, which covers every
+// face with i, j >= 1, takes up the adjoint the x-fill transpose leaves in
+// row 2.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
@@ -54,7 +58,7 @@ namespace {
 struct AdjDesc {
     Grid g;
     Phys ph;
-    int bc_y;
+    int bc_x, bc_y;
     int ntj;                      // 64-row tiles per column
     double *u, *us;               // U^n and U* (the forward batch's buffers)
     const double *eta;            // per-cell resistivity or nullptr (uniform ph.eta)
@@ -282,9 +286,17 @@ __global__ __launch_bounds__(256) void bct_y_kernel(const AdjDesc *runs, int whi
     }
 }
 
-// Transpose of the periodic x fill (P/boundary.cu:4-71); blockIdx.y = run.
-// One ghost cell per lane, as the forward fill: wavefront w takes ghost
-// column w % 6 (0, 1, 2, nx-3, nx-2, nx-1) and rows (w / 6) * 64 .. + 63.
+// Transpose of the x fill (periodic: P/boundary.cu:4-71; symmetric:
+// S/boundary.cu:4-104); blockIdx.y = run.  One ghost cell per lane, as the
+// forward fill: wavefront w takes ghost column w % 6 (0, 1, 2, nx-3, nx-2,
+// nx-1) and rows (w / 6) * 64 .. + 63.
+// Symmetric x: the Bx rebuild of row j's left (right) ghost faces is
+// transposed by its q = 0 (q = 3) lane.  The rebuild reads By of rows j and
+// j-1 in columns 3, 4 (nx-4, nx-5, nx-6); at j = 3 that is the bottom
+// boundary face By(i, 2) as the stage's CT left it (the y fill overwrites it
+// afterwards), so its adjoint stays in row 2 for the CT adjoint to take up.
+// By adds are atomic: neighbouring rows' chains and the mirror copies share
+// those cells.
 __global__ __launch_bounds__(256) void bct_x_kernel(const AdjDesc *runs, int which)
 {
     const AdjDesc &d = runs[blockIdx.y];
@@ -296,11 +308,61 @@ __global__ __launch_bounds__(256) void bct_x_kernel(const AdjDesc *runs, int whi
     const int nx = d.g.nx;
     const size_t P = d.g.pitch;
     const size_t dst = (size_t)(q < 3 ? q : nx - 6 + q) * P + j;   // columns 0, 1, 2, nx-3, nx-2, nx-1
-    const size_t src = (size_t)(q < 3 ? nx - 6 + q : q) * P + j;   // columns nx-6 .. nx-4, 3 .. 5
+    if (d.bc_x == MHD_BC_PERIODIC) {
+        const size_t src = (size_t)(q < 3 ? nx - 6 + q : q) * P + j;   // columns nx-6 .. nx-4, 3 .. 5
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+            L[v * pl + src] += L[v * pl + dst];
+            L[v * pl + dst] = 0.0;
+        }
+        return;
+    }
+    const size_t src = (size_t)(q < 3 ? 5 - q : nx - 1 - q) * P + j;   // columns 5, 4, 3, nx-4, nx-5, nx-6
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-        L[v * pl + src] += L[v * pl + dst];
+        if (v == 4) continue;
+        const double b = L[v * pl + dst];
         L[v * pl + dst] = 0.0;
+        if (v == 5) unsafeAtomicAdd(&L[5 * pl + src], b);
+        else L[v * pl + src] += b;
+    }
+    if (q != 0 && q != 3) return;
+    double *bx = L + 4 * pl;
+    double *by = L + 5 * pl;
+    const double k = d.ph.dx / d.ph.dy;
+    if (q == 0) {
+        // columns 2, 1, 0: b2 = bx3 + k d3, b1 = b2 + k d3, b0 = b1 + k d4,
+        // d_c = By(c, j) - By(c, j-1) (row 0: b0 = b1 = b2 = bx3)
+        double l0 = bx[j], l1 = bx[P + j], l2 = bx[2 * P + j];
+        bx[j] = bx[P + j] = bx[2 * P + j] = 0.0;
+        l1 += l0;
+        l2 += l1;
+        bx[3 * P + j] += l2;
+        if (j > 0) {
+            const double ld4 = k * l0, ld3 = k * l1 + k * l2;
+            unsafeAtomicAdd(&by[3 * P + j], ld3);
+            unsafeAtomicAdd(&by[3 * P + j - 1], -ld3);
+            unsafeAtomicAdd(&by[4 * P + j], ld4);
+            unsafeAtomicAdd(&by[4 * P + j - 1], -ld4);
+        }
+    } else {
+        // columns nx-3, nx-2, nx-1: bA = bx(nx-4) - k dA, bB = bA - k dB,
+        // bC = bB - k dC from columns nx-4, nx-5, nx-6
+        const size_t cA = (size_t)(nx - 3) * P + j;
+        double lA = bx[cA], lB = bx[cA + P], lC = bx[cA + 2 * P];
+        bx[cA] = bx[cA + P] = bx[cA + 2 * P] = 0.0;
+        lB += lC;
+        lA += lB;
+        bx[cA - P] += lA;
+        if (j > 0) {
+            const double lds[3] = {-k * lA, -k * lB, -k * lC};   // columns nx-4, nx-5, nx-6
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const size_t c = (size_t)(nx - 4 - m) * P + j;
+                unsafeAtomicAdd(&by[c], lds[m]);
+                unsafeAtomicAdd(&by[c - 1], -lds[m]);
+            }
+        }
     }
 }
 
@@ -879,10 +941,20 @@ __global__ __launch_bounds__(256) void stage_tlm_kernel(const AdjDesc *runs, con
     if (!tile_cell(first, nruns, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     const Grid &g = d.g;
-    if (i < 3 || i > g.nx - 4 || j < 3 || j > g.ny - 4) return;
+    if (i < 3 || i > g.nx - 4 || j < 2 || j > g.ny - 4) return;
     const size_t o = (size_t)i * g.pitch + j;
     const double dt = d.dts[step];
     const double dtdx = dt / d.ph.dx, dtdy = dt / d.ph.dy;
+    if (j == 2) {
+        // By of the bottom boundary face after the CT, which the symmetric-x
+        // fill reads before the y fill overwrites it; stage 2 carries the
+        // same value (U^n's face and the stage-1 EMF, both unchanged)
+        const size_t q = 5 * g.plane + o;
+        const double v = d.du[q] + dtdx * (d.dE[o] - d.dE[o - g.pitch]);
+        if (stage == 1) d.dus[q] = v;
+        else d.du[q] = v;
+        return;
+    }
     for (int v = 0; v < 8; ++v) {
         const size_t q = v * g.plane + o;
         if (v == 4 || v == 5) {
@@ -903,9 +975,10 @@ __global__ __launch_bounds__(256) void stage_tlm_kernel(const AdjDesc *runs, con
     }
 }
 
-// The boundary fill applied to a tangent state (it is linear): periodic x,
-// then periodic or symmetric y (the same copies and Bx-rebuild of By as the
-// forward fill).
+// The boundary fill applied to a tangent state (it is linear): periodic or
+// symmetric x, then periodic or symmetric y (the same copies and Bx / By
+// rebuilds as the forward fill, in its operation order, so that the
+// decision replay advances a state bit for bit as the forward step does).
 __global__ __launch_bounds__(256) void bc_tlm_x_kernel(const AdjDesc *runs, int stage)
 {
     const AdjDesc &d = runs[blockIdx.y];
@@ -914,10 +987,40 @@ __global__ __launch_bounds__(256) void bc_tlm_x_kernel(const AdjDesc *runs, int 
     double *T = stage == 1 ? d.dus : d.du;
     const int nx = d.g.nx;
     const size_t P = d.g.pitch;
+    const long long pl = d.g.plane;
     const int dst[6] = {0, 1, 2, nx - 3, nx - 2, nx - 1};
-    const int src[6] = {nx - 6, nx - 5, nx - 4, 3, 4, 5};
-    for (int v = 0; v < 8; ++v)
-        for (int q = 0; q < 6; ++q) T[v * d.g.plane + dst[q] * P + j] = T[v * d.g.plane + src[q] * P + j];
+    if (d.bc_x == MHD_BC_PERIODIC) {
+        const int src[6] = {nx - 6, nx - 5, nx - 4, 3, 4, 5};
+        for (int v = 0; v < 8; ++v)
+            for (int q = 0; q < 6; ++q) T[v * pl + dst[q] * P + j] = T[v * pl + src[q] * P + j];
+        return;
+    }
+    const int src[6] = {5, 4, 3, nx - 4, nx - 5, nx - 6};
+    for (int v = 0; v < 8; ++v) {
+        if (v == 4) continue;
+        for (int q = 0; q < 6; ++q) T[v * pl + dst[q] * P + j] = T[v * pl + src[q] * P + j];
+    }
+    const double *by = T + 5 * pl;
+    double *bx = T + 4 * pl;
+    const double dx = d.ph.dx, dy = d.ph.dy;
+    if (j > 0) {
+        const size_t c3 = 3 * P + j, cA = (size_t)(nx - 4) * P + j;
+        const double d3 = by[c3] - by[c3 - 1];
+        double b = bx[c3] + d3 / dy * dx;
+        bx[2 * P + j] = b;
+        b = b + d3 / dy * dx;
+        bx[P + j] = b;
+        bx[j] = b + (by[c3 + P] - by[c3 + P - 1]) / dy * dx;
+        b = bx[cA] - (by[cA] - by[cA - 1]) / dy * dx;
+        bx[cA + P] = b;
+        b = b - (by[cA - P] - by[cA - P - 1]) / dy * dx;
+        bx[cA + 2 * P] = b;
+        bx[cA + 3 * P] = b - (by[cA - 2 * P] - by[cA - 2 * P - 1]) / dy * dx;
+    } else {
+        bx[0] = bx[P] = bx[2 * P] = bx[3 * P];
+        const size_t cA = (size_t)(nx - 4) * P;
+        bx[cA + P] = bx[cA + 2 * P] = bx[cA + 3 * P] = bx[cA];
+    }
 }
 
 __global__ __launch_bounds__(256) void bc_tlm_y_kernel(const AdjDesc *runs, int stage)
@@ -1323,11 +1426,6 @@ int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out)
     if (!out) return afail(nullptr, MHD_ERR_INVALID, "out is NULL");
     *out = nullptr;
     if (!cfgs || nruns < 1) return afail(nullptr, MHD_ERR_INVALID, "need at least one run configuration");
-    for (int r = 0; r < nruns; ++r)
-        if (cfgs[r].bc_x != MHD_BC_PERIODIC)
-            return afail(nullptr, MHD_ERR_INVALID,
-                         "the adjoint needs periodic x (run " + std::to_string(r) + "): the symmetric-x fill reads a "
-                         "stale CT value and is not differentiated");
     mhd_adjoint *a = new (std::nothrow) mhd_adjoint();
     if (!a) return afail(nullptr, MHD_ERR_NOMEM, "host allocation failed");
     int rc = mhd_batch_create(cfgs, nruns, &a->fwd);
@@ -1346,6 +1444,7 @@ int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out)
         std::memset(&d, 0, sizeof d);
         d.g = rd.g;
         d.ph = rd.ph;
+        d.bc_x = rd.bc_x;
         d.bc_y = rd.bc_y;
         d.ntj = (rd.g.ny + 63) / 64;
         d.u = rd.u;
