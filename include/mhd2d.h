@@ -202,11 +202,26 @@ int mhd_step_rk2_noalloc(const mhd_config *cfg, double *dev_state, long long pit
  *    CT-contact corner EMF averaged over the two stages applied once to the
  *    step's initial face B (ct.cu:26-218 there), and dt over interior cells
  *    without the diffusion limit (resistiveMHD2D.cu:489-556 there); a step
- *    with dt < 0 counts as crashed.  Needs periodic x and y and a uniform
- *    eta (no eta field); MHD_ERR_STATE otherwise. */
+ *    with dt < 0 counts as crashed.  Needs periodic x and y and no per-cell
+ *    eta field (MHD_ERR_STATE otherwise); its resistivity is cfg.eta or the
+ *    planes of mhd_set_eta_staggered.  Switching back to MHD_SCHEME_SINGLE
+ *    drops those planes. */
 enum mhd_scheme { MHD_SCHEME_SINGLE = 0, MHD_SCHEME_MULTI_SHARE = 1 };
 int mhd_set_scheme(mhd_handle *h, int scheme);
 int mhd_get_scheme(const mhd_handle *h, int *scheme);
+
+/* MHD_SCHEME_MULTI_SHARE only: the MPI library's plug-in getEta(x, y)
+ * (ResistiveMHD_multiGPU_cuda/ResistiveMHD2D_GPU/get_eta.hpp:6) sampled where
+ * that library calls it, one nx*ny plane each (index j + i*ny, ghosts
+ * included): x-faces at ((i + 1/2) dx, j dy) (flux_solver.cu:84,94 there),
+ * y-faces at (i dx, (j + 1/2) dy) (flux_solver.cu:168,179) and cells at
+ * (i dx, j dy) for the CT's cell EMF (ct.cu:41,56).  All three NULL: back to
+ * the uniform cfg.eta.  The subdomain is its own periodic neighbour, so each
+ * plane's ghost entries must equal their periodic images (as a periodic
+ * getEta gives) and every entry must be >= 0: MHD_ERR_INVALID otherwise;
+ * MHD_ERR_STATE under MHD_SCHEME_SINGLE. */
+int mhd_set_eta_staggered(mhd_handle *h, const double *host_xface, const double *host_yface,
+                          const double *host_cell);
 
 /* Arithmetic of the step (no counterpart in the reference, whose single
  * code path is what MHD_NUMERICS_EXACT reproduces):

@@ -57,6 +57,11 @@ struct oracle_sim {
     /* per-cell resistivity sampled at cell (i, j): plays the role of the
      * example's getEta(i*dx, j*dy) (EX/plasmoid/main_plasmoid.cu:159-172) */
     double *etaField;
+    /* the MPI library's getEta at its three sampling positions, per index
+     * (ghosts included): x-face ((i + 0.5) dx, j dy) (M/flux_solver.cu:84,94),
+     * y-face (i dx, (j + 0.5) dy) (M/flux_solver.cu:168,179) and cell
+     * (i dx, j dy) (M/ct.cu:41,56); NULL: the uniform eta */
+    double *etaXFaceM, *etaYFaceM, *etaCellM;
     /* scheme: ORACLE_SCHEME_SINGLE (the single-GPU library) or
      * ORACLE_SCHEME_MULTI_SHARE (the MPI library's step on one rank) */
     int scheme;
@@ -521,7 +526,7 @@ static void m_hlld_sweep(struct oracle_sim *s, const Cons *U, int shift, Flux *o
         out[k] = hlld_flux(&s->dQLeft[k], &s->dQRight[k], s->gamma, s->eps);
 }
 
-/* M/flux_solver.cu:73-104 (eta uniform: getEta at face positions) */
+/* M/flux_solver.cu:73-104 (getEta at the x-face position) */
 static void m_resistive_x(struct oracle_sim *s, const Cons *U, Flux *flux)
 {
     int nx = s->nx, ny = s->ny;
@@ -536,7 +541,7 @@ static void m_resistive_x(struct oracle_sim *s, const Cons *U, Flux *flux)
                  (U[index + ny].bY - U[index - ny].bY) / (2.0 * dx) - (U[index + 1].bX - U[index - 1].bX) / (2.0 * dy)
                + (U[index + 2 * ny].bY - U[index].bY) / (2.0 * dx) - (U[index + ny + 1].bX - U[index + ny - 1].bX) / (2.0 * dy)
             );
-            eta = s->eta;
+            eta = s->etaXFaceM ? s->etaXFaceM[index] : s->eta;
             etaJY = eta * jY;
             etaJZ = eta * jZ;
             etaJYBZ = etaJY * 0.5 * (U[index].bZ + U[index + ny].bZ);
@@ -547,7 +552,7 @@ static void m_resistive_x(struct oracle_sim *s, const Cons *U, Flux *flux)
         }
 }
 
-/* M/flux_solver.cu:158-189 */
+/* M/flux_solver.cu:158-189 (getEta at the y-face position) */
 static void m_resistive_y(struct oracle_sim *s, const Cons *U, Flux *flux)
 {
     int nx = s->nx, ny = s->ny;
@@ -562,7 +567,7 @@ static void m_resistive_y(struct oracle_sim *s, const Cons *U, Flux *flux)
                  (U[index + ny].bY - U[index - ny].bY) / (2.0 * dx) - (U[index + 1].bX - U[index - 1].bX) / (2.0 * dy)
                + (U[index + ny + 1].bY - U[index - ny + 1].bY) / (2.0 * dx) - (U[index + 2].bX - U[index].bX) / (2.0 * dy)
             );
-            eta = s->eta;
+            eta = s->etaYFaceM ? s->etaYFaceM[index] : s->eta;
             etaJX = eta * jX;
             etaJZ = eta * jZ;
             etaJXBZ = etaJX * 0.5 * (U[index].bZ + U[index + 1].bZ);
@@ -624,7 +629,7 @@ static void m_set_flux(struct oracle_sim *s, const Flux *F, const Flux *G, const
               + (U[index - 1].bY - U[index - 1 - ny].bY) / dx - (U[index - ny].bX - U[index - ny - 1].bX) / dy
               + (U[index].bY - U[index - ny].bY) / dx - (U[index + 1 - ny].bX - U[index - ny].bX) / dy
             );
-            double eta = s->eta;
+            double eta = s->etaCellM ? s->etaCellM[index] : s->eta;
             NF5[index] = F[index].f5;
             NG4[index] = G[index].f4;
             F5[index] = u * bY - v * bX - eta * jZ;
@@ -802,6 +807,7 @@ void oracle_destroy(struct oracle_sim *s)
     free(s->bYOld); free(s->tmpVector); free(s->solverFlux); free(s->hlldFlux); free(s->dQCenter);
     free(s->dQLeft); free(s->dQRight); free(s->tmpUForFluxG); free(s->oldFluxF); free(s->oldFluxG);
     free(s->eZVector); free(s->etaField); free(s->bXStage1); free(s->bYStage1);
+    free(s->etaXFaceM); free(s->etaYFaceM); free(s->etaCellM);
     free(s->oNF5); free(s->oNG4); free(s->oF5); free(s->oG4); free(s->oNF0); free(s->oNG0);
     free(s->nNF5); free(s->nNG4); free(s->nF5); free(s->nG4); free(s->nNF0); free(s->nNG0);
     free(s);
@@ -892,6 +898,26 @@ int oracle_set_eta_field(struct oracle_sim *s, const double *eta)
     return 0;
 }
 
+/* The MPI library's getEta sampled at x-faces, y-faces and cells (all three
+ * planes, or all NULL for the uniform eta). */
+int oracle_set_eta_staggered(struct oracle_sim *s, const double *xface, const double *yface, const double *cell)
+{
+    if (!xface && !yface && !cell) {
+        free(s->etaXFaceM); free(s->etaYFaceM); free(s->etaCellM);
+        s->etaXFaceM = s->etaYFaceM = s->etaCellM = NULL;
+        return 0;
+    }
+    if (!xface || !yface || !cell) return -1;
+    double **dst[3] = {&s->etaXFaceM, &s->etaYFaceM, &s->etaCellM};
+    const double *src[3] = {xface, yface, cell};
+    for (int q = 0; q < 3; ++q) {
+        if (!*dst[q]) *dst[q] = (double *)malloc(sizeof(double) * s->N);
+        if (!*dst[q]) return -1;
+        memcpy(*dst[q], src[q], sizeof(double) * s->N);
+    }
+    return 0;
+}
+
 void oracle_apply_boundary(struct oracle_sim *s) { apply_bc(s, s->U); }
 
 void oracle_apply_boundary_axis(struct oracle_sim *s, int axis, int kind)
@@ -954,8 +980,13 @@ void oracle_compute_fluxes(struct oracle_sim *s, double *aos_state, double *F, d
 {
     Cons *U = (Cons *)aos_state;
     shift_to_center(s, U);
-    get_flux_f(s, U, (Flux *)F);
-    get_flux_g(s, U, (Flux *)G);
+    if (s->scheme == ORACLE_SCHEME_MULTI_SHARE) {
+        m_get_flux_f(s, U, (Flux *)F);
+        m_get_flux_g(s, U, (Flux *)G);
+    } else {
+        get_flux_f(s, U, (Flux *)F);
+        get_flux_g(s, U, (Flux *)G);
+    }
     back_to_center_half(s, U);
 }
 

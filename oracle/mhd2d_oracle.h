@@ -21,6 +21,7 @@ void oracle_face_regions(struct oracle_sim *s, unsigned char *rx, unsigned char 
 double oracle_ct_contact_emf(const double *NF5, const double *NG4, const double *F5, const double *G4,
                              const double *NF0, const double *NG0);
 int oracle_set_eta_field(struct oracle_sim *s, const double *eta);
+int oracle_set_eta_staggered(struct oracle_sim *s, const double *xface, const double *yface, const double *cell);
 void oracle_apply_boundary(struct oracle_sim *s);
 void oracle_apply_boundary_axis(struct oracle_sim *s, int axis, int kind);
 double oracle_calculate_dt(struct oracle_sim *s);

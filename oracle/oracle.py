@@ -44,6 +44,7 @@ def load(path=LIB_PATH):
         "oracle_face_regions": (None, [P, ctypes.c_void_p, ctypes.c_void_p]),
         "oracle_ct_contact_emf": (c_d, [D, D, D, D, D, D]),
         "oracle_set_eta_field": (c_i, [P, D]),
+        "oracle_set_eta_staggered": (c_i, [P, D, D, D]),
         "oracle_apply_boundary": (None, [P]),
         "oracle_apply_boundary_axis": (None, [P, c_i, c_i]),
         "oracle_calculate_dt": (c_d, [P]),
@@ -134,6 +135,20 @@ class OracleSim:
         if self._lib.oracle_set_eta_field(self._s, _p(eta)) != 0:
             raise MemoryError("oracle_set_eta_field")
 
+    def set_eta_staggered(self, xface, yface, cell):
+        """The MPI library's getEta at x-faces ((i + 1/2) dx, j dy), y-faces
+        (i dx, (j + 1/2) dy) and cells (i dx, j dy), one (nx, ny) plane each;
+        all None: the uniform eta."""
+        if xface is None and yface is None and cell is None:
+            self._lib.oracle_set_eta_staggered(self._s, None, None, None)
+            return
+        planes = [np.ascontiguousarray(a, dtype=np.float64) for a in (xface, yface, cell)]
+        for a in planes:
+            assert a.shape == (self.nx, self.ny)
+        self._eta_m = planes
+        if self._lib.oracle_set_eta_staggered(self._s, *(_p(a) for a in planes)) != 0:
+            raise MemoryError("oracle_set_eta_staggered")
+
     def apply_boundary(self):
         self._lib.oracle_apply_boundary(self._s)
 
@@ -165,7 +180,8 @@ class OracleSim:
 
     def set_scheme(self, scheme):
         """SCHEME_SINGLE (single-GPU library) or SCHEME_MULTI_SHARE (the MPI
-        library's step on one rank: periodic halo, uniform eta)."""
+        library's step on one rank: periodic halo; eta uniform or from
+        set_eta_staggered, not from the per-cell plane)."""
         if self._lib.oracle_set_scheme(self._s, scheme) != 0:
             raise ValueError("the multi-GPU scheme needs periodic x and y and a uniform eta")
         self.scheme = scheme

@@ -48,6 +48,7 @@ struct mhd_handle {
     double *aux = nullptr;    // MULTI_SHARE: face fluxes F0 F5 G0 G4 of stage 1, then of stage 2
     double *ecE = nullptr;    // MULTI_SHARE: cell EMF of U^n, of U2, corner EMF E
     double *eta = nullptr;    // optional per-cell resistivity plane
+    double *eta_m = nullptr;  // MULTI_SHARE: getEta at x-faces, y-faces, cells (3 planes) or null
     double *side = nullptr;   // CT value of the bottom boundary-face By per column
     double *aos = nullptr;    // AoS staging for host transfers (allocated on first use)
     Ctrl *ctrl = nullptr;
@@ -411,6 +412,7 @@ int mhd_destroy(mhd_handle *h)
     (void)hipFree(h->aux);
     (void)hipFree(h->ecE);
     (void)hipFree(h->eta);
+    (void)hipFree(h->eta_m);
     (void)hipFree(h->side);
     (void)hipFree(h->aos);
     (void)hipFree(h->ctrl);
@@ -455,7 +457,8 @@ int mhd_set_eta_field(mhd_handle *h, const double *host_eta)
 {
     if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
     if (host_eta && multi(h))
-        return fail(h, MHD_ERR_STATE, "MHD_SCHEME_MULTI_SHARE takes a uniform eta (the MPI library's getEta)");
+        return fail(h, MHD_ERR_STATE, "MHD_SCHEME_MULTI_SHARE takes eta planes at its own sampling positions "
+                                      "(mhd_set_eta_staggered), not a per-cell plane");
     if (!host_eta) {
         HIPCHECK(h, hipStreamSynchronize(h->stream));
         HIPCHECK(h, hipFree(h->eta));
@@ -469,6 +472,58 @@ int mhd_set_eta_field(mhd_handle *h, const double *host_eta)
     HIPCHECK(h, hipMemcpyAsync(h->aos, host_eta, bytes, hipMemcpyHostToDevice, h->stream));
     HIPCHECK(h, mhd::launch_scatter_plane(h->aos, h->eta, h->g, h->stream));
     HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return MHD_OK;
+}
+
+// Ghost entries of a plane equal to their periodic images (the one-rank
+// subdomain's halo is the periodic wrap, so the library evaluates the
+// ghost-layer EMFs and fluxes at the images).
+static bool periodic_images(const double *p, int nx, int ny)
+{
+    const int lx = nx - 2 * mhd::kGhost, ly = ny - 2 * mhd::kGhost;
+    auto wrap = [](int a, int n, int l) { return a < mhd::kGhost ? a + l : (a >= n - mhd::kGhost ? a - l : a); };
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            const int wi = wrap(i, nx, lx), wj = wrap(j, ny, ly);
+            if ((wi != i || wj != j) && p[j + (size_t)i * ny] != p[wj + (size_t)wi * ny]) return false;
+        }
+    return true;
+}
+
+int mhd_set_eta_staggered(mhd_handle *h, const double *host_xface, const double *host_yface,
+                          const double *host_cell)
+{
+    if (!h) return fail(h, MHD_ERR_INVALID, "NULL handle");
+    if (!multi(h)) return fail(h, MHD_ERR_STATE, "staggered eta planes belong to MHD_SCHEME_MULTI_SHARE");
+    if (!host_xface && !host_yface && !host_cell) {
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        HIPCHECK(h, hipFree(h->eta_m));
+        h->eta_m = nullptr;
+        return MHD_OK;
+    }
+    if (!host_xface || !host_yface || !host_cell)
+        return fail(h, MHD_ERR_INVALID, "give all three eta planes (x-face, y-face, cell) or none");
+    const double *src[3] = {host_xface, host_yface, host_cell};
+    const char *name[3] = {"x-face", "y-face", "cell"};
+    for (int q = 0; q < 3; ++q) {
+        const size_t n = (size_t)h->cfg.nx * h->cfg.ny;
+        for (size_t k = 0; k < n; ++k)
+            if (!(src[q][k] >= 0.0))
+                return fail(h, MHD_ERR_INVALID, std::string("the ") + name[q] + " eta plane has a negative or NaN entry");
+        if (!periodic_images(src[q], h->cfg.nx, h->cfg.ny))
+            return fail(h, MHD_ERR_INVALID, std::string("the ") + name[q] + " eta plane's ghost entries differ from "
+                        "their periodic images (the subdomain's halo is the periodic wrap)");
+    }
+    const size_t plane = (size_t)h->g.plane;
+    if (!h->eta_m) HIPCHECK(h, hipMalloc(&h->eta_m, sizeof(double) * 3 * plane));
+    int rc = ensure_aos(h);
+    if (rc) return rc;
+    const size_t bytes = sizeof(double) * (size_t)h->cfg.nx * h->cfg.ny;
+    for (int q = 0; q < 3; ++q) {
+        HIPCHECK(h, hipMemcpyAsync(h->aos, src[q], bytes, hipMemcpyHostToDevice, h->stream));
+        HIPCHECK(h, mhd::launch_scatter_plane(h->aos, h->eta_m + q * plane, h->g, h->stream));
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+    }
     return MHD_OK;
 }
 
@@ -544,18 +599,20 @@ int mhd_step_rk2(mhd_handle *h, int nsteps)
         double *aux1 = m ? h->aux : nullptr, *aux2 = m ? h->aux + 4 * plane : nullptr;
         double *out2 = m ? h->U2 : h->U;
         if (ev[0]) HIPCHECK(h, hipEventRecord(ev[0], h->stream));
-        HIPCHECK(h, mhd::launch_stage(1, h->exact, h->scheme, h->U, h->Us, h->U, h->eta, h->side, aux1, h->ctrl, g,
+        const double *eta_st = m ? h->eta_m : h->eta;   // MULTI: x-face and y-face planes
+        const double *eta_c = h->eta_m ? h->eta_m + 2 * plane : nullptr;
+        HIPCHECK(h, mhd::launch_stage(1, h->exact, h->scheme, h->U, h->Us, h->U, eta_st, h->side, aux1, h->ctrl, g,
                                       h->ph, h->cur, h->stream));
         if (ev[1]) HIPCHECK(h, hipEventRecord(ev[1], h->stream));
-        if (m) HIPCHECK(h, mhd::launch_m_ec(true, h->U, h->ecE, g, h->ph, h->stream));
+        if (m) HIPCHECK(h, mhd::launch_m_ec(true, h->U, eta_c, h->ecE, g, h->ph, h->stream));
         if ((rc = apply_bc_pair(h, h->Us, nullptr, 0))) return rc;
         if (ev[2]) HIPCHECK(h, hipEventRecord(ev[2], h->stream));
-        HIPCHECK(h, mhd::launch_stage(2, h->exact, h->scheme, h->Us, out2, h->U, h->eta, h->side, aux2, h->ctrl, g,
+        HIPCHECK(h, mhd::launch_stage(2, h->exact, h->scheme, h->Us, out2, h->U, eta_st, h->side, aux2, h->ctrl, g,
                                       h->ph, h->cur, h->stream));
         if (ev[3]) HIPCHECK(h, hipEventRecord(ev[3], h->stream));
         if (m) {
             if ((rc = apply_bc_pair(h, h->U2, nullptr, 0))) return rc;
-            HIPCHECK(h, mhd::launch_m_ec(false, h->U2, h->ecE + plane, g, h->ph, h->stream));
+            HIPCHECK(h, mhd::launch_m_ec(false, h->U2, eta_c, h->ecE + plane, g, h->ph, h->stream));
             HIPCHECK(h, mhd::launch_m_emf(aux1, aux2, h->ecE, h->ecE + plane, h->ecE + 2 * plane, g, h->stream));
             HIPCHECK(h, mhd::launch_m_ct(h->exact, h->U, h->ecE + 2 * plane, h->U2, h->ctrl, h->cur ^ 1, g, h->ph,
                                          h->stream));
@@ -596,6 +653,11 @@ int mhd_set_scheme(mhd_handle *h, int scheme)
             HIPCHECK(h, hipMemsetAsync(h->ecE, 0, 3 * plane, h->stream));
             HIPCHECK(h, hipStreamSynchronize(h->stream));
         }
+    }
+    if (scheme == MHD_SCHEME_SINGLE && h->eta_m) {
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        HIPCHECK(h, hipFree(h->eta_m));   // the MPI library's getEta planes have no meaning there
+        h->eta_m = nullptr;
     }
     h->scheme = scheme;
     h->dt_valid = false;   // the two schemes reduce dt over different cells

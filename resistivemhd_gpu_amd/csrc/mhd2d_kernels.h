@@ -83,7 +83,8 @@ hipError_t launch_dt_all(bool exact, bool interior, const double *u, Ctrl *ctrl,
                          const Phys &ph, hipStream_t s);
 // MPI library's CT-contact EMF: cell-centred EMF of a state (smoothed or as
 // stored), corner EMF from two flux sets, final CT with the next dt.
-hipError_t launch_m_ec(bool smooth, const double *u, double *ec, const Grid &g, const Phys &ph, hipStream_t s);
+hipError_t launch_m_ec(bool smooth, const double *u, const double *eta_c, double *ec, const Grid &g, const Phys &ph,
+                      hipStream_t s);
 hipError_t launch_m_emf(const double *aux_old, const double *aux_now, const double *ec_old, const double *ec_now,
                         double *E, const Grid &g, hipStream_t s);
 hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *u, Ctrl *ctrl, int slot,

@@ -102,6 +102,9 @@ __device__ __forceinline__ Cell make_cell(double rho, double mx, double my, doub
 // no CT in the stages; the face fluxes the CT-contact EMF needs are stored
 // in `aux` (planes F0, F5 at x-faces and G0, G4 at y-faces, owned by the
 // cell on their low side) and the stage-2 state goes to uout != uold.
+// SCHEME 1 with etaf: the MPI library's getEta sampled per index, etaf the
+// x-face plane ((i + 1/2) dx, j dy), etaf + plane the y-face plane
+// (i dx, (j + 1/2) dy) (M/flux_solver.cu:84,168); null: the uniform ph.eta.
 //
 // stage_item is one work item (strip sy, segment sx) of one run; the
 // single-run kernel and the batched work-list kernel both call it.
@@ -192,7 +195,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     // LDS per wavefront: 15,632 B in stage 1 and 18,800 B in stage 2, so
     // eight wavefronts (two per SIMD, the VGPR limit) fit a CU's 160 KiB.
     constexpr bool DMA = !TAIL;
-    __shared__ __attribute__((aligned(16))) double slot[kStagePlanes][kStageRows];
+    __shared__ __attribute__((aligned(16))) double slot[MULTI ? kStagePlanes + 1 : kStagePlanes][kStageRows];
     const int r0 = j0 - 3;
     const int rpair = min(r0 + 2 * lane, (g.ny - 2) & ~1);
     auto stage_col = [&](int col) {
@@ -206,6 +209,9 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                 __builtin_amdgcn_global_load_lds((glb_void_t *)(uin + (size_t)v * g.plane + off),
                                                  (lds_void_t *)slot[v], 16, 0, 0);
             if (etaf) __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + off), (lds_void_t *)slot[8], 16, 0, 0);
+            if (MULTI && etaf)
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + g.plane + off), (lds_void_t *)slot[MULTI ? 9 : 8],
+                                                 16, 0, 0);
         }
     };
     // the stage input (and in stage 2 U^n) of the cell being updated,
@@ -248,10 +254,12 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     double dtloc = __longlong_as_double((long long)kInfBits);
     int nanflag = 0;
 
-    double eta_prev = 0.0;   // resistivity of cell c-1
+    double eta_prev = 0.0;   // resistivity of cell c-1 (SCHEME 1: of x-face c-1 + 1/2)
+    double eta_prev2 = 0.0;  // SCHEME 1: x-face c-2 + 1/2
+    double etay_prev = 0.0;  // SCHEME 1: y-face (c-1, j + 1/2)
 
     for (int c = i0 - 2; c <= iend + 1; ++c) {
-        double rho, mx, my, mz, bx, by, bz, e, bym, eta_c;
+        double rho, mx, my, mz, bx, by, bz, e, bym, eta_c, etay_c = ph.eta;
         if (DMA) {
             // s_waitcnt vmcnt(0): this column's staging loads have landed
             __builtin_amdgcn_s_waitcnt(0x0F70);
@@ -259,6 +267,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             bx = slot[4][lane + 1]; by = slot[5][lane + 1]; bz = slot[6][lane + 1]; e = slot[7][lane + 1];
             bym = slot[5][lane];
             eta_c = etaf ? slot[8][lane + 1] : ph.eta;
+            if (MULTI && etaf) etay_c = slot[MULTI ? 9 : 8][lane + 1];
             // warm-up columns stage the next column at once; the others after
             // their x-face flux
             if (c < i0) {
@@ -335,7 +344,8 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     // face (c-2+1/2): cells A = c-2 and B = c-1 (M/flux_solver.cu:73-104)
                     const double fjY = EXACT ? -(B.bz - A.bz) / ph.dx : -(B.bz - A.bz) * (2.0 * inv_2dx);
                     const double fjZ = 0.5 * (((rTA - rTB) + tA) - tB);
-                    const double fJY = ph.eta * fjY, fJZ = ph.eta * fjZ;
+                    const double feta = etaf ? eta_prev2 : ph.eta;
+                    const double fJY = feta * fjY, fJZ = feta * fjZ;
                     const double fJYBZ = fJY * 0.5 * (A.bz + B.bz);
                     const double fJZBY = fJZ * 0.5 * (A.byc + B.byc);
                     F.f5 -= fJZ;
@@ -399,7 +409,8 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     const double tA1 = lane_next(tA), tB1 = lane_next(tB);
                     const double gjX = EXACT ? (bz_up - B.bz) / ph.dy : (bz_up - B.bz) * (2.0 * inv_2dy);
                     const double gjZ = 0.5 * (((tA - tB) + tA1) - tB1);
-                    const double gJX = ph.eta * gjX, gJZ = ph.eta * gjZ;
+                    const double geta = etaf ? etay_prev : ph.eta;
+                    const double gJX = geta * gjX, gJZ = geta * gjZ;
                     const double gJXBZ = gJX * 0.5 * (B.bz + bz_up);
                     const double gJZBX = gJZ * 0.5 * (B.bxc + bxc_up);
                     G.f4 += gJZ;
@@ -581,6 +592,10 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
         }
         A = B;
         B = C;
+        if (MULTI) {
+            eta_prev2 = eta_prev;
+            etay_prev = etay_c;
+        }
         eta_prev = eta_c;
     }
 
@@ -955,8 +970,8 @@ __device__ __forceinline__ double smooth_by(const double *by, size_t o)
 // interior.  SMOOTH: of the face-smoothed state (stage 1 sees U^n after
 // backUToCenterHalfForCT); otherwise of the state as stored.
 template <bool SMOOTH>
-__global__ __launch_bounds__(256) void m_ec_kernel(const double *__restrict__ u, double *__restrict__ ec, Grid g,
-                                                   Phys ph)
+__global__ __launch_bounds__(256) void m_ec_kernel(const double *__restrict__ u, const double *__restrict__ eta_c,
+                                                   double *__restrict__ ec, Grid g, Phys ph)
 {
     const int j = kGhost + blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= g.ny - kGhost) return;
@@ -976,7 +991,8 @@ __global__ __launch_bounds__(256) void m_ec_kernel(const double *__restrict__ u,
             + (BY(o - 1 + P) - BY(o - 1)) / ph.dx - (BX(o) - BX(o - 1)) / ph.dy
             + (BY(o - 1) - BY(o - 1 - P)) / ph.dx - (BX(o - P) - BX(o - P - 1)) / ph.dy
             + (BY(o) - BY(o - P)) / ph.dx - (BX(o + 1 - P) - BX(o - P)) / ph.dy);
-        ec[o] = uu * bY - vv * bX - ph.eta * jZ;
+        const double eta = eta_c ? eta_c[o] : ph.eta;   // getEta(i dx, j dy), M/ct.cu:41,56
+        ec[o] = uu * bY - vv * bX - eta * jZ;
     }
 }
 
@@ -1271,10 +1287,11 @@ static dim3 interior_grid(const Grid &g, int extra)
     return dim3((cols + 255) / 256, rows < 16384 ? rows : 16384);
 }
 
-hipError_t launch_m_ec(bool smooth, const double *u, double *ec, const Grid &g, const Phys &ph, hipStream_t s)
+hipError_t launch_m_ec(bool smooth, const double *u, const double *eta_c, double *ec, const Grid &g, const Phys &ph,
+                      hipStream_t s)
 {
-    if (smooth) hipLaunchKernelGGL((m_ec_kernel<true>), interior_grid(g, 0), dim3(256), 0, s, u, ec, g, ph);
-    else hipLaunchKernelGGL((m_ec_kernel<false>), interior_grid(g, 0), dim3(256), 0, s, u, ec, g, ph);
+    if (smooth) hipLaunchKernelGGL((m_ec_kernel<true>), interior_grid(g, 0), dim3(256), 0, s, u, eta_c, ec, g, ph);
+    else hipLaunchKernelGGL((m_ec_kernel<false>), interior_grid(g, 0), dim3(256), 0, s, u, eta_c, ec, g, ph);
     return hipGetLastError();
 }
 

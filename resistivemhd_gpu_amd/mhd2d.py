@@ -44,7 +44,7 @@ EXPORTS = (
     "mhd_adjoint_set_states_device", "mhd_adjoint_get_states_device", "mhd_adjoint_set_seeds_device",
     "mhd_adjoint_get_grad_states_device", "mhd_adjoint_get_grad_etas_device", "mhd_adjoint_set_face_kernel",
     "mhd_adjoint_autotune", "mhd_adjoint_set_degenerate_tol", "mhd_adjoint_replay", "mhd_step_rk2_noalloc",
-    "mhd_adjoint_set_eta_fields_device",
+    "mhd_adjoint_set_eta_fields_device", "mhd_set_eta_staggered",
 )
 
 MHD_NUMERICS_FAST = 0
@@ -103,6 +103,7 @@ def load_library(path=LIB_PATH):
         "mhd_set_state": (I, [P, D]),
         "mhd_get_state": (I, [P, D]),
         "mhd_set_eta_field": (I, [P, D]),
+        "mhd_set_eta_staggered": (I, [P, D, D, D]),
         "mhd_set_eta": (I, [P, ctypes.c_double]),
         "mhd_apply_boundary": (I, [P, I, I]),
         "mhd_set_boundary": (I, [P, I, I]),
@@ -390,6 +391,20 @@ class ResistiveMHD2D:
         if eta.shape != self.shape[:2]:
             raise ValueError(f"eta shape {eta.shape} != {self.shape[:2]}")
         self._ok(self._lib.mhd_set_eta_field(self._h, _dptr(eta)))
+
+    def setEtaStaggered(self, xface, yface, cell):
+        """MHD_SCHEME_MULTI_SHARE: the MPI library's getEta sampled at
+        x-faces ((i + 1/2) dx, j dy), y-faces (i dx, (j + 1/2) dy) and cells
+        (i dx, j dy), one (nx, ny) plane each, ghost entries equal to their
+        periodic images; all None: the uniform eta."""
+        if xface is None and yface is None and cell is None:
+            self._ok(self._lib.mhd_set_eta_staggered(self._h, None, None, None))
+            return
+        planes = [np.ascontiguousarray(a, dtype=np.float64) for a in (xface, yface, cell)]
+        for a in planes:
+            if a.shape != self.shape[:2]:
+                raise ValueError(f"eta shape {a.shape} != {self.shape[:2]}")
+        self._ok(self._lib.mhd_set_eta_staggered(self._h, *(_dptr(a) for a in planes)))
 
     def setEta(self, eta):
         self._ok(self._lib.mhd_set_eta(self._h, float(eta)))

@@ -31,10 +31,12 @@ def _halo(cfg, U0):
     return o.get_state()
 
 
-def _gpu(cfg, U0, steps, numerics="exact", sweep=None):
+def _gpu(cfg, U0, steps, numerics="exact", sweep=None, planes=None):
     sim = mhd.ResistiveMHD2D(cfg)
     sim.set_numerics(numerics)
     sim.set_scheme("multi")
+    if planes is not None:
+        sim.setEtaStaggered(*planes)
     sim.setU(U0)
     if sweep:
         sim.set_sweep_length(sweep)
@@ -42,13 +44,15 @@ def _gpu(cfg, U0, steps, numerics="exact", sweep=None):
     return sim
 
 
-def _compare(cfg, U0, steps, numerics, sweep=None, tag=""):
+def _compare(cfg, U0, steps, numerics, sweep=None, tag="", planes=None):
     print(f"case {tag}: nx={cfg.nx} ny={cfg.ny} steps={steps} numerics={numerics} sweep={sweep}", flush=True)
     o = OracleSim.from_config(cfg)
     o.set_scheme(SCHEME_MULTI_SHARE)
+    if planes is not None:
+        o.set_eta_staggered(*planes)
     o.set_state(U0)
     o.step(steps)
-    g = _gpu(cfg, U0, steps, numerics, sweep)
+    g = _gpu(cfg, U0, steps, numerics, sweep, planes)
     Uo, Ug = o.get_state(), g.getU()
     err = rel_l2(Ug, Uo)
     print(tag, numerics, " ".join(f"{n}={e:.2e}" for n, e in zip(NAMES, err)), flush=True)
@@ -85,6 +89,79 @@ def test_multi_share_turbulence_ragged(numerics):
     cfg = mhd.default_config(96, 70 + 61, 0.0, 2.0 * np.pi, 0.0, 3.0 * np.pi, eta=1e-4)
     U0 = _halo(cfg, mhd.fill_turbulence(cfg, seed=7, kmax=8, rms_v=0.5, rms_b=0.5))
     _compare(cfg, U0, 60, numerics, sweep=16, tag="m-turb")
+
+
+def _periodic_eta_planes(cfg, seed, base):
+    """getEta of a smooth periodic resistivity (a random Fourier sum around
+    `base`, positive) sampled at the MPI library's x-face, y-face and cell
+    positions of every interior index; the ghost entries are copies of
+    their periodic images (which a periodic getEta gives up to rounding)."""
+    rng = np.random.default_rng(seed)
+    lx = (cfg.nx - 6) * cfg.dx
+    ly = (cfg.ny - 6) * cfg.dy
+    modes = [(int(rng.integers(1, 4)), int(rng.integers(0, 4)), rng.uniform(0, 2 * np.pi), rng.uniform(0.05, 0.2))
+             for _ in range(4)]
+
+    def eta(x, y):
+        v = np.ones(np.broadcast(x, y).shape)
+        for kx, ky, ph, a in modes:
+            v += a * np.sin(2 * np.pi * (kx * x / lx + ky * y / ly) + ph)
+        return base * v
+
+    i = np.arange(cfg.nx - 6)[:, None] + 0.0
+    j = np.arange(cfg.ny - 6)[None, :] + 0.0
+    planes = (eta((i + 0.5) * cfg.dx, j * cfg.dy), eta(i * cfg.dx, (j + 0.5) * cfg.dy), eta(i * cfg.dx, j * cfg.dy))
+    return [np.pad(p, 3, mode="wrap") for p in planes]
+
+
+@pytest.mark.parametrize("numerics", ["exact", "fast"])
+def test_multi_share_eta_planes_orszag_tang(numerics):
+    """Spatial resistivity at the MPI library's three sampling positions
+    (M/flux_solver.cu:84,168, M/ct.cu:41): bitwise the oracle's step."""
+    cfg = mhd.orszag_tang_config(64, eta=1e-3)
+    U0 = _halo(cfg, mhd.fill_orszag_tang(cfg))
+    _compare(cfg, U0, 60, numerics, tag="m-ot64-eta", planes=_periodic_eta_planes(cfg, 3, 5e-3))
+
+
+def test_multi_share_eta_planes_turbulence_ragged():
+    cfg = mhd.default_config(96, 70 + 61, 0.0, 2.0 * np.pi, 0.0, 3.0 * np.pi, eta=1e-4)
+    U0 = _halo(cfg, mhd.fill_turbulence(cfg, seed=7, kmax=8, rms_v=0.5, rms_b=0.5))
+    _compare(cfg, U0, 40, "exact", sweep=16, tag="m-turb-eta", planes=_periodic_eta_planes(cfg, 4, 2e-3))
+
+
+def test_multi_share_eta_planes_refusals():
+    """Planes only under the multi-GPU scheme, all three or none, no
+    negative entries, ghost entries equal to their periodic images; switching
+    back to the single-GPU scheme drops them."""
+    cfg = mhd.orszag_tang_config(32, eta=1e-3)
+    planes = _periodic_eta_planes(cfg, 5, 1e-3)
+    sim = mhd.ResistiveMHD2D(cfg)
+    with pytest.raises(mhd.MHDError):
+        sim.setEtaStaggered(*planes)
+    sim.set_scheme("multi")
+    sim.setEtaStaggered(*planes)
+    bad = [p.copy() for p in planes]
+    bad[1][0, 10] += 1e-6                       # a ghost entry that is not its image
+    with pytest.raises(mhd.MHDError):
+        sim.setEtaStaggered(*bad)
+    bad = [p.copy() for p in planes]
+    bad[2][10, 10] = -1e-3
+    with pytest.raises(mhd.MHDError):
+        sim.setEtaStaggered(*bad)
+    with pytest.raises(mhd.MHDError):
+        sim._ok(sim._lib.mhd_set_eta_staggered(sim._h, mhd.mhd2d._dptr(planes[0]), None, None))
+    # uniform planes of cfg.eta are the uniform step, bit for bit
+    U0 = _halo(cfg, mhd.fill_orszag_tang(cfg))
+    sim.setEtaStaggered(*[np.full((cfg.nx, cfg.ny), cfg.eta)] * 3)
+    sim.setU(U0)
+    sim.steps(5)
+    ref = _gpu(cfg, U0, 5)
+    assert np.array_equal(sim.getU(), ref.getU())
+    sim.set_scheme("single")
+    sim.set_scheme("multi")
+    sim.setU(U0)
+    sim.steps(5)
+    assert np.array_equal(sim.getU(), ref.getU())
 
 
 def test_multi_share_init_matches_host_fill():

@@ -8,6 +8,8 @@ function is checked against closed forms, invariants and special cases:
                        written here), exact resolution of stationary contacts,
                        mirror symmetry under x -> -x
   resistive flux    -- linear B: constant current, closed-form eta*J flux
+                       (also at the MPI library's face and cell sampling
+                       positions of getEta)
   dt                -- closed form for a uniform state
   RK2 + CT + BC     -- conservation of mass, momentum and energy under
                        periodic boundaries, div B held to round-off, a uniform
@@ -512,6 +514,108 @@ def test_multi_scheme_rejects_symmetric_and_eta_field():
         o.set_scheme(SCHEME_MULTI_SHARE)
 
 
+# ------------------- the MPI library's getEta at its three sampling positions
+def _periodic_planes(nx, ny, seed, g=3):
+    """Three eta planes (x-face, y-face, cell) of smooth random values on the
+    interior whose ghost entries are their periodic images, as a periodic
+    getEta sampled at every stored index gives on one rank."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(3):
+        I = 1e-3 * (1.0 + 0.5 * rng.uniform(-1, 1, (nx - 2 * g, ny - 2 * g)))
+        out.append(np.pad(I, g, mode="wrap"))
+    return out
+
+
+def test_multi_eta_planes_of_the_uniform_value_are_the_uniform_step():
+    cfg, o = _ot_multi(24, 0)
+    U0 = o.get_state()
+    o.step(3)
+    A = o.get_state()
+    b = OracleSim.from_config(cfg)
+    b.set_scheme(SCHEME_MULTI_SHARE)
+    b.set_state(U0)
+    full = np.full((cfg.nx, cfg.ny), cfg.eta)
+    b.set_eta_staggered(full, full, full)
+    b.step(3)
+    assert np.array_equal(b.get_state(), A)
+
+
+def test_multi_resistive_flux_reads_eta_at_its_face():
+    """By = a x (centred By linear), Bx = 0, at rest: the MPI library's
+    face current is J_z = a exactly and the HLLD part of f5 and of G's f4
+    vanishes, so F.f5 = -a eta_x(i, j) on x-face (i + 1/2, j) and G.f4 =
+    a eta_y(i, j) on y-face (i, j + 1/2), each from its own plane at its own
+    index (M/flux_solver.cu:84-96,168-181)."""
+    nx, ny, dx, dy, a = 20, 18, 0.1, 0.1, 0.8
+    U = _uniform_state(nx, ny, u=0.0, v=0.0, w=0.0, bx=0.0, by=0.0, bz=0.0, p=1.0)
+    x = (np.arange(nx) - 2.5) * dx
+    U[:, :, 5] = a * x[:, None]
+    U[:, :, 7] += 0.5 * U[:, :, 5] ** 2
+    o = OracleSim(nx, ny, dx, dy, GAMMA, 0.7, EPS, 0.0)
+    o.set_scheme(SCHEME_MULTI_SHARE)
+    rng = np.random.default_rng(4)
+    ex, ey, ec = (1e-3 * (1.0 + rng.uniform(size=(nx, ny))) for _ in range(3))
+    o.set_eta_staggered(ex, ey, ec)
+    F, G = o.compute_fluxes(U)
+    i, j = slice(3, nx - 4), slice(3, ny - 4)
+    np.testing.assert_allclose(F[i, j, 5], -a * ex[i, j], rtol=1e-12)
+    np.testing.assert_allclose(G[i, j, 4], a * ey[i, j], rtol=1e-12)
+
+
+def test_multi_cell_eta_enters_the_ct_at_its_cell():
+    """The cell plane enters only the cell EMF of the CT-contact average
+    (M/ct.cu:26-65).  Where every face's mass flux is positive, the corner
+    EMF (i + 1/2, j + 1/2) of M/ct.cu:147-169 reads the cell EMFs with
+    weights 1/2 (cell i, j), 1/4 (i, j+1) and 1/4 (i+1, j), so raising eta at
+    one cell (i0, j0) changes the corners (i0, j0), (i0, j0-1), (i0-1, j0) by
+    q/2, q/4, q/4 (q the same for all three), and after one step exactly
+    these faces, in these ratios (dx = dy): Bx(i0, j0-1), Bx(i0, j0),
+    Bx(i0-1, j0) by -q/4, Bx(i0-1, j0+1) by q/4, Bx(i0, j0+1) by q/2 (times
+    dt/dy), and By(i0-1, j0), By(i0, j0-1), By(i0, j0) by q/4, By(i0+1, j0-1)
+    by -q/4, By(i0+1, j0) by -q/2 (times dt/dx).  The cell plane read one
+    cell off, or at the transposed index, moves the pattern."""
+    import resistivemhd_gpu_amd as mhd
+    cfg = mhd.turbulence_config(24, eta=2e-3)
+    U = mhd.fill_turbulence(cfg, seed=7, kmax=3, rms_v=0.05, rms_b=0.4)
+    rho = U[..., 0]
+    ke0 = 0.5 * (U[..., 1] ** 2 + U[..., 2] ** 2) / rho
+    U[..., 1] += 0.8 * rho
+    U[..., 2] += 0.6 * rho
+    U[..., 7] += 0.5 * (U[..., 1] ** 2 + U[..., 2] ** 2) / rho - ke0
+    o = OracleSim.from_config(cfg)
+    o.set_state(U)
+    o.apply_boundary()
+    U = o.get_state()
+    base = [np.full((cfg.nx, cfg.ny), cfg.eta) for _ in range(3)]
+    runs = []
+    i0, j0 = 11, 13
+    for bump in (0.0, 2e-2):
+        planes = [p.copy() for p in base]
+        planes[2][i0, j0] += bump
+        o = OracleSim.from_config(cfg)
+        o.set_scheme(SCHEME_MULTI_SHARE)
+        o.set_fixed_dt(0.02)
+        o.set_eta_staggered(*planes)
+        F, G = o.compute_fluxes(U)   # the stage-1 fluxes: mass flux positive on every face
+        assert np.all(F[2:-3, 2:-3, 0] > 0) and np.all(G[2:-3, 2:-3, 0] > 0)
+        o.set_state(U)
+        o.step(1)
+        runs.append(o.get_state())
+    D = runs[1] - runs[0]
+    assert np.array_equal(D[..., [0, 1, 2, 3, 6, 7]], np.zeros_like(D[..., :6]))
+    dbx, dby = D[..., 4], D[..., 5]
+    r = dbx[i0, j0]            # -q/4 dt/dy
+    assert abs(r) > 1e-6
+    want_bx = {(i0, j0 - 1): 1.0, (i0, j0): 1.0, (i0 - 1, j0): 1.0, (i0 - 1, j0 + 1): -1.0, (i0, j0 + 1): -2.0}
+    want_by = {(i0 - 1, j0): -1.0, (i0, j0 - 1): -1.0, (i0, j0): -1.0, (i0 + 1, j0 - 1): 1.0, (i0 + 1, j0): 2.0}
+    for plane, want in ((dbx, want_bx), (dby, want_by)):
+        got = {(int(a), int(b)) for a, b in np.argwhere(plane != 0)}
+        assert got == set(want), (sorted(got), sorted(want))
+        for (i, j), k in want.items():
+            assert abs(plane[i, j] - k * r) <= 1e-9 * abs(r), (i, j, plane[i, j], k * r)
+
+
 # ------------------------------------- rotation by 90 degrees (both schemes)
 def _rotate(U, g=3):
     """The state rotated by +90 degrees about the grid centre, (x, y) ->
@@ -617,6 +721,37 @@ def test_step_commutes_with_transpose(scheme):
     o.apply_boundary()
     U = o.get_state()
     _assert_commutes(_transpose(_run_fixed_dt(cfg, U, scheme, 3)), _run_fixed_dt(cfg, _transpose(U), scheme, 3))
+
+
+def test_multi_step_with_eta_planes_commutes_with_transpose():
+    """The transpose maps an x-face ((i + 1/2) dx, j dy) onto the y-face
+    (j dx, (i + 1/2) dy) and a cell onto a cell, so the step with eta
+    planes (x-face, y-face, cell) commutes with it when the transposed run
+    takes the planes (cell^T ... ): x-face' = y-face^T, y-face' = x-face^T,
+    cell' = cell^T.  A plane read at the other face direction, or the cell
+    plane read one cell off along one axis, breaks the symmetry."""
+    cfg, U = _generic_periodic_state(32, 5)
+    o = OracleSim.from_config(cfg)
+    o.set_state(U)
+    o.apply_boundary()
+    U = o.get_state()
+    ex, ey, ec = _periodic_planes(cfg.nx, cfg.ny, 6)
+
+    def run(state, planes):
+        r = OracleSim.from_config(cfg)
+        r.set_scheme(SCHEME_MULTI_SHARE)
+        r.set_fixed_dt(0.02)
+        r.set_eta_staggered(*planes)
+        r.set_state(state)
+        r.apply_boundary()
+        r.step(3)
+        return r.get_state()
+
+    A = _transpose(run(U, (ex, ey, ec)))
+    _assert_commutes(A, run(_transpose(U), (ey.T, ex.T, ec.T)))
+    # the planes matter: the same run with the face planes swapped differs
+    B = _transpose(run(U, (ey, ex, ec)))
+    assert np.abs(B[3:-3, 3:-3, 7] - A[3:-3, 3:-3, 7]).max() > 1e-9 * np.abs(A[3:-3, 3:-3, 7]).max()
 
 
 def test_step_commutes_with_quarter_turn():
