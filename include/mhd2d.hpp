@@ -115,6 +115,15 @@ public:
         check(mhd_set_eta_field(h_, eta.data()), h_);
     }
     void setUniformEta(double eta) { check(mhd_set_eta_field(h_, nullptr), h_); check(mhd_set_eta(h_, eta), h_); cfg_.eta = eta; }
+    // the MPI library's getEta plug-in under MHD_SCHEME_MULTI_SHARE: sampled
+    // at x-faces, y-faces and cells (nx*ny each, periodic ghost entries)
+    void setEtaStaggered(const std::vector<double> &xface, const std::vector<double> &yface,
+                         const std::vector<double> &cell)
+    {
+        const size_t n = (size_t)cfg_.nx * cfg_.ny;
+        if (xface.size() != n || yface.size() != n || cell.size() != n) throw Error(MHD_ERR_INVALID, "eta size != nx*ny");
+        check(mhd_set_eta_staggered(h_, xface.data(), yface.data(), cell.data()), h_);
+    }
 
     mhd_handle *handle() { return h_; }
     const mhd_config &config() const { return cfg_; }
