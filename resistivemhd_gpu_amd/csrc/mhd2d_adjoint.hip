@@ -78,23 +78,23 @@ struct AdjDesc {
 // ---------------------------------------------------------------- kernels
 // Work list: run r owns tiles [first[r], first[r+1]); a tile is 64
 // consecutive rows of one column, one wavefront (blocks hold blockDim.x / 64
-// tiles).
-__device__ __forceinline__ bool tile_cell(const long long *first, int nruns, int &r, int &i, int &j,
-                                          const AdjDesc *runs)
+// tiles).  run[t] is the run of tile t (one load instead of a search over
+// the runs).
+struct TileMap {
+    const long long *first;
+    const int *run;
+    int nruns;
+};
+
+__device__ __forceinline__ bool tile_cell(TileMap tm, int &r, int &i, int &j, const AdjDesc *runs)
 {
     const long long t = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (t >= first[nruns]) return false;
-    int lo = 0, hi = nruns - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (first[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    r = lo;
-    const long long lt = t - first[lo];
-    const int ntj = runs[lo].ntj;
-    i = (int)(lt / ntj);
-    j = (int)(lt % ntj) * 64 + (int)(threadIdx.x & 63);
+    if (t >= tm.first[tm.nruns]) return false;
+    r = tm.run[t];
+    const int lt = (int)(t - tm.first[r]);
+    const int ntj = runs[r].ntj;
+    i = lt / ntj;
+    j = (lt - i * ntj) * 64 + (int)(threadIdx.x & 63);
     return true;
 }
 
@@ -141,11 +141,11 @@ __device__ __forceinline__ double *buf_ptr(const AdjDesc &d, int which, int slot
     }
 }
 
-__global__ __launch_bounds__(256) void copy_kernel(const AdjDesc *runs, const long long *first, int nruns, int src,
+__global__ __launch_bounds__(256) void copy_kernel(const AdjDesc *runs, TileMap tm, int src,
                                                    int src_slot, int dst, int dst_slot)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     if (j >= d.g.ny) return;
     const double *a = buf_ptr(d, src, src_slot);
@@ -159,11 +159,11 @@ __global__ __launch_bounds__(256) void copy_kernel(const AdjDesc *runs, const lo
 // transfer layout (8 doubles per cell, k = j + i*ny, runs back to back at
 // AdjDesc::aos), one launch for the whole batch.
 template <bool TO_AOS>
-__global__ __launch_bounds__(256) void aos_all_kernel(const AdjDesc *runs, const long long *first, int nruns, int which,
+__global__ __launch_bounds__(256) void aos_all_kernel(const AdjDesc *runs, TileMap tm, int which,
                                                       double *aos)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     if (j >= d.g.ny) return;
     double *p = buf_ptr(d, which, 0) + (size_t)i * d.g.pitch + j;
@@ -177,21 +177,21 @@ __global__ __launch_bounds__(256) void aos_all_kernel(const AdjDesc *runs, const
 
 // Every run's dL/deta plane to the concatenated layout (nx*ny per run, runs
 // back to back at AdjDesc::aos / 8), one launch for the whole batch.
-__global__ __launch_bounds__(256) void geta_all_kernel(const AdjDesc *runs, const long long *first, int nruns,
+__global__ __launch_bounds__(256) void geta_all_kernel(const AdjDesc *runs, TileMap tm,
                                                        double *dst)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     if (j >= d.g.ny) return;
     dst[d.aos / 8 + (size_t)i * d.g.ny + j] = d.geta[(size_t)i * d.g.pitch + j];
 }
 
 // mask bits: 1 lamC, 2 geta
-__global__ __launch_bounds__(256) void zero_kernel(const AdjDesc *runs, const long long *first, int nruns, int mask)
+__global__ __launch_bounds__(256) void zero_kernel(const AdjDesc *runs, TileMap tm, int mask)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     if (j >= d.g.ny) return;
     const size_t o = (size_t)i * d.g.pitch + j;
@@ -373,14 +373,25 @@ __device__ __forceinline__ bool non_b(int v) { return v != 4 && v != 5; }
 // One wavefront per block: at ~500 registers a SIMD holds one such wave, and
 // single-wave blocks let it take the next tile as soon as its own ends.
 // slot >= 0: the primal state of this stage is history slot `slot` (U^n for
-// stage 1, U* for stage 2); slot < 0: the live buffers u / us.  DEGEN:
+// stage 1, U* for stage 2), except stage 1 at slot 0 with ck >= 0, whose
+// U^n is checkpoint ck (a segment starts from its checkpoint in place);
+// slot < 0: the live buffers u / us.  DEGEN:
 // the degenerate star-state rule is on (d.dtol > 0); without it the rule's
 // test folds away at compile time.
 // Adjoint of face (i, j) of direction DIR: its flux adjoint from the stage's
 // cell adjoints, then the flux's input adjoints added to lamC / geta with
 // global atomics.
+template <int STAGE>
+__device__ __forceinline__ const double *stage_prim(const AdjDesc &d, int slot, int ck)
+{
+    const size_t s8 = (size_t)8 * d.g.plane;
+    if (slot < 0) return STAGE == 2 ? d.us : d.u;
+    if (STAGE == 1 && slot == 0 && ck >= 0) return d.ckpt + (size_t)ck * s8;
+    return (STAGE == 2 ? d.hists : d.hist) + (size_t)slot * s8;
+}
+
 template <int STAGE, int DIR, bool DEGEN>
-__device__ __forceinline__ void face_adj_at(const AdjDesc &d, int i, int j, int step, int slot)
+__device__ __forceinline__ void face_adj_at(const AdjDesc &d, int i, int j, int step, int slot, int ck)
 {
     const Grid &g = d.g;
     const double dt = d.dts[step];
@@ -402,9 +413,7 @@ __device__ __forceinline__ void face_adj_at(const AdjDesc &d, int i, int j, int 
         any |= lamF[4] != 0.0 || lamF[5] != 0.0;
     }
     if (!any) return;
-    const double *prim = slot < 0 ? (STAGE == 2 ? d.us : d.u)
-                                   : (STAGE == 2 ? d.hists : d.hist) + (size_t)slot * 8 * g.plane;
-    const Soa S{prim, g.pitch, g.plane};
+    const Soa S{stage_prim<STAGE>(d, slot, ck), g.pitch, g.plane};
     const EtaAcc E{d.eta, g.pitch, d.ph.eta};
     AtomicAdd add{d.lamC, d.geta, g.pitch, g.plane};
     const double par[4] = {d.ph.gamma - 1.0, d.ph.gamma, d.ph.eps, DEGEN ? d.dtol : 0.0};
@@ -413,16 +422,16 @@ __device__ __forceinline__ void face_adj_at(const AdjDesc &d, int i, int j, int 
 }
 
 template <int STAGE, int DIR, bool DEGEN>
-__global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, const long long *first, int nruns,
-                                                       int step, int slot)
+__global__ __launch_bounds__(64) void face_adj_kernel(const AdjDesc *runs, TileMap tm,
+                                                       int step, int slot, int ck)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     const Grid &g = d.g;
     if (DIR == 0 ? (i < 2 || i > g.nx - 4 || j < 2 || j > g.ny - 3) : (i < 2 || i > g.nx - 3 || j < 2 || j > g.ny - 4))
         return;
-    face_adj_at<STAGE, DIR, DEGEN>(d, i, j, step, slot);
+    face_adj_at<STAGE, DIR, DEGEN>(d, i, j, step, slot, ck);
 }
 
 // Face rows a run's sweep strips leave over (rows tail0 .. ny-3 of the x
@@ -437,7 +446,7 @@ __device__ __forceinline__ int tail_rows(const AdjDesc &d, int dir)
 
 template <int STAGE, int DIR, bool DEGEN>
 __device__ __forceinline__ void face_adj_tail(const AdjDesc *runs, const long long *tfirst, int nruns, long long t,
-                                              int step, int slot)
+                                              int step, int slot, int ck)
 {
     if (t >= tfirst[nruns]) return;
     int lo = 0, hi = nruns - 1;
@@ -449,18 +458,18 @@ __device__ __forceinline__ void face_adj_tail(const AdjDesc *runs, const long lo
     const AdjDesc &d = runs[lo];
     const int rows = tail_rows(d, DIR);
     const long long k = t - tfirst[lo];
-    face_adj_at<STAGE, DIR, DEGEN>(d, 2 + (int)(k / rows), d.tail0 + (int)(k % rows), step, slot);
+    face_adj_at<STAGE, DIR, DEGEN>(d, 2 + (int)(k / rows), d.tail0 + (int)(k % rows), step, slot, ck);
 }
 
 template <int STAGE, bool DEGEN>
 __global__ __launch_bounds__(64) void face_adj_tail_kernel(const AdjDesc *runs, const long long *tfirst, int nruns,
-                                                            unsigned xblocks, int step, int slot)
+                                                            unsigned xblocks, int step, int slot, int ck)
 {
     if (blockIdx.x < xblocks)
-        face_adj_tail<STAGE, 0, DEGEN>(runs, tfirst, nruns, (long long)blockIdx.x * 64 + threadIdx.x, step, slot);
+        face_adj_tail<STAGE, 0, DEGEN>(runs, tfirst, nruns, (long long)blockIdx.x * 64 + threadIdx.x, step, slot, ck);
     else
         face_adj_tail<STAGE, 1, DEGEN>(runs, tfirst + nruns + 1, nruns,
-                                       (long long)(blockIdx.x - xblocks) * 64 + threadIdx.x, step, slot);
+                                       (long long)(blockIdx.x - xblocks) * 64 + threadIdx.x, step, slot, ck);
 }
 
 // ------------------------------------------------ face adjoints, sweep form
@@ -631,7 +640,7 @@ __device__ __forceinline__ void sweep_issue(const SweepRun &g, int wave, int cs,
 
 template <int STAGE, bool DEGEN>
 __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs, const long long *sfirst, int nruns,
-                                                             int cols, int step, int slot)
+                                                             int cols, int step, int slot, int ck)
 {
     __shared__ SweepLds l;
     __shared__ __attribute__((aligned(16))) double stg[kSwStage][kSwRows];
@@ -647,7 +656,7 @@ __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs
     g.ny = d.g.ny;
     g.pitch = d.g.pitch;
     g.plane = d.g.plane;
-    g.prim = slot < 0 ? (STAGE == 2 ? d.us : d.u) : (STAGE == 2 ? d.hists : d.hist) + (size_t)slot * 8 * d.g.plane;
+    g.prim = stage_prim<STAGE>(d, slot, ck);
     g.L = STAGE == 2 ? d.lam : d.lamBar;
     g.lamE = d.lamE;
     g.eta = d.eta;
@@ -754,10 +763,10 @@ __device__ __forceinline__ double shift_t(const AdjDesc &d, int v, int i, int j)
 }
 
 // step 3: lamBar = shift^T lamC + lam_nonB / 2
-__global__ __launch_bounds__(256) void cell2_kernel(const AdjDesc *runs, const long long *first, int nruns)
+__global__ __launch_bounds__(256) void cell2_kernel(const AdjDesc *runs, TileMap tm)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     if (j >= d.g.ny) return;
     const size_t o = (size_t)i * d.g.pitch + j;
@@ -777,10 +786,10 @@ __device__ __forceinline__ double lam_bstar(const AdjDesc &d, int v, int i, int 
     return d.lamBar[o] + d.lam[o];
 }
 
-__global__ __launch_bounds__(256) void ct_adj_kernel(const AdjDesc *runs, const long long *first, int nruns, int step)
+__global__ __launch_bounds__(256) void ct_adj_kernel(const AdjDesc *runs, TileMap tm, int step)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     const Grid &g = d.g;
     if (j >= g.ny) return;
@@ -800,10 +809,10 @@ __global__ __launch_bounds__(256) void ct_adj_kernel(const AdjDesc *runs, const 
 // (i, j >= 1), 0 elsewhere (CT overwrites B from B^n there).  In place: no
 // other thread reads this cell's lam (ct_adj, which reads neighbours, ran
 // before).
-__global__ __launch_bounds__(256) void cell1_kernel(const AdjDesc *runs, const long long *first, int nruns)
+__global__ __launch_bounds__(256) void cell1_kernel(const AdjDesc *runs, TileMap tm)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     if (j >= d.g.ny) return;
     const size_t o = (size_t)i * d.g.pitch + j;
@@ -851,11 +860,11 @@ __global__ __launch_bounds__(256) void pack_plane_kernel(const double *__restric
 // dF, dG of the faces the step reads (zero elsewhere), from the primal state
 // (U or U*) and its tangent, with the tangent of eta.
 template <int DIR>
-__global__ __launch_bounds__(256) void face_tlm_kernel(const AdjDesc *runs, const long long *first, int nruns,
+__global__ __launch_bounds__(256) void face_tlm_kernel(const AdjDesc *runs, TileMap tm,
                                                        int stage)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     const Grid &g = d.g;
     if (j >= g.ny) return;
@@ -887,11 +896,11 @@ __global__ __launch_bounds__(256) void face_tlm_kernel(const AdjDesc *runs, cons
 // linear, so the tangent's EMF, stage and boundary kernels advance P
 // exactly as the forward step advances a state.
 template <int DIR>
-__global__ __launch_bounds__(256) void face_rep_kernel(const AdjDesc *runs, const long long *first, int nruns,
+__global__ __launch_bounds__(256) void face_rep_kernel(const AdjDesc *runs, TileMap tm,
                                                        int stage)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     const Grid &g = d.g;
     if (j >= g.ny) return;
@@ -917,10 +926,10 @@ __global__ __launch_bounds__(256) void face_rep_kernel(const AdjDesc *runs, cons
 }
 
 // Tangent of the corner EMF E(i, j) = (G4(i,j) + G4(i+1,j) - F5(i,j) - F5(i,j+1)) / 4.
-__global__ __launch_bounds__(256) void emf_tlm_kernel(const AdjDesc *runs, const long long *first, int nruns)
+__global__ __launch_bounds__(256) void emf_tlm_kernel(const AdjDesc *runs, TileMap tm)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     const Grid &g = d.g;
     if (j >= g.ny) return;
@@ -934,11 +943,11 @@ __global__ __launch_bounds__(256) void emf_tlm_kernel(const AdjDesc *runs, const
 
 // Tangent of a stage update of the interior cells (ghosts are refilled by
 // the boundary): stage 1 writes dU* (B by CT), stage 2 writes dU (B = dB*).
-__global__ __launch_bounds__(256) void stage_tlm_kernel(const AdjDesc *runs, const long long *first, int nruns,
+__global__ __launch_bounds__(256) void stage_tlm_kernel(const AdjDesc *runs, TileMap tm,
                                                         int stage, int step)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     const Grid &g = d.g;
     if (i < 3 || i > g.nx - 4 || j < 2 || j > g.ny - 4) return;
@@ -1063,10 +1072,10 @@ __global__ __launch_bounds__(256) void bc_tlm_y_kernel(const AdjDesc *runs, int 
     }
 }
 
-__global__ __launch_bounds__(256) void tlm_init_kernel(const AdjDesc *runs, const long long *first, int nruns)
+__global__ __launch_bounds__(256) void tlm_init_kernel(const AdjDesc *runs, TileMap tm)
 {
     int r, i, j;
-    if (!tile_cell(first, nruns, r, i, j, runs)) return;
+    if (!tile_cell(tm, r, i, j, runs)) return;
     const AdjDesc &d = runs[r];
     if (j >= d.g.ny) return;
     const size_t o = (size_t)i * d.g.pitch + j;
@@ -1087,6 +1096,8 @@ struct mhd_adjoint {
     std::vector<AdjDesc> desc;
     AdjDesc *d_desc = nullptr;
     long long *d_first = nullptr;
+    int *d_tile_run = nullptr;    // run of each tile
+    TileMap tiles{};
     long long total_tiles = 0;
     // sweep form of the face adjoints: item prefix per run, columns per item
     long long *d_sfirst = nullptr;
@@ -1211,36 +1222,35 @@ bool use_sweep(const mhd_adjoint *a)
 }
 
 template <int STAGE, bool DEGEN>
-void launch_face_adj_t(const mhd_adjoint *a, int step, int slot, hipStream_t s)
+void launch_face_adj_t(const mhd_adjoint *a, int step, int slot, int ck, hipStream_t s)
 {
     if (use_sweep(a)) {
         if (a->sweep_items)
             hipLaunchKernelGGL((face_adj_sweep_kernel<STAGE, DEGEN>), dim3((unsigned)a->sweep_items), dim3(128), 0, s,
-                               a->d_desc, a->d_sfirst, a->nruns, a->sweep_cols, step, slot);
+                               a->d_desc, a->d_sfirst, a->nruns, a->sweep_cols, step, slot, ck);
         const unsigned xb = (unsigned)((a->tail_faces[0] + 63) / 64), yb = (unsigned)((a->tail_faces[1] + 63) / 64);
         if (xb + yb)
             hipLaunchKernelGGL((face_adj_tail_kernel<STAGE, DEGEN>), dim3(xb + yb), dim3(64), 0, s, a->d_desc,
-                               a->d_tfirst, a->nruns, xb, step, slot);
+                               a->d_tfirst, a->nruns, xb, step, slot, ck);
     } else {
         const dim3 tg1 = tiles_grid1(a), blk(64);
-        hipLaunchKernelGGL((face_adj_kernel<STAGE, 0, DEGEN>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step,
-                           slot);
-        hipLaunchKernelGGL((face_adj_kernel<STAGE, 1, DEGEN>), tg1, blk, 0, s, a->d_desc, a->d_first, a->nruns, step,
-                           slot);
+        hipLaunchKernelGGL((face_adj_kernel<STAGE, 0, DEGEN>), tg1, blk, 0, s, a->d_desc, a->tiles, step, slot, ck);
+        hipLaunchKernelGGL((face_adj_kernel<STAGE, 1, DEGEN>), tg1, blk, 0, s, a->d_desc, a->tiles, step, slot, ck);
     }
 }
 
-// The face-adjoint launches of one step (stage s, both directions).
-int launch_face_adj(mhd_adjoint *a, int stage, int step, int slot)
+// The face-adjoint launches of one step (stage s, both directions); ck: the
+// checkpoint that holds slot 0's U^n.
+int launch_face_adj(mhd_adjoint *a, int stage, int step, int slot, int ck)
 {
     hipStream_t s = stream_of(a);
     const bool degen = a->dtol > 0.0;
     if (stage == 2) {
-        if (degen) launch_face_adj_t<2, true>(a, step, slot, s);
-        else launch_face_adj_t<2, false>(a, step, slot, s);
+        if (degen) launch_face_adj_t<2, true>(a, step, slot, ck, s);
+        else launch_face_adj_t<2, false>(a, step, slot, ck, s);
     } else {
-        if (degen) launch_face_adj_t<1, true>(a, step, slot, s);
-        else launch_face_adj_t<1, false>(a, step, slot, s);
+        if (degen) launch_face_adj_t<1, true>(a, step, slot, ck, s);
+        else launch_face_adj_t<1, false>(a, step, slot, ck, s);
     }
     AHIP(a, hipGetLastError());
     return MHD_OK;
@@ -1248,7 +1258,7 @@ int launch_face_adj(mhd_adjoint *a, int stage, int step, int slot)
 
 int copy_all(mhd_adjoint *a, int src, int src_slot, int dst, int dst_slot)
 {
-    hipLaunchKernelGGL(copy_kernel, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->d_first, a->nruns, src,
+    hipLaunchKernelGGL(copy_kernel, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->tiles, src,
                        src_slot, dst, dst_slot);
     AHIP(a, hipGetLastError());
     return MHD_OK;
@@ -1313,17 +1323,20 @@ int layout_storage(mhd_adjoint *a, int nsteps, int stride)
     a->stride = K;
     a->nckpt = nck;
     const mhd_batch *b = a->fwd;
-    std::vector<mhd::RunDesc> sd((size_t)3 * K * a->nruns);
-    for (int k = 0; k < K; ++k)
+    // K history slots, then nck checkpoint variants of slot 0 (U^n read from
+    // the checkpoint in place)
+    std::vector<mhd::RunDesc> sd((size_t)3 * (K + nck) * a->nruns);
+    for (int k = 0; k < K + nck; ++k)
         for (int r = 0; r < a->nruns; ++r) {
             const AdjDesc &d = a->desc[r];
             const size_t s8 = (size_t)8 * d.g.plane;
+            const int slot = k < K ? k : 0;
             mhd::RunDesc A = b->desc[r];
-            A.u = d.hist + k * s8;
-            A.us = d.hists + k * s8;
+            A.u = k < K ? d.hist + slot * s8 : d.ckpt + (size_t)(k - K) * s8;
+            A.us = d.hists + slot * s8;
             A.uo = nullptr;
             mhd::RunDesc B = A, C = A;
-            B.uo = d.hist + (size_t)std::min(k + 1, K - 1) * s8;
+            B.uo = d.hist + (size_t)std::min(slot + 1, K - 1) * s8;
             C.uo = b->desc[r].u;
             sd[(size_t)(3 * k) * a->nruns + r] = A;
             sd[(size_t)(3 * k + 1) * a->nruns + r] = B;
@@ -1347,17 +1360,20 @@ const mhd::RunDesc *slot_desc(const mhd_adjoint *a, int k, int which)
     return a->d_slot_desc + (size_t)(3 * k + which) * a->nruns;
 }
 
-// Steps s0 .. s0+n-1 with every state written into the history: slot k gets
-// U^{s0+k} (slot 0 must hold U^{s0}) and U* of that step.  last_to_live: the
-// final step's U^{s0+n} goes to the live state (the forward pass's result);
-// otherwise the final step stops after stage 1 (the backward pass needs its
-// U* only).  Steps take dt as the control blocks say (record or replay).
-int history_steps(mhd_adjoint *a, int n, bool last_to_live)
+// Steps s0 .. s0+n-1 of the segment that starts from checkpoint ck, with
+// every state written into the history: slot k gets U^{s0+k} (slot 0's is
+// the checkpoint itself, read in place) and U* of that step.  last_to_live:
+// the final step's U^{s0+n} goes to the live state (the forward pass's
+// result); otherwise the final step stops after stage 1 (the backward pass
+// needs its U* only).  Steps take dt as the control blocks say (record or
+// replay).
+int history_steps(mhd_adjoint *a, int n, bool last_to_live, int ck)
 {
     mhd_batch *b = a->fwd;
     hipStream_t s = stream_of(a);
     for (int k = 0; k < n; ++k) {
-        const mhd::RunDesc *A = slot_desc(a, k, 0);
+        const int kd = k == 0 ? a->stride + ck : k;   // descriptor set of slot k
+        const mhd::RunDesc *A = slot_desc(a, kd, 0);
         AHIP(a, mhd::launch_stage_batch(1, b->exact, A, b->d_item_run, b->total_items, b->xcds, b->items_per_xcd,
                                         b->blocks_per_xcd, b->cur, s, b->d_tail_run, b->tail_items));
         AHIP(a, mhd::launch_bc_batch(b->exact, A, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
@@ -1365,10 +1381,10 @@ int history_steps(mhd_adjoint *a, int n, bool last_to_live)
         // k+1 (or the live state); the boundary fill follows it there
         const mhd::RunDesc *B = nullptr, *after = nullptr;
         if (k + 1 < n) {
-            B = slot_desc(a, k, 1);
+            B = slot_desc(a, kd, 1);
             after = slot_desc(a, k + 1, 0);
         } else if (last_to_live) {
-            B = slot_desc(a, k, 2);
+            B = slot_desc(a, kd, 2);
             after = b->d_desc;
         }
         if (B) {
@@ -1460,10 +1476,23 @@ int mhd_adjoint_create(const mhd_config *cfgs, int nruns, mhd_adjoint **out)
         (e = hipMalloc(&a->d_first, sizeof(long long) * (nruns + 1))) != hipSuccess ||
         (e = hipMemsetAsync(a->pool, 0, sizeof(double) * total, a->fwd->stream)) != hipSuccess ||
         (e = hipMemcpyAsync(a->d_first, first.data(), sizeof(long long) * (nruns + 1), hipMemcpyHostToDevice,
-                            a->fwd->stream)) != hipSuccess) {
+                            a->fwd->stream)) != hipSuccess ||
+        (e = hipMalloc(&a->d_tile_run, sizeof(int) * (size_t)a->total_tiles)) != hipSuccess) {
         const std::string msg = std::string("mhd_adjoint_create: ") + hipGetErrorString(e);
         mhd_adjoint_destroy(a);
         return afail(nullptr, e == hipErrorOutOfMemory ? MHD_ERR_NOMEM : MHD_ERR_HIP, msg);
+    }
+    {
+        std::vector<int> tile_run((size_t)a->total_tiles);
+        for (int r = 0; r < nruns; ++r)
+            std::fill(tile_run.begin() + first[r], tile_run.begin() + first[r + 1], r);
+        if ((e = hipMemcpy(a->d_tile_run, tile_run.data(), sizeof(int) * tile_run.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+            const std::string msg = std::string("mhd_adjoint_create: ") + hipGetErrorString(e);
+            mhd_adjoint_destroy(a);
+            return afail(nullptr, MHD_ERR_HIP, msg);
+        }
+        a->tiles = TileMap{a->d_first, a->d_tile_run, nruns};
     }
     size_t off = 0;
     for (AdjDesc &d : a->desc) {
@@ -1491,6 +1520,7 @@ int mhd_adjoint_destroy(mhd_adjoint *a)
     (void)hipFree(a->pool);
     (void)hipFree(a->d_desc);
     (void)hipFree(a->d_first);
+    (void)hipFree(a->d_tile_run);
     (void)hipFree(a->d_sfirst);
     (void)hipFree(a->d_tfirst);
     (void)hipFree(a->hist);
@@ -1612,8 +1642,7 @@ int mhd_adjoint_forward(mhd_adjoint *a, int nsteps, int stride, int frozen_dt)
             // the last segment keeps its states in the history: the backward
             // pass starts there without recomputing it
             if ((rc = mhd_batch_refresh_dt(b))) return afail(a, rc, b->err);
-            if ((rc = copy_all(a, kU, 0, kHist, 0))) return rc;
-            if ((rc = history_steps(a, n, true))) return rc;
+            if ((rc = history_steps(a, n, true, c))) return rc;
         }
     }
     a->last_in_hist = true;
@@ -1653,7 +1682,7 @@ int mhd_adjoint_backward(mhd_adjoint *a)
     hipStream_t s = stream_of(a);
     const dim3 tg = tiles_grid(a), blk(256);
     int rc;
-    hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+    hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->tiles, 2);
     for (int c = a->nckpt - 1; c >= 0; --c) {
         const int s0 = c * a->stride, s1 = std::min(a->nsteps, s0 + a->stride);
         // recompute the segment with the recorded dt, each step's states
@@ -1663,21 +1692,20 @@ int mhd_adjoint_backward(mhd_adjoint *a)
         if (c == a->nckpt - 1 && a->last_in_hist) {
             a->last_in_hist = false;   // left there by the forward pass; later segments overwrite it
         } else {
-            if ((rc = copy_all(a, kCkpt, c, kHist, 0))) return rc;
             if ((rc = set_replay(a, true, false, s0))) return rc;
-            if ((rc = history_steps(a, s1 - s0, false))) return rc;
+            if ((rc = history_steps(a, s1 - s0, false, c))) return rc;
         }
         for (int st = s1 - 1; st >= s0; --st) {
             const int slot = st - s0;
             if ((rc = bc_transpose(a, kLam))) return rc;
-            hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            if ((rc = launch_face_adj(a, 2, st, slot))) return rc;
-            hipLaunchKernelGGL(cell2_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
+            hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->tiles, 1);
+            if ((rc = launch_face_adj(a, 2, st, slot, c))) return rc;
+            hipLaunchKernelGGL(cell2_kernel, tg, blk, 0, s, a->d_desc, a->tiles);
             if ((rc = bc_transpose(a, kLamBar))) return rc;
-            hipLaunchKernelGGL(ct_adj_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, st);
-            hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            if ((rc = launch_face_adj(a, 1, st, slot))) return rc;
-            hipLaunchKernelGGL(cell1_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
+            hipLaunchKernelGGL(ct_adj_kernel, tg, blk, 0, s, a->d_desc, a->tiles, st);
+            hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->tiles, 1);
+            if ((rc = launch_face_adj(a, 1, st, slot, c))) return rc;
+            hipLaunchKernelGGL(cell1_kernel, tg, blk, 0, s, a->d_desc, a->tiles);
             AHIP(a, hipGetLastError());
         }
     }
@@ -1781,21 +1809,21 @@ static int run_tlm(mhd_adjoint *a, int nsteps, bool replay)
     const dim3 gx((b->max_ny + 255) / 256, a->nruns), gy((b->max_nx + 255) / 256, a->nruns);
     if ((rc = copy_all(a, kU0, 0, kU, 0))) return rc;
     if ((rc = mhd_batch_apply_boundary(b))) return afail(a, rc, b->err);
-    hipLaunchKernelGGL(tlm_init_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
+    hipLaunchKernelGGL(tlm_init_kernel, tg, blk, 0, s, a->d_desc, a->tiles);
     hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 2);
     hipLaunchKernelGGL(bc_tlm_y_kernel, gy, blk, 0, s, a->d_desc, 2);
     if ((rc = set_replay(a, true, false, 0))) return rc;
     for (int st = 0; st < nsteps; ++st) {
         // tangents of stage 1 from (U, dU), primal U* = BC(stage 1 of U)
         if (replay) {
-            hipLaunchKernelGGL(face_rep_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            hipLaunchKernelGGL(face_rep_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+            hipLaunchKernelGGL(face_rep_kernel<0>, tg, blk, 0, s, a->d_desc, a->tiles, 1);
+            hipLaunchKernelGGL(face_rep_kernel<1>, tg, blk, 0, s, a->d_desc, a->tiles, 1);
         } else {
-            hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
-            hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1);
+            hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->tiles, 1);
+            hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->tiles, 1);
         }
-        hipLaunchKernelGGL(emf_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns);
-        hipLaunchKernelGGL(stage_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 1, st);
+        hipLaunchKernelGGL(emf_tlm_kernel, tg, blk, 0, s, a->d_desc, a->tiles);
+        hipLaunchKernelGGL(stage_tlm_kernel, tg, blk, 0, s, a->d_desc, a->tiles, 1, st);
         hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 1);
         hipLaunchKernelGGL(bc_tlm_y_kernel, gy, blk, 0, s, a->d_desc, 1);
         AHIP(a, mhd::launch_stage_batch(1, b->exact, b->d_desc, b->d_item_run, b->total_items, b->xcds,
@@ -1803,13 +1831,13 @@ static int run_tlm(mhd_adjoint *a, int nsteps, bool replay)
         AHIP(a, mhd::launch_bc_batch(b->exact, b->d_desc, b->nruns, b->max_nx, b->max_ny, true, false, 0, s));
         // tangents of stage 2 from (U*, dU*) (B keeps the stage-1 CT value)
         if (replay) {
-            hipLaunchKernelGGL(face_rep_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
-            hipLaunchKernelGGL(face_rep_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+            hipLaunchKernelGGL(face_rep_kernel<0>, tg, blk, 0, s, a->d_desc, a->tiles, 2);
+            hipLaunchKernelGGL(face_rep_kernel<1>, tg, blk, 0, s, a->d_desc, a->tiles, 2);
         } else {
-            hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
-            hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2);
+            hipLaunchKernelGGL(face_tlm_kernel<0>, tg, blk, 0, s, a->d_desc, a->tiles, 2);
+            hipLaunchKernelGGL(face_tlm_kernel<1>, tg, blk, 0, s, a->d_desc, a->tiles, 2);
         }
-        hipLaunchKernelGGL(stage_tlm_kernel, tg, blk, 0, s, a->d_desc, a->d_first, a->nruns, 2, st);
+        hipLaunchKernelGGL(stage_tlm_kernel, tg, blk, 0, s, a->d_desc, a->tiles, 2, st);
         hipLaunchKernelGGL(bc_tlm_x_kernel, gx, blk, 0, s, a->d_desc, 2);
         hipLaunchKernelGGL(bc_tlm_y_kernel, gy, blk, 0, s, a->d_desc, 2);
         AHIP(a, hipGetLastError());
@@ -1865,8 +1893,8 @@ static int upload_all(mhd_adjoint *a, const double *src, int which, bool device)
                                stream_of(a)));
         stage = a->stage_all;
     }
-    hipLaunchKernelGGL(aos_all_kernel<false>, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->d_first,
-                       a->nruns, which, const_cast<double *>(stage));
+    hipLaunchKernelGGL(aos_all_kernel<false>, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->tiles,
+                       which, const_cast<double *>(stage));
     AHIP(a, hipGetLastError());
     AHIP(a, hipStreamSynchronize(stream_of(a)));
     if (which == kU0) invalidate_rollout(a);   // new initial states: the recorded rollout is stale
@@ -1880,8 +1908,8 @@ static int download_all(mhd_adjoint *a, double *dst, int which, bool device)
     int rc;
     if ((rc = ensure_stage_all(a))) return rc;
     double *stage = device ? dst : a->stage_all;
-    hipLaunchKernelGGL(aos_all_kernel<true>, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->d_first,
-                       a->nruns, which, stage);
+    hipLaunchKernelGGL(aos_all_kernel<true>, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->tiles,
+                       which, stage);
     AHIP(a, hipGetLastError());
     if (!device)
         AHIP(a, hipMemcpyAsync(dst, a->stage_all, sizeof(double) * a->aos_off[a->nruns], hipMemcpyDeviceToHost,
@@ -1911,7 +1939,7 @@ static int grad_eta_all(mhd_adjoint *a, double *dst, bool device)
     if ((rc = ensure_stage_all(a))) return rc;
     double *stage = device ? dst : a->stage_all;
     const size_t off = a->aos_off[a->nruns] / 8;
-    hipLaunchKernelGGL(geta_all_kernel, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->d_first, a->nruns,
+    hipLaunchKernelGGL(geta_all_kernel, tiles_grid(a), dim3(256), 0, stream_of(a), a->d_desc, a->tiles,
                        stage);
     AHIP(a, hipGetLastError());
     if (!device)
