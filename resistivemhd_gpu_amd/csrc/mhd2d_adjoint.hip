@@ -691,6 +691,15 @@ __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs
     }
 
     for (int i = ia; i < ib; ++i) {
+        // the lane index, opaque to the compiler afresh each column: masks
+        // derived from it (flush and copy bounds, face validity) are then
+        // recomputed per column rather than hoisted out of the loop and held
+        // in scalar registers across the face computation (which spilled
+        // them to VGPR lanes)
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63;
+        const int j = j0 + lane;
         wait_vm_all();       // this wave's loads for step i landed
         __syncthreads();     // and the other wave's; step i-1 done
         if (i > ia) {
