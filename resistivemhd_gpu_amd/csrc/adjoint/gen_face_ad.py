@@ -23,7 +23,8 @@ Elementary operations and their derivatives:
   tmin(a, b) = b < a ? b : a   routes to the selected argument
   tmax(a, b) = a < b ? b : a   routes to the selected argument
   sgn(a), ind(condition)       piecewise constant: derivative 0
-  minmod(a, b)                 expanded as s = sgn(a); s * tmax(tmin(|a|, s b), eps)
+  minmod(a, b)                 expanded as s = sgn(a); s * tmax(tmin(a s, s b), eps)
+                               (a s = |a|)
 The branch decisions (minmod, tmin/tmax, the HLLD region indicators) are
 taken from the primal values, which the adjoint recomputes first, so the
 adjoint differentiates exactly the branch the forward step took.
@@ -74,7 +75,11 @@ class Lowering:
                 a = self.operand(node.args[0])
                 b = self.operand(node.args[1])
                 s = self.emit("sgn", [a])
-                ax = self.emit("fabs", [a])
+                # |a| as a * sgn(a): the same value (bar the sign of a zero,
+                # which the limiter's eps floor absorbs, so the minmod is
+                # bitwise the reference's), and its derivative reads s, which
+                # the reverse pass holds anyway, instead of testing a again
+                ax = self.emit("mul", [a, s])
                 sy = self.emit("mul", [s, b])
                 m = self.emit("tmin", [ax, sy])
                 mm = self.emit("tmax", [m, "eps"])
@@ -177,17 +182,45 @@ def branch_inputs(ops, guards):
     return need
 
 
-def primal_lines(params, inputs, ops, guards=None, fast=False):
+def decision_bits(ops):
+    """Bit (word, bit) of each tmin / tmax decision in the adjoint's packed
+    decision words."""
+    out = {}
+    for dst, op, _ in ops:
+        if op in ("tmin", "tmax"):
+            k = len(out)
+            out[dst] = (k // 32, k % 32)
+    return out
+
+
+def select_cond(op, a, b):
+    """The condition under which tmin / tmax (a, b) selects b."""
+    return f"({b}) < ({a})" if op == "tmin" else f"({a}) < ({b})"
+
+
+def primal_lines(params, inputs, ops, guards=None, fast=False, decisions=None):
     """Primal statements.  With guards (tangent and adjoint only), an op
     under a guard is evaluated only where the guard holds (its result feeds
     nothing else there); the plain primal evaluates every op.  fast: the
-    adjoint's primal arithmetic (C_OP_FAST off every branch's inputs)."""
+    adjoint's primal arithmetic (C_OP_FAST off every branch's inputs).
+    decisions: the adjoint's tmin / tmax decisions, recorded as bits of
+    integer words (one VGPR holds 32 of them) for the reverse pass to test,
+    instead of lane masks that would stay live across the face in scalar
+    registers."""
     exact = branch_inputs(ops, guards) if fast else None
 
     def C(dst, op):
         return C_OP_FAST[op] if fast and dst not in exact else C_OP[op]
+
+    def record(dst, op, args, indent):
+        if decisions and dst in decisions:
+            w, bit = decisions[dst]
+            return [f"{indent}dec{w} |= ({select_cond(op, *args)}) ? {1 << bit}u : 0u;"]
+        return []
     out = [f"    const double {p} = par[{i}];" for i, p in enumerate(params)]
     out += [f"    const double {v} = in[{i}];" for i, v in enumerate(inputs)]
+    if decisions:
+        out += [f"    unsigned dec{w} = 0u;" for w in sorted({w for w, _ in decisions.values()})]
     if guards is None:
         guards = [None] * len(ops)
     for g, idx in groups(guards):
@@ -195,12 +228,14 @@ def primal_lines(params, inputs, ops, guards=None, fast=False):
             for n in idx:
                 dst, op, args = ops[n]
                 out.append(f"    const double {dst} = {C(dst, op).format(*args)};")
+                out += record(dst, op, args, "    ")
         else:
             out += [f"    double {ops[n][0]} = 0.0;" for n in idx]
             out.append(f"    if ({g}) {{")
             for n in idx:
                 dst, op, args = ops[n]
                 out.append(f"    {dst} = {C(dst, op).format(*args)};")
+                out += record(dst, op, args, "    ")
             out.append("    }")
     return out
 
@@ -310,7 +345,7 @@ def replay_lines(params, inputs, ops):
     return out
 
 
-def reverse_op(dst, op, args, active, ren, first):
+def reverse_op(dst, op, args, active, ren, first, decisions=None):
     """Adjoint statements of one three-address op; ren maps a primal name to
     the local (recomputed) copy its value is read from.  first holds the
     adjoints already written in program order: the first contribution to
@@ -345,7 +380,11 @@ def reverse_op(dst, op, args, active, ren, first):
     elif op == "fabs":
         terms = [(a, f"({V(a)} > 0.0 ? {t} : ({V(a)} < 0.0 ? -{t} : 0.0))")]
     elif op in ("tmin", "tmax"):
-        sel = f"({V(b)}) < ({V(a)})" if op == "tmin" else f"({V(a)}) < ({V(b)})"
+        if decisions and dst in decisions:
+            w, bit = decisions[dst]
+            sel = f"(dec{w} & {1 << bit}u) != 0u"
+        else:
+            sel = select_cond(op, V(a), V(b))
         if active(a) and active(b):
             sb, sa = acc(b, t), acc(a, t)
             out.append(f"    if ({sel}) {sb} else {sa}")
@@ -385,7 +424,7 @@ def grad_live(ops, outputs):
     return live
 
 
-def adjoint_lines(params, inputs, outputs, ops, guards=None):
+def adjoint_lines(params, inputs, outputs, ops, guards=None, decisions=None):
     cst = set(params)
     live = grad_live(ops, outputs)
     active = lambda a: not (is_lit(a) or a in cst) and a in live
@@ -404,7 +443,7 @@ def adjoint_lines(params, inputs, outputs, ops, guards=None):
         for n in reversed(idx):
             dst, op, args = ops[n]
             if dst in live:
-                body += reverse_op(dst, op, args, active, {}, first)
+                body += reverse_op(dst, op, args, active, {}, first, decisions)
         if g is None:
             out += body
         else:
@@ -523,6 +562,16 @@ def main(argv):
             "#else",
             "    return sqrt(x);",
             "#endif",
+            "}",
+            "// x, as a value the compiler cannot see into (the adjoint's packed",
+            "// branch decisions: tested bit by bit in the reverse pass instead of",
+            "// keeping each comparison's lane mask live across the face).",
+            "MHD_HD inline unsigned mhd_ad_hide(unsigned x)",
+            "{",
+            "#if defined(__HIP_DEVICE_COMPILE__)",
+            "    asm volatile(\"\" : \"+v\"(x));",
+            "#endif",
+            "    return x;",
             "}",
             ""]
     for spec in argv[2:]:
