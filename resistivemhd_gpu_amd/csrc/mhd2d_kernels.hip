@@ -102,9 +102,11 @@ __device__ __forceinline__ Cell make_cell(double rho, double mx, double my, doub
 // no CT in the stages; the face fluxes the CT-contact EMF needs are stored
 // in `aux` (planes F0, F5 at x-faces and G0, G4 at y-faces, owned by the
 // cell on their low side) and the stage-2 state goes to uout != uold.
-// SCHEME 1 with etaf: the MPI library's getEta sampled per index, etaf the
-// x-face plane ((i + 1/2) dx, j dy), etaf + plane the y-face plane
-// (i dx, (j + 1/2) dy) (M/flux_solver.cu:84,168); null: the uniform ph.eta.
+// SCHEME 2: SCHEME 1 with the MPI library's getEta sampled per index, etaf
+// the x-face plane ((i + 1/2) dx, j dy), etaf + plane the y-face plane
+// (i dx, (j + 1/2) dy) (M/flux_solver.cu:84,168); SCHEME 1 uses the
+// uniform ph.eta (a separate instantiation, so the planes' loads and
+// registers cost it nothing).
 //
 // stage_item is one work item (strip sy, segment sx) of one run; the
 // single-run kernel and the batched work-list kernel both call it.
@@ -122,7 +124,9 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                                            double *__restrict__ aux, Ctrl *__restrict__ ctrl, const Grid &g,
                                            const Phys &ph, int cur)
 {
-    constexpr bool MULTI = SCHEME == 1;
+    constexpr bool MULTI = SCHEME >= 1;
+    constexpr bool ETAM = SCHEME == 2;        // the MPI library's getEta planes
+    constexpr bool ETA_PLANE = !MULTI || ETAM;   // etaf may be set (the uniform multi scheme never reads it)
     static_assert(!(TAIL && MULTI), "packed strips belong to the single-GPU scheme");
     const int lane = threadIdx.x;
     int sy, sx, row = lane, dcol = 0;
@@ -195,7 +199,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     // LDS per wavefront: 15,632 B in stage 1 and 18,800 B in stage 2, so
     // eight wavefronts (two per SIMD, the VGPR limit) fit a CU's 160 KiB.
     constexpr bool DMA = !TAIL;
-    __shared__ __attribute__((aligned(16))) double slot[MULTI ? kStagePlanes + 1 : kStagePlanes][kStageRows];
+    __shared__ __attribute__((aligned(16))) double slot[ETAM ? kStagePlanes + 1 : kStagePlanes][kStageRows];
     const int r0 = j0 - 3;
     const int rpair = min(r0 + 2 * lane, (g.ny - 2) & ~1);
     auto stage_col = [&](int col) {
@@ -208,9 +212,10 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             for (int v = 0; v < 8; ++v)
                 __builtin_amdgcn_global_load_lds((glb_void_t *)(uin + (size_t)v * g.plane + off),
                                                  (lds_void_t *)slot[v], 16, 0, 0);
-            if (etaf) __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + off), (lds_void_t *)slot[8], 16, 0, 0);
-            if (MULTI && etaf)
-                __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + g.plane + off), (lds_void_t *)slot[MULTI ? 9 : 8],
+            if (ETA_PLANE && etaf)
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + off), (lds_void_t *)slot[8], 16, 0, 0);
+            if (ETAM)
+                __builtin_amdgcn_global_load_lds((glb_void_t *)(etaf + g.plane + off), (lds_void_t *)slot[ETAM ? 9 : 8],
                                                  16, 0, 0);
         }
     };
@@ -254,9 +259,9 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     double dtloc = __longlong_as_double((long long)kInfBits);
     int nanflag = 0;
 
-    double eta_prev = 0.0;   // resistivity of cell c-1 (SCHEME 1: of x-face c-1 + 1/2)
-    double eta_prev2 = 0.0;  // SCHEME 1: x-face c-2 + 1/2
-    double etay_prev = 0.0;  // SCHEME 1: y-face (c-1, j + 1/2)
+    double eta_prev = 0.0;   // resistivity of cell c-1 (SCHEME 2: of x-face c-1 + 1/2)
+    double eta_prev2 = 0.0;  // SCHEME 2: x-face c-2 + 1/2
+    double etay_prev = 0.0;  // SCHEME 2: y-face (c-1, j + 1/2)
 
     for (int c = i0 - 2; c <= iend + 1; ++c) {
         double rho, mx, my, mz, bx, by, bz, e, bym, eta_c, etay_c = ph.eta;
@@ -266,8 +271,8 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             rho = slot[0][lane + 1]; mx = slot[1][lane + 1]; my = slot[2][lane + 1]; mz = slot[3][lane + 1];
             bx = slot[4][lane + 1]; by = slot[5][lane + 1]; bz = slot[6][lane + 1]; e = slot[7][lane + 1];
             bym = slot[5][lane];
-            eta_c = etaf ? slot[8][lane + 1] : ph.eta;
-            if (MULTI && etaf) etay_c = slot[MULTI ? 9 : 8][lane + 1];
+            eta_c = ETA_PLANE && etaf ? slot[8][lane + 1] : ph.eta;
+            if (ETAM) etay_c = slot[ETAM ? 9 : 8][lane + 1];
             // warm-up columns stage the next column at once; the others after
             // their x-face flux
             if (c < i0) {
@@ -344,7 +349,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     // face (c-2+1/2): cells A = c-2 and B = c-1 (M/flux_solver.cu:73-104)
                     const double fjY = EXACT ? -(B.bz - A.bz) / ph.dx : -(B.bz - A.bz) * (2.0 * inv_2dx);
                     const double fjZ = 0.5 * (((rTA - rTB) + tA) - tB);
-                    const double feta = etaf ? eta_prev2 : ph.eta;
+                    const double feta = ETAM ? eta_prev2 : ph.eta;
                     const double fJY = feta * fjY, fJZ = feta * fjZ;
                     const double fJYBZ = fJY * 0.5 * (A.bz + B.bz);
                     const double fJZBY = fJZ * 0.5 * (A.byc + B.byc);
@@ -409,7 +414,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     const double tA1 = lane_next(tA), tB1 = lane_next(tB);
                     const double gjX = EXACT ? (bz_up - B.bz) / ph.dy : (bz_up - B.bz) * (2.0 * inv_2dy);
                     const double gjZ = 0.5 * (((tA - tB) + tA1) - tB1);
-                    const double geta = etaf ? etay_prev : ph.eta;
+                    const double geta = ETAM ? etay_prev : ph.eta;
                     const double gJX = geta * gjX, gJZ = geta * gjZ;
                     const double gJXBZ = gJX * 0.5 * (B.bz + bz_up);
                     const double gJZBX = gJZ * 0.5 * (B.bxc + bxc_up);
@@ -592,7 +597,7 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
         }
         A = B;
         B = C;
-        if (MULTI) {
+        if (ETAM) {
             eta_prev2 = eta_prev;
             etay_prev = etay_c;
         }
@@ -1205,8 +1210,10 @@ hipError_t launch_stage(int stage, bool exact, int scheme, const double *uin, do
     do { if (exact) MHD_LAUNCH_STAGE(ST, true, SC); else MHD_LAUNCH_STAGE(ST, false, SC); } while (0)
     if (scheme == 0) {
         if (stage == 1) MHD_LAUNCH_STAGE_EX(1, 0); else MHD_LAUNCH_STAGE_EX(2, 0);
-    } else {
+    } else if (!eta) {
         if (stage == 1) MHD_LAUNCH_STAGE_EX(1, 1); else MHD_LAUNCH_STAGE_EX(2, 1);
+    } else {   // the multi-GPU scheme with getEta planes
+        if (stage == 1) MHD_LAUNCH_STAGE_EX(1, 2); else MHD_LAUNCH_STAGE_EX(2, 2);
     }
 #undef MHD_LAUNCH_STAGE_EX
 #undef MHD_LAUNCH_STAGE
