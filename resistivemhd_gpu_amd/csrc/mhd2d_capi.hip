@@ -46,7 +46,7 @@ struct mhd_handle {
     double *Us = nullptr;     // stage-1 state U*
     double *U2 = nullptr;     // MULTI_SHARE: stage-2 state (8 planes)
     double *aux = nullptr;    // MULTI_SHARE: face fluxes F0 F5 G0 G4 of stage 1, then of stage 2
-    double *ecE = nullptr;    // MULTI_SHARE: cell EMF of U^n, of U2, corner EMF E
+    double *ecE = nullptr;    // MULTI_SHARE: cell EMF of U^n and of U2
     double *eta = nullptr;    // optional per-cell resistivity plane
     double *eta_m = nullptr;  // MULTI_SHARE: getEta at x-faces, y-faces, cells (3 planes) or null
     double *side = nullptr;   // CT value of the bottom boundary-face By per column
@@ -613,9 +613,8 @@ int mhd_step_rk2(mhd_handle *h, int nsteps)
         if (m) {
             if ((rc = apply_bc_pair(h, h->U2, nullptr, 0))) return rc;
             HIPCHECK(h, mhd::launch_m_ec(false, h->U2, eta_c, h->ecE + plane, g, h->ph, h->stream));
-            HIPCHECK(h, mhd::launch_m_emf(aux1, aux2, h->ecE, h->ecE + plane, h->ecE + 2 * plane, g, h->stream));
-            HIPCHECK(h, mhd::launch_m_ct(h->exact, h->U, h->ecE + 2 * plane, h->U2, h->ctrl, h->cur ^ 1, g, h->ph,
-                                         h->stream));
+            HIPCHECK(h, mhd::launch_m_emf_ct(h->exact, aux1, aux2, h->ecE, h->ecE + plane, h->U, h->U2, h->ctrl,
+                                             h->cur ^ 1, g, h->ph, h->stream));
             if ((rc = apply_bc_pair(h, h->U2, nullptr, 0))) return rc;
             std::swap(h->U, h->U2);
         } else {
@@ -640,7 +639,7 @@ int mhd_set_scheme(mhd_handle *h, int scheme)
         if (!h->U2) {
             hipError_t e;
             if ((e = hipMalloc(&h->U2, 8 * plane)) != hipSuccess || (e = hipMalloc(&h->aux, 8 * plane)) != hipSuccess ||
-                (e = hipMalloc(&h->ecE, 3 * plane)) != hipSuccess) {
+                (e = hipMalloc(&h->ecE, 2 * plane)) != hipSuccess) {
                 (void)hipFree(h->U2);
                 (void)hipFree(h->aux);
                 (void)hipFree(h->ecE);
@@ -650,7 +649,7 @@ int mhd_set_scheme(mhd_handle *h, int scheme)
             }
             HIPCHECK(h, hipMemsetAsync(h->U2, 0, 8 * plane, h->stream));
             HIPCHECK(h, hipMemsetAsync(h->aux, 0, 8 * plane, h->stream));
-            HIPCHECK(h, hipMemsetAsync(h->ecE, 0, 3 * plane, h->stream));
+            HIPCHECK(h, hipMemsetAsync(h->ecE, 0, 2 * plane, h->stream));
             HIPCHECK(h, hipStreamSynchronize(h->stream));
         }
     }

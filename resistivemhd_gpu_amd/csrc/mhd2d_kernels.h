@@ -82,13 +82,13 @@ hipError_t launch_bc_y(int kind, bool exact, double *u, Ctrl *ctrl, int slot, co
 hipError_t launch_dt_all(bool exact, bool interior, const double *u, Ctrl *ctrl, int slot, const Grid &g,
                          const Phys &ph, hipStream_t s);
 // MPI library's CT-contact EMF: cell-centred EMF of a state (smoothed or as
-// stored), corner EMF from two flux sets, final CT with the next dt.
+// stored); corner EMF from two flux sets and the two cell EMFs, with the
+// final CT and the next dt in the same pass.
 hipError_t launch_m_ec(bool smooth, const double *u, const double *eta_c, double *ec, const Grid &g, const Phys &ph,
                       hipStream_t s);
-hipError_t launch_m_emf(const double *aux_old, const double *aux_now, const double *ec_old, const double *ec_now,
-                        double *E, const Grid &g, hipStream_t s);
-hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *u, Ctrl *ctrl, int slot,
-                       const Grid &g, const Phys &ph, hipStream_t s);
+hipError_t launch_m_emf_ct(bool exact, const double *aux_old, const double *aux_now, const double *ec_old,
+                           const double *ec_now, const double *uold, double *u, Ctrl *ctrl, int slot, const Grid &g,
+                           const Phys &ph, hipStream_t s);
 // Resident 64-lane stage workgroups per CU (the lower of the two stages).
 int stage_waves_per_cu(bool exact);
 // Batched stage over a work list of total_items items (item_run: run of

@@ -955,124 +955,249 @@ __global__ void ctrl_reset_kernel(Ctrl *ctrl, int slot)
 // ------------------------------------------------ MPI library's step (M/)
 // The CT-contact EMF of M/ct.cu and the final CT, on one periodic subdomain.
 
-// Face-smoothed B: shiftUToCenterForCT then backUToCenterHalfForCT
-// (M/resistiveMHD2D.cu:597-690) of the face values around plane offset o.
-__device__ __forceinline__ double smooth_bx(const double *bx, size_t o, size_t pitch)
+// Columns per wavefront of the column-walking kernels below (smoothed
+// cell EMF; corner EMF with the final CT).
+#ifndef MHD_EC_COLUMNS
+#define MHD_EC_COLUMNS 8
+#endif
+#ifndef MHD_CT_COLUMNS
+#define MHD_CT_COLUMNS 32
+#endif
+constexpr int kEcSeg = MHD_EC_COLUMNS;
+constexpr int kMSeg = MHD_CT_COLUMNS;
+
+// Face value smoothed along its normal (shiftUToCenterForCT then
+// backUToCenterHalfForCT, M/resistiveMHD2D.cu:597-690) from the face values
+// before (m), at (c) and after (p) it.
+__device__ __forceinline__ double smooth3(double m, double c, double p)
 {
-    const double c0 = 0.5 * (bx[o] + bx[o - pitch]);
-    const double c1 = 0.5 * (bx[o + pitch] + bx[o]);
-    return 0.5 * (c0 + c1);
-}
-__device__ __forceinline__ double smooth_by(const double *by, size_t o)
-{
-    const double c0 = 0.5 * (by[o] + by[o - 1]);
-    const double c1 = 0.5 * (by[o + 1] + by[o]);
+    const double c0 = 0.5 * (c + m);
+    const double c1 = 0.5 * (p + c);
     return 0.5 * (c0 + c1);
 }
 
 // Cell-centred reference EMF u*bY - v*bX - eta*jZ of CT::setOldFlux2D /
-// setNowFlux2D (M/ct.cu:26-65; the G4 array there is its negation) over the
-// interior.  SMOOTH: of the face-smoothed state (stage 1 sees U^n after
-// backUToCenterHalfForCT); otherwise of the state as stored.
-template <bool SMOOTH>
+// setNowFlux2D (M/ct.cu:26-65; the G4 array there is its negation) at
+// cell (i, j) from its centred state and the face B around it: X0, X1 the
+// x-face B of columns i-1 and i at rows j-1 .. j+1, Y0, Y1, Y2 the y-face B
+// of columns i-1 .. i+1 at rows j-1 and j.
+__device__ __forceinline__ double cell_emf(const double *u, const double *eta_c, size_t o, const double *X0,
+                                           const double *X1, const double *Y0, const double *Y1, const double *Y2,
+                                           const Grid &g, const Phys &ph)
+{
+    const double rho = PL(u, 0)[o];
+    const double uu = PL(u, 1)[o] / rho;
+    const double vv = PL(u, 2)[o] / rho;
+    const double bX = 0.5 * (X1[1] + X0[1]);
+    const double bY = 0.5 * (Y1[1] + Y1[0]);
+    const double jZ = 0.25 * (
+          (Y2[1] - Y1[1]) / ph.dx - (X1[2] - X1[1]) / ph.dy
+        + (Y2[0] - Y1[0]) / ph.dx - (X1[1] - X1[0]) / ph.dy
+        + (Y1[0] - Y0[0]) / ph.dx - (X0[1] - X0[0]) / ph.dy
+        + (Y1[1] - Y0[1]) / ph.dx - (X0[2] - X0[1]) / ph.dy);
+    const double eta = eta_c ? eta_c[o] : ph.eta;   // getEta(i dx, j dy), M/ct.cu:41,56
+    return uu * bY - vv * bX - eta * jZ;
+}
+
+// Cell EMF of the state as stored, over the interior: one cell per lane,
+// a workgroup per 256 rows of one column (neighbouring columns' workgroups
+// run together, so the face values read around a cell come from L2).
 __global__ __launch_bounds__(256) void m_ec_kernel(const double *__restrict__ u, const double *__restrict__ eta_c,
                                                    double *__restrict__ ec, Grid g, Phys ph)
 {
     const int j = kGhost + blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= g.ny - kGhost) return;
-    const double *bx = PL(u, 4), *by = PL(u, 5);
     const size_t P = g.pitch;
-    auto BX = [&](size_t q) { return SMOOTH ? smooth_bx(bx, q, P) : bx[q]; };
-    auto BY = [&](size_t q) { return SMOOTH ? smooth_by(by, q) : by[q]; };
+    const double *bx = PL(u, 4), *by = PL(u, 5);
     for (int i = kGhost + blockIdx.y; i < g.nx - kGhost; i += gridDim.y) {
         const size_t o = (size_t)i * P + j;
-        const double rho = PL(u, 0)[o];
-        const double uu = PL(u, 1)[o] / rho;
-        const double vv = PL(u, 2)[o] / rho;
-        const double bX = 0.5 * (BX(o) + BX(o - P));
-        const double bY = 0.5 * (BY(o) + BY(o - 1));
-        const double jZ = 0.25 * (
-              (BY(o + P) - BY(o)) / ph.dx - (BX(o + 1) - BX(o)) / ph.dy
-            + (BY(o - 1 + P) - BY(o - 1)) / ph.dx - (BX(o) - BX(o - 1)) / ph.dy
-            + (BY(o - 1) - BY(o - 1 - P)) / ph.dx - (BX(o - P) - BX(o - P - 1)) / ph.dy
-            + (BY(o) - BY(o - P)) / ph.dx - (BX(o + 1 - P) - BX(o - P)) / ph.dy);
-        const double eta = eta_c ? eta_c[o] : ph.eta;   // getEta(i dx, j dy), M/ct.cu:41,56
-        ec[o] = uu * bY - vv * bX - eta * jZ;
+        double X0[3], X1[3], Y0[2], Y1[2], Y2[2];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            X0[q] = bx[o - P - 1 + q];
+            X1[q] = bx[o - 1 + q];
+        }
+        Y0[0] = by[o - P - 1]; Y0[1] = by[o - P];
+        Y1[0] = by[o - 1];     Y1[1] = by[o];
+        Y2[0] = by[o + P - 1]; Y2[1] = by[o + P];
+        ec[o] = cell_emf(u, eta_c, o, X0, X1, Y0, Y1, Y2, g, ph);
+    }
+}
+
+// Cell EMF of the face-smoothed state (stage 1 sees U^n after
+// backUToCenterHalfForCT), over the interior.  Each smoothed face value
+// takes three raw ones, so one lane per row walks kEcSeg columns and
+// carries what it has formed: the smoothed X0, X1, Y0, Y1 and the raw bx of
+// columns i and i+1 (the raw by of rows j-2 .. j+1 of column i+1 is read
+// for Y2), so each face value is read once per lane.
+__global__ __launch_bounds__(256) void m_ec_smooth_kernel(const double *__restrict__ u,
+                                                          const double *__restrict__ eta_c, double *__restrict__ ec,
+                                                          Grid g, Phys ph)
+{
+    const int jj = kGhost + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = jj < g.ny - kGhost;
+    const int j = on ? jj : kGhost;   // lanes past the interior read row kGhost's stencil and store nothing
+    const int ib = kGhost + blockIdx.y * kEcSeg, ie = min(ib + kEcSeg, g.nx - kGhost);
+    const size_t P = g.pitch;
+    const double *bx = PL(u, 4), *by = PL(u, 5);
+    auto raw = [&](int c, double *r) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) r[q] = bx[(size_t)c * P + j - 1 + q];
+    };
+    auto col_y = [&](int c, double *y) {
+        const double *b = by + (size_t)c * P + j;
+        y[0] = smooth3(b[-2], b[-1], b[0]);
+        y[1] = smooth3(b[-1], b[0], b[1]);
+    };
+    double X0[3], X1[3], Y0[2], Y1[2], Y2[2], r0[3], r1[3];
+    {
+        double a[3], b[3];
+        raw(ib - 2, a);
+        raw(ib - 1, b);
+        raw(ib, r0);
+        raw(ib + 1, r1);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            X0[q] = smooth3(a[q], b[q], r0[q]);
+            X1[q] = smooth3(b[q], r0[q], r1[q]);
+        }
+    }
+    col_y(ib - 1, Y0);
+    col_y(ib, Y1);
+    for (int i = ib; i < ie; ++i) {
+        col_y(i + 1, Y2);
+        if (on) {
+            const size_t o = (size_t)i * P + j;
+            ec[o] = cell_emf(u, eta_c, o, X0, X1, Y0, Y1, Y2, g, ph);
+        }
+        if (i + 1 == ie) break;
+        double r2[3];
+        raw(i + 2, r2);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            X0[q] = X1[q];
+            X1[q] = smooth3(r0[q], r1[q], r2[q]);
+            r0[q] = r1[q];
+            r1[q] = r2[q];
+        }
+        Y0[0] = Y1[0]; Y0[1] = Y1[1];
+        Y1[0] = Y2[0]; Y1[1] = Y2[1];
     }
 }
 
 __device__ __forceinline__ double msign(double x) { return (double)((x > 0.0) - (x < 0.0)); }   // M/sign.cu
 
-// Corner EMF of one flux set (M/ct.cu:147-169).  aux planes: 0 = F.f0,
-// 1 = F.f5 (x-faces), 2 = G.f0, 3 = G.f4 (y-faces); ec = F5, G4 = -ec.
-// k, kN, kE, kNE: offsets of (i, j), (i, j+1), (i+1, j), (i+1, j+1).
-__device__ __forceinline__ double contact_emf(const double *aux, const double *ec, long long plane, size_t k,
-                                              size_t kN, size_t kE, size_t kNE)
+// Corner EMF of one flux set at (i+1/2, j+1/2) (M/ct.cu:147-169) from the
+// x-face values F.f0, F.f5 of (i, j) and (i, j+1) (f0, f0N, f5, f5N), the
+// y-face values G.f0, G.f4 of (i, j) and (i+1, j) (g0, g4, g0E, g4E) and
+// the cell EMFs ec = F5 = -G4 of (i, j), (i, j+1), (i+1, j), (i+1, j+1).
+struct CornerIn {
+    double f0, f0N, f5, f5N, g0, g4, g0E, g4E, ec, ecN, ecE, ecNE;
+};
+
+__device__ __forceinline__ double contact_emf(const CornerIn &c)
 {
-    const double *NF0 = aux, *NF5 = aux + plane, *NG0 = aux + 2 * plane, *NG4 = aux + 3 * plane;
-    const double avg = 0.25 * (-NF5[k] - NF5[kN] + NG4[k] + NG4[kE]);
-    const double sS = msign(NF0[k]), sN = msign(NF0[kN]);
-    const double sW = msign(NG0[k]), sE = msign(NG0[kE]);
-    const double eS = (1.0 + sS) * (NG4[k] - (-ec[k])) + (1.0 - sS) * (NG4[kE] - (-ec[kE]));
-    const double eN = (1.0 + sN) * ((-ec[kN]) - NG4[k]) + (1.0 - sN) * ((-ec[kNE]) - NG4[kE]);
-    const double eW = -(1.0 + sW) * (NF5[k] - ec[k]) - (1.0 - sW) * (NF5[kN] - ec[kN]);
-    const double eE = -(1.0 + sE) * (ec[kE] - NF5[k]) - (1.0 - sE) * (ec[kNE] - NF5[kN]);
+    const double avg = 0.25 * (-c.f5 - c.f5N + c.g4 + c.g4E);
+    const double sS = msign(c.f0), sN = msign(c.f0N);
+    const double sW = msign(c.g0), sE = msign(c.g0E);
+    const double eS = (1.0 + sS) * (c.g4 - (-c.ec)) + (1.0 - sS) * (c.g4E - (-c.ecE));
+    const double eN = (1.0 + sN) * ((-c.ecN) - c.g4) + (1.0 - sN) * ((-c.ecNE) - c.g4E);
+    const double eW = -(1.0 + sW) * (c.f5 - c.ec) - (1.0 - sW) * (c.f5N - c.ecN);
+    const double eE = -(1.0 + sE) * (c.ecE - c.f5) - (1.0 - sE) * (c.ecNE - c.f5N);
     return avg + 0.125 * (eS - eN + eW - eE);
 }
 
-// E = (E_old + E_now) / 2 at corners (i+1/2, j+1/2), i in [2, nx-3),
-// j in [2, ny-3) (CT::divBClean, M/ct.cu:120-218).  The per-face and
-// per-cell inputs are stored for the interior only; the ghost layer's values
-// are those of the periodic image, which the self-exchange halo makes equal.
-__global__ __launch_bounds__(256) void m_emf_kernel(const double *__restrict__ aux_old,
-                                                    const double *__restrict__ aux_now,
-                                                    const double *__restrict__ ec_old,
-                                                    const double *__restrict__ ec_now, double *__restrict__ E, Grid g)
+// Rows of the interior each wavefront of m_emf_ct_kernel updates.
+constexpr int kMCtRows = 63;
+
+// The corner EMF E = (E_old + E_now) / 2 at (i+1/2, j+1/2) (CT::divBClean,
+// M/ct.cu:120-218) and the final CT from the step's initial face B
+// (M/ct.cu:172-218) over the interior, then the per-cell dt and NaN flag of
+// the new state (the next step's calculateDt, M/resistiveMHD2D.cu:489-524),
+// in one pass: a wavefront takes kMCtRows rows and walks kMSeg columns.
+// Lane L holds corner row jb - 1 + L; at column i it forms its corner's
+// E(i, row) and updates cell (i, row) with E(i, row - 1) from the lane
+// below (lane 0 only supplies that value) and E(i - 1, row) from the column
+// before.  Column i+1's y-face values and cell EMFs, read for corner i,
+// are carried to corner i+1, so each input is read once per lane.  The
+// per-face and per-cell inputs are stored for the interior only; the ghost
+// layer's values are those of the periodic image, which the self-exchange
+// halo makes equal.
+template <bool EXACT>
+__global__ __launch_bounds__(256) void m_emf_ct_kernel(const double *__restrict__ aux_old,
+                                                       const double *__restrict__ aux_now,
+                                                       const double *__restrict__ ec_old,
+                                                       const double *__restrict__ ec_now,
+                                                       const double *__restrict__ uold, double *__restrict__ u,
+                                                       Ctrl *__restrict__ ctrl, int slot, Grid g, Phys ph)
 {
-    const int j = kGhost - 1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= g.ny - kGhost) return;
+    const int lane = threadIdx.x & 63;
+    const int jb = kGhost + kMCtRows * (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    if (jb >= g.ny - kGhost) return;   // whole wavefront past the interior
     const int lnx = g.nx - 2 * kGhost, lny = g.ny - 2 * kGhost;
     auto wrap = [](int a, int n, int l) { return a < kGhost ? a + l : (a >= n - kGhost ? a - l : a); };
+    const bool on = jb - 1 + lane < g.ny - kGhost;   // corner rows kGhost-1 .. ny-kGhost-1
+    const int j = on ? jb - 1 + lane : jb;           // lanes past the interior: a valid row, nothing stored
+    const bool out = on && lane > 0;
     const int wj = wrap(j, g.ny, lny), wj1 = wrap(j + 1, g.ny, lny);
-    for (int i = kGhost - 1 + blockIdx.y; i < g.nx - kGhost; i += gridDim.y) {
-        const size_t wi = (size_t)wrap(i, g.nx, lnx) * g.pitch, wi1 = (size_t)wrap(i + 1, g.nx, lnx) * g.pitch;
-        const size_t k = wi + wj, kN = wi + wj1, kE = wi1 + wj, kNE = wi1 + wj1;
-        const double eOld = contact_emf(aux_old, ec_old, g.plane, k, kN, kE, kNE);
-        const double eNow = contact_emf(aux_now, ec_now, g.plane, k, kN, kE, kNE);
-        E[(size_t)i * g.pitch + j] = 0.5 * (eOld + eNow);
+    const int ib = kGhost + blockIdx.y * kMSeg, ie = min(ib + kMSeg, g.nx - kGhost);
+    const long long plane = g.plane;
+    const double *aux[2] = {aux_old, aux_now}, *ecp[2] = {ec_old, ec_now};
+    CornerIn c[2];
+    {
+        const size_t wi = (size_t)wrap(ib - 1, g.nx, lnx) * g.pitch;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            c[s].g0E = aux[s][2 * plane + wi + wj];
+            c[s].g4E = aux[s][3 * plane + wi + wj];
+            c[s].ecE = ecp[s][wi + wj];
+            c[s].ecNE = ecp[s][wi + wj1];
+        }
     }
-}
-
-// Final CT from the step's initial face B (M/ct.cu:172-218) over the
-// interior, then the per-cell dt and NaN flag of the new state (the next
-// step's calculateDt, M/resistiveMHD2D.cu:489-524).
-template <bool EXACT>
-__global__ __launch_bounds__(256) void m_ct_kernel(const double *__restrict__ uold, const double *__restrict__ E,
-                                                   double *__restrict__ u, Ctrl *__restrict__ ctrl, int slot, Grid g,
-                                                   Phys ph)
-{
-    const int j = kGhost + blockIdx.x * blockDim.x + threadIdx.x;
     const double dt = ctrl->dt;
     double dloc = __longlong_as_double((long long)kInfBits);
     int nanflag = 0;
-    if (j < g.ny - kGhost) {
-        for (int i = kGhost + blockIdx.y; i < g.nx - kGhost; i += gridDim.y) {
+    double eW = 0.0;
+    for (int i = ib - 1; i < ie; ++i) {
+        const size_t wi = (size_t)wrap(i, g.nx, lnx) * g.pitch, wi1 = (size_t)wrap(i + 1, g.nx, lnx) * g.pitch;
+        double es[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            CornerIn &q = c[s];
+            q.g0 = q.g0E;
+            q.g4 = q.g4E;
+            q.ec = q.ecE;
+            q.ecN = q.ecNE;
+            q.f0 = aux[s][wi + wj];
+            q.f0N = aux[s][wi + wj1];
+            q.f5 = aux[s][plane + wi + wj];
+            q.f5N = aux[s][plane + wi + wj1];
+            q.g0E = aux[s][2 * plane + wi1 + wj];
+            q.g4E = aux[s][3 * plane + wi1 + wj];
+            q.ecE = ecp[s][wi1 + wj];
+            q.ecNE = ecp[s][wi1 + wj1];
+            es[s] = contact_emf(q);
+        }
+        const double e = 0.5 * (es[0] + es[1]);
+        const double eS = lane_prev(e);   // every lane active: the loop is uniform
+        if (i >= ib && out) {
             const size_t o = (size_t)i * g.pitch + j;
-            const double nbx = PL(uold, 4)[o] - dt / ph.dy * (E[o] - E[o - 1]);
-            const double nby = PL(uold, 5)[o] + dt / ph.dx * (E[o] - E[o - g.pitch]);
+            const double nbx = PL(uold, 4)[o] - dt / ph.dy * (e - eS);
+            const double nby = PL(uold, 5)[o] + dt / ph.dx * (e - eW);
             PL(u, 4)[o] = nbx;
             PL(u, 5)[o] = nby;
-            const double e = PL(u, 7)[o];
+            const double en = PL(u, 7)[o];
             const double d = cell_dt<EXACT>(PL(u, 0)[o], PL(u, 1)[o], PL(u, 2)[o], PL(u, 3)[o], nbx, nby, PL(u, 6)[o],
-                                            e, ph);
+                                            en, ph);
             if (d < dloc) dloc = d;
             if (i == kGhost && j == kGhost && isnan(d)) ctrl->first_nan[slot] = 1;
-            nanflag |= isnan(e) ? 1 : 0;
+            nanflag |= isnan(en) ? 1 : 0;
         }
+        eW = e;
     }
     accumulate_dt(ctrl, slot, dloc);
     const unsigned long long any = __ballot(nanflag);
-    if ((threadIdx.x & 63) == 0 && any) atomicOr(&ctrl->crashed, 1);
+    if (lane == 0 && any) atomicOr(&ctrl->crashed, 1);
 }
 
 // AoS (reference ConservationParameter order) <-> SoA planes.
@@ -1297,28 +1422,27 @@ static dim3 interior_grid(const Grid &g, int extra)
 hipError_t launch_m_ec(bool smooth, const double *u, const double *eta_c, double *ec, const Grid &g, const Phys &ph,
                       hipStream_t s)
 {
-    if (smooth) hipLaunchKernelGGL((m_ec_kernel<true>), interior_grid(g, 0), dim3(256), 0, s, u, eta_c, ec, g, ph);
-    else hipLaunchKernelGGL((m_ec_kernel<false>), interior_grid(g, 0), dim3(256), 0, s, u, eta_c, ec, g, ph);
+    if (smooth) {
+        const dim3 grid((g.ny - 2 * kGhost + 255) / 256, (g.nx - 2 * kGhost + kEcSeg - 1) / kEcSeg);
+        hipLaunchKernelGGL(m_ec_smooth_kernel, grid, dim3(256), 0, s, u, eta_c, ec, g, ph);
+    } else {
+        hipLaunchKernelGGL(m_ec_kernel, interior_grid(g, 0), dim3(256), 0, s, u, eta_c, ec, g, ph);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_m_emf(const double *aux_old, const double *aux_now, const double *ec_old, const double *ec_now,
-                        double *E, const Grid &g, hipStream_t s)
+hipError_t launch_m_emf_ct(bool exact, const double *aux_old, const double *aux_now, const double *ec_old,
+                           const double *ec_now, const double *uold, double *u, Ctrl *ctrl, int slot, const Grid &g,
+                           const Phys &ph, hipStream_t s)
 {
-    hipLaunchKernelGGL(m_emf_kernel, interior_grid(g, 1), dim3(256), 0, s, aux_old, aux_now, ec_old, ec_now, E, g);
-    return hipGetLastError();
-}
-
-hipError_t launch_m_ct(bool exact, const double *uold, const double *E, double *u, Ctrl *ctrl, int slot,
-                       const Grid &g, const Phys &ph, hipStream_t s)
-{
-    // each thread folds many rows before the one dt atomic per wavefront:
-    // about 8k workgroups in all
-    dim3 grid = interior_grid(g, 0);
-    const unsigned rows = (unsigned)(8192 / grid.x > 0 ? 8192 / grid.x : 1);
-    if (grid.y > rows) grid.y = rows;
-    if (exact) hipLaunchKernelGGL((m_ct_kernel<true>), grid, dim3(256), 0, s, uold, E, u, ctrl, slot, g, ph);
-    else hipLaunchKernelGGL((m_ct_kernel<false>), grid, dim3(256), 0, s, uold, E, u, ctrl, slot, g, ph);
+    const int waves = (g.ny - 2 * kGhost + kMCtRows - 1) / kMCtRows;
+    const dim3 grid((waves + 3) / 4, (g.nx - 2 * kGhost + kMSeg - 1) / kMSeg);
+    if (exact)
+        hipLaunchKernelGGL((m_emf_ct_kernel<true>), grid, dim3(256), 0, s, aux_old, aux_now, ec_old, ec_now, uold, u,
+                           ctrl, slot, g, ph);
+    else
+        hipLaunchKernelGGL((m_emf_ct_kernel<false>), grid, dim3(256), 0, s, aux_old, aux_now, ec_old, ec_now, uold, u,
+                           ctrl, slot, g, ph);
     return hipGetLastError();
 }
 
