@@ -213,7 +213,7 @@ def batch_bench(nruns, steps, plain_steps, run_steps, numerics):
     }
 
 
-def adjoint_bench(nruns, steps, run_steps):
+def adjoint_bench(nruns, steps, run_steps, batched_only=False):
     """Runs/s of batched forward+adjoint against a loop of single-run
     forward+adjoint objects, on the batched workload.  States stay on the
     GPU: dL/dU_N = U_N as the seed, moved with the device-pointer transfers
@@ -223,7 +223,8 @@ def adjoint_bench(nruns, steps, run_steps):
     state; a first call with a new step count also sizes the dt table and
     uploads the per-slot descriptors, which costs a single-run object about
     a millisecond of host round trips).  runs/s = runs * (steps / run_steps)
-    / seconds."""
+    / seconds.  batched_only: the batched leg alone (for profiling it;
+    no plain loop, no ratio)."""
     import numpy as np
     import torch
     import resistivemhd_gpu_amd as mhd
@@ -251,6 +252,11 @@ def adjoint_bench(nruns, steps, run_steps):
     t_batch = time.perf_counter() - t0
     finite = all(np.isfinite(a.grad_state(r)).all() for r in range(0, nruns, 17))
     del a, buf
+    if batched_only:
+        return {"runs": nruns, "steps_per_run": run_steps, "timed_steps": steps,
+                "batched_fwd_adj_runs_per_s": nruns * steps / run_steps / t_batch, "batched_s": t_batch,
+                "gradients_finite": bool(finite), "autotuned_sweep_columns": cols,
+                "note": "batched leg only, after a warm-up of the same step count"}
 
     singles = []
     for c, kind, seed in runs:
@@ -591,6 +597,8 @@ def main():
     ap.add_argument("--workload", choices=["harris", "turbulence", "batch", "adjoint", "gradcheck"],
                     default="harris")
     ap.add_argument("--adj-steps", type=int, default=20, help="timed forward+adjoint steps per run")
+    ap.add_argument("--adj-batched-only", action="store_true",
+                    help="adjoint workload: time the batched leg only (no plain loop, no ratio)")
     ap.add_argument("--check-nx", type=int, default=1024)
     ap.add_argument("--check-ny", type=int, default=512)
     ap.add_argument("--check-runs", type=int, default=32)
@@ -632,7 +640,7 @@ def main():
     import resistivemhd_gpu_amd as mhd
 
     if args.workload == "adjoint":
-        res = adjoint_bench(args.batch_runs, args.adj_steps, 500)
+        res = adjoint_bench(args.batch_runs, args.adj_steps, 500, args.adj_batched_only)
         print(json.dumps({"metric": "batched forward+adjoint runs/s vs plain per-instance loop",
                           "value": res["batched_fwd_adj_runs_per_s"], "unit": "runs/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
