@@ -486,10 +486,9 @@ __global__ __launch_bounds__(64) void face_adj_tail_kernel(const AdjDesc *runs, 
 // LDS ring of five columns and leave for HBM once per column when
 // complete, instead of with 38 atomics per face.
 constexpr int kSwRows = 68;   // LDS rows: global rows j0-2 .. j0+65
-constexpr int kSwSt = 5;      // state columns i-2 .. i+2
-constexpr int kSwRing = 2;    // adjoint / resistivity columns
+constexpr int kSwSt = 6;      // state columns i-2 .. i+2, and i+3 landing
+constexpr int kSwRing = 3;    // adjoint / resistivity columns i, i+1 and i+2 landing; corner EMF i-1, i, i+1
 constexpr int kSwAcc = 5;     // accumulator columns i-2 .. i+2
-constexpr int kSwStage = 16;  // staged planes: 8 state, 6 adjoint, corner EMF, resistivity
 
 struct __attribute__((aligned(16))) SweepLds {   // every row 16-byte aligned
     double st[8][kSwSt][kSwRows];
@@ -617,24 +616,24 @@ __device__ __forceinline__ double *sweep_slot(SweepLds &l, int p, int col)
     return l.et[col % kSwRing];
 }
 
-// One wave's share of the loads of a step: wave 0 the 8 state planes of
-// column cs, wave 1 the six adjoint planes and the resistivity of column cl
-// and the corner-EMF adjoint of column ce.  dst_* pick the LDS target
-// (staging or ring slot) per plane.
-template <int STAGE, class Dst>
-__device__ __forceinline__ void sweep_issue(const SweepRun &g, int wave, int cs, int cl, int ce, int j0, int lane,
-                                            Dst dst)
+// One wave's share of the loads of a step, straight into their ring slots:
+// wave 0 the 8 state planes of column cs, wave 1 the six adjoint planes and
+// the resistivity of column cl and the corner-EMF adjoint of column ce.
+template <int STAGE>
+__device__ __forceinline__ void sweep_issue(SweepLds &l, const SweepRun &g, int wave, int cs, int cl, int ce, int j0,
+                                            int lane)
 {
     if (wave == 0) {
         const double *base = g.prim + (size_t)cs * g.pitch;
 #pragma unroll
-        for (int v = 0; v < 8; ++v) sweep_dma(base + v * g.plane, j0, g.ny, dst(v, cs), lane);
+        for (int v = 0; v < 8; ++v) sweep_dma(base + v * g.plane, j0, g.ny, sweep_slot(l, v, cs), lane);
     } else {
         const double *base = g.L + (size_t)cl * g.pitch;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) sweep_dma(base + (q < 4 ? q : q + 2) * g.plane, j0, g.ny, dst(8 + q, cl), lane);
-        if (STAGE == 1) sweep_dma(g.lamE + (size_t)ce * g.pitch, j0, g.ny, dst(14, ce), lane);
-        if (g.eta) sweep_dma(g.eta + (size_t)cl * g.pitch, j0, g.ny, dst(15, cl), lane);
+        for (int q = 0; q < 6; ++q)
+            sweep_dma(base + (q < 4 ? q : q + 2) * g.plane, j0, g.ny, sweep_slot(l, 8 + q, cl), lane);
+        if (STAGE == 1) sweep_dma(g.lamE + (size_t)ce * g.pitch, j0, g.ny, sweep_slot(l, 14, ce), lane);
+        if (g.eta) sweep_dma(g.eta + (size_t)cl * g.pitch, j0, g.ny, sweep_slot(l, 15, cl), lane);
     }
 }
 
@@ -643,7 +642,6 @@ __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs
                                                              int cols, int step, int slot, int ck)
 {
     __shared__ SweepLds l;
-    __shared__ __attribute__((aligned(16))) double stg[kSwStage][kSwRows];
     int r, j0, ia, ib;
     if (!sweep_item(runs, sfirst, nruns, cols, r, j0, ia, ib)) return;
     r = __builtin_amdgcn_readfirstlane(r);
@@ -702,24 +700,12 @@ __global__ __launch_bounds__(128) void face_adj_sweep_kernel(const AdjDesc *runs
         const int j = j0 + lane;
         wait_vm_all();       // this wave's loads for step i landed
         __syncthreads();     // and the other wave's; step i-1 done
-        if (i > ia) {
-            // the staged columns into their ring slots, whose previous
-            // columns no wave reads from step i on
-            {
-                // 8 threads per plane, 16-byte moves: 34 row pairs each
-                const int p = tid >> 3;
-                const int c = p < 8 ? i + 2 : (p == 14 ? i : i + 1);
-                double2 *to = reinterpret_cast<double2 *>(sweep_slot(l, p, c));
-                const double2 *from = reinterpret_cast<const double2 *>(stg[p]);
-                for (int q = tid & 7; q < kSwRows / 2; q += 8) to[q] = from[q];
-            }
-            __syncthreads();
-        }
         if (i - 2 >= ia - 1) sweep_flush(l, g, i - 2, j0, tid);
         if (i + 1 < ib) {
-            // the columns step i+1 adds: state i+3, adjoint and resistivity
-            // i+2, corner-EMF adjoint i+1
-            sweep_issue<STAGE>(g, wave, i + 3, i + 2, i + 1, j0, lane, [&](int p, int) { return stg[p]; });
+            // the columns step i+1 adds, into ring slots no wave reads at
+            // step i: state i+3 (slot of i-3), adjoint and resistivity i+2
+            // (slot of i-1), corner-EMF adjoint i+1 (slot of i-2)
+            sweep_issue<STAGE>(l, g, wave, i + 3, i + 2, i + 1, j0, lane);
         }
 
         const bool valid = wave == 0 ? (i <= g.nx - 4 && j >= 2 && j <= g.ny - 3)
