@@ -590,14 +590,27 @@ struct SweepRun {
 __device__ __forceinline__ void sweep_flush(SweepLds &l, const SweepRun &g, int m, int j0, int tid)
 {
     const int sa = m % kSwAcc;
-    // components 0..7: 16 threads each; the resistivity adjoint: 68 threads
+    // components 0..7: 16 threads each, rows tid & 15 + 16 k (five passes,
+    // unrolled so that the five LDS reads issue together); the resistivity
+    // adjoint: 68 threads
     const int v = tid >> 4;
     double *dst = g.lamC + v * g.plane + (size_t)m * g.pitch + (j0 - 2);
-    for (int r = tid & 15; r < kSwRows; r += 16) {
-        const double b = l.acc[v][sa][r];
-        l.acc[v][sa][r] = 0.0;
+    double b[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int r = (tid & 15) + 16 * k;
+        b[k] = r < kSwRows ? l.acc[v][sa][r] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int r = (tid & 15) + 16 * k;
+        if (r < kSwRows) l.acc[v][sa][r] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const int r = (tid & 15) + 16 * k;
         const int gj = j0 - 2 + r;
-        if (b != 0.0 && gj >= 0 && gj < g.ny) unsafeAtomicAdd(dst + r, b);
+        if (b[k] != 0.0 && gj >= 0 && gj < g.ny) unsafeAtomicAdd(dst + r, b[k]);
     }
     if (tid < kSwRows) {
         const double b = l.acc[8][sa][tid];
