@@ -223,12 +223,14 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     // staged before the y-face flux of the same column
     constexpr bool UPD_DMA = DMA && !MULTI;
     __shared__ __attribute__((aligned(16))) double upd[UPD_DMA ? (STAGE == 2 ? 14 : 8) : 1][kStageRows];
-    // the previous column's face fluxes (the components the update reads),
-    // one slot per lane, instead of registers held across both HLLD solves
-    constexpr bool FLDS = DMA && !MULTI;
-    constexpr bool GLDS = DMA && !MULTI;
-    __shared__ double fb[FLDS ? 6 : 1][64];
-    __shared__ double gb[GLDS ? 7 : 1][64];
+    // the previous column's face fluxes (the components the update reads:
+    // six of F and seven of G here, all eight of each in the MPI library's
+    // step, whose B advances by flux divergence), one slot per lane, instead
+    // of registers held across both HLLD solves
+    constexpr bool FLDS = DMA;
+    constexpr bool GLDS = DMA;
+    __shared__ double fb[FLDS ? (MULTI ? 8 : 6) : 1][64];
+    __shared__ double gb[GLDS ? (MULTI ? 8 : 7) : 1][64];
     auto stage_upd = [&](int col) {
         if (lane < kStageRows / 2) {
             // opaque, so the lane offset is not folded into one loop-invariant
@@ -442,11 +444,17 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             if (c >= i0 + 2) {
                 const int s = c - 2;
                 Flux8 Fp = Fprev, Gp = Gprev;
-                if (FLDS) {
+                if (FLDS && MULTI) {
+                    Fp.f0 = fb[0][lane]; Fp.f1 = fb[1][lane]; Fp.f2 = fb[2][lane]; Fp.f3 = fb[3][lane];
+                    Fp.f4 = fb[4][lane]; Fp.f5 = fb[5][lane]; Fp.f6 = fb[6][lane]; Fp.f7 = fb[7][lane];
+                } else if (FLDS) {
                     Fp.f0 = fb[0][lane]; Fp.f1 = fb[1][lane]; Fp.f2 = fb[2][lane];
                     Fp.f3 = fb[3][lane]; Fp.f6 = fb[4][lane]; Fp.f7 = fb[5][lane];
                 }
-                if (GLDS) {
+                if (GLDS && MULTI) {
+                    Gp.f0 = gb[0][lane]; Gp.f1 = gb[1][lane]; Gp.f2 = gb[2][lane]; Gp.f3 = gb[3][lane];
+                    Gp.f4 = gb[4][lane]; Gp.f5 = gb[5][lane]; Gp.f6 = gb[6][lane]; Gp.f7 = gb[7][lane];
+                } else if (GLDS) {
                     Gp.f0 = gb[0][lane]; Gp.f1 = gb[1][lane]; Gp.f2 = gb[2][lane]; Gp.f3 = gb[3][lane];
                     Gp.f4 = gb[4][lane]; Gp.f6 = gb[5][lane]; Gp.f7 = gb[6][lane];
                 }
@@ -581,13 +589,19 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     }
                 }
             }
-            if (FLDS) {
+            if (FLDS && MULTI) {
+                fb[0][lane] = F.f0; fb[1][lane] = F.f1; fb[2][lane] = F.f2; fb[3][lane] = F.f3;
+                fb[4][lane] = F.f4; fb[5][lane] = F.f5; fb[6][lane] = F.f6; fb[7][lane] = F.f7;
+            } else if (FLDS) {
                 fb[0][lane] = F.f0; fb[1][lane] = F.f1; fb[2][lane] = F.f2;
                 fb[3][lane] = F.f3; fb[4][lane] = F.f6; fb[5][lane] = F.f7;
             } else {
                 Fprev = F;
             }
-            if (GLDS) {
+            if (GLDS && MULTI) {
+                gb[0][lane] = G.f0; gb[1][lane] = G.f1; gb[2][lane] = G.f2; gb[3][lane] = G.f3;
+                gb[4][lane] = G.f4; gb[5][lane] = G.f5; gb[6][lane] = G.f6; gb[7][lane] = G.f7;
+            } else if (GLDS) {
                 gb[0][lane] = G.f0; gb[1][lane] = G.f1; gb[2][lane] = G.f2; gb[3][lane] = G.f3;
                 gb[4][lane] = G.f4; gb[5][lane] = G.f6; gb[6][lane] = G.f7;
             } else {
