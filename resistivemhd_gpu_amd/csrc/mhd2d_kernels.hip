@@ -220,8 +220,10 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
         }
     };
     // the stage input (and in stage 2 U^n) of the cell being updated,
-    // staged before the y-face flux of the same column
-    constexpr bool UPD_DMA = DMA && !MULTI;
+    // staged before the y-face flux of the same column (not in stage 2 of
+    // the multi-GPU scheme with getEta planes, whose extra staging plane
+    // would take its LDS past eight wavefronts per CU)
+    constexpr bool UPD_DMA = DMA && !(ETAM && STAGE == 2);
     __shared__ __attribute__((aligned(16))) double upd[UPD_DMA ? (STAGE == 2 ? 14 : 8) : 1][kStageRows];
     // the previous column's face fluxes (the components the update reads:
     // six of F and seven of G here, all eight of each in the MPI library's
@@ -485,26 +487,45 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     if (s < iend && upd_row) {
                         const size_t os = (size_t)s * g.pitch + jl;
                         double n0, n1, n2, n3, n4, n5, n6, n7;
+                        // the stage input's (and U^n's) non-B components of cell s
+                        double S0, S1, S2, S3, S6, S7, O0 = 0.0, O1 = 0.0, O2 = 0.0, O3 = 0.0, O6 = 0.0, O7 = 0.0;
+                        if (UPD_DMA) {
+                            // s_waitcnt vmcnt(0): the cell's staging loads have landed
+                            __builtin_amdgcn_s_waitcnt(0x0F70);
+                            const int r = lane + 1;
+                            S0 = upd[0][r]; S1 = upd[1][r]; S2 = upd[2][r]; S3 = upd[3][r];
+                            S6 = upd[6][r]; S7 = upd[7][r];
+                            if (STAGE == 2) {
+                                O0 = upd[8][r]; O1 = upd[9][r]; O2 = upd[10][r]; O3 = upd[11][r];
+                                O6 = upd[12][r]; O7 = upd[13][r];
+                            }
+                        } else {
+                            S0 = Rh[os]; S1 = Mx[os]; S2 = My[os]; S3 = Mz[os]; S6 = Bz[os]; S7 = En[os];
+                            if (STAGE == 2) {
+                                O0 = PL(uold, 0)[os]; O1 = PL(uold, 1)[os]; O2 = PL(uold, 2)[os];
+                                O3 = PL(uold, 3)[os]; O6 = PL(uold, 6)[os]; O7 = PL(uold, 7)[os];
+                            }
+                        }
                         if (STAGE == 1) {
                             // oneStepFirst_kernel, M/resistiveMHD2D.cu:64-103
-                            n0 = Rh[os] - dtdx * (F.f0 - Fp.f0) - dtdy * (Gp.f0 - g0m);
-                            n1 = Mx[os] - dtdx * (F.f1 - Fp.f1) - dtdy * (Gp.f1 - g1m);
-                            n2 = My[os] - dtdx * (F.f2 - Fp.f2) - dtdy * (Gp.f2 - g2m);
-                            n3 = Mz[os] - dtdx * (F.f3 - Fp.f3) - dtdy * (Gp.f3 - g3m);
+                            n0 = S0 - dtdx * (F.f0 - Fp.f0) - dtdy * (Gp.f0 - g0m);
+                            n1 = S1 - dtdx * (F.f1 - Fp.f1) - dtdy * (Gp.f1 - g1m);
+                            n2 = S2 - dtdx * (F.f2 - Fp.f2) - dtdy * (Gp.f2 - g2m);
+                            n3 = S3 - dtdx * (F.f3 - Fp.f3) - dtdy * (Gp.f3 - g3m);
                             n4 = sbx - dtdx * (F.f4 - Fp.f4) - dtdy * (Gp.f4 - g4m);
                             n5 = sby - dtdx * (F.f5 - Fp.f5) - dtdy * (Gp.f5 - g5m);
-                            n6 = Bz[os] - dtdx * (F.f6 - Fp.f6) - dtdy * (Gp.f6 - g6m);
-                            n7 = En[os] - dtdx * (F.f7 - Fp.f7) - dtdy * (Gp.f7 - g7m);
+                            n6 = S6 - dtdx * (F.f6 - Fp.f6) - dtdy * (Gp.f6 - g6m);
+                            n7 = S7 - dtdx * (F.f7 - Fp.f7) - dtdy * (Gp.f7 - g7m);
                         } else {
                             // oneStepSecond_kernel, M/resistiveMHD2D.cu:106-144
-                            n0 = 0.5 * (PL(uold, 0)[os] + Rh[os] - dtdx * (F.f0 - Fp.f0) - dtdy * (Gp.f0 - g0m));
-                            n1 = 0.5 * (PL(uold, 1)[os] + Mx[os] - dtdx * (F.f1 - Fp.f1) - dtdy * (Gp.f1 - g1m));
-                            n2 = 0.5 * (PL(uold, 2)[os] + My[os] - dtdx * (F.f2 - Fp.f2) - dtdy * (Gp.f2 - g2m));
-                            n3 = 0.5 * (PL(uold, 3)[os] + Mz[os] - dtdx * (F.f3 - Fp.f3) - dtdy * (Gp.f3 - g3m));
+                            n0 = 0.5 * (O0 + S0 - dtdx * (F.f0 - Fp.f0) - dtdy * (Gp.f0 - g0m));
+                            n1 = 0.5 * (O1 + S1 - dtdx * (F.f1 - Fp.f1) - dtdy * (Gp.f1 - g1m));
+                            n2 = 0.5 * (O2 + S2 - dtdx * (F.f2 - Fp.f2) - dtdy * (Gp.f2 - g2m));
+                            n3 = 0.5 * (O3 + S3 - dtdx * (F.f3 - Fp.f3) - dtdy * (Gp.f3 - g3m));
                             n4 = 0.5 * (obx + sbx - dtdx * (F.f4 - Fp.f4) - dtdy * (Gp.f4 - g4m));
                             n5 = 0.5 * (oby + sby - dtdx * (F.f5 - Fp.f5) - dtdy * (Gp.f5 - g5m));
-                            n6 = 0.5 * (PL(uold, 6)[os] + Bz[os] - dtdx * (F.f6 - Fp.f6) - dtdy * (Gp.f6 - g6m));
-                            n7 = 0.5 * (PL(uold, 7)[os] + En[os] - dtdx * (F.f7 - Fp.f7) - dtdy * (Gp.f7 - g7m));
+                            n6 = 0.5 * (O6 + S6 - dtdx * (F.f6 - Fp.f6) - dtdy * (Gp.f6 - g6m));
+                            n7 = 0.5 * (O7 + S7 - dtdx * (F.f7 - Fp.f7) - dtdy * (Gp.f7 - g7m));
                         }
                         PL(uout, 0)[os] = n0; PL(uout, 1)[os] = n1; PL(uout, 2)[os] = n2; PL(uout, 3)[os] = n3;
                         PL(uout, 4)[os] = n4; PL(uout, 5)[os] = n5; PL(uout, 6)[os] = n6; PL(uout, 7)[os] = n7;
