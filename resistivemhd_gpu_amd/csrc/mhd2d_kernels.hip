@@ -240,9 +240,15 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
             size_t off = (size_t)col * g.pitch + rpair;
             asm volatile("" : "+v"(off));
 #pragma unroll
-            for (int v = 0; v < 8; ++v)
-                __builtin_amdgcn_global_load_lds((glb_void_t *)(uin + (size_t)v * g.plane + off),
-                                                 (lds_void_t *)upd[v], 16, 0, 0);
+            for (int v = 0; v < 8; ++v) {
+                // the multi-GPU scheme's stage 2 reads the smoothed face B
+                // instead of the stage input's: planes 4 and 5 stage U^n's Bx
+                // of the next column and By of this one for its U^n smoothing
+                const double *src = uin + (size_t)v * g.plane + off;
+                if (MULTI && STAGE == 2 && v == 4) src = uold + (size_t)4 * g.plane + off + g.pitch;
+                if (MULTI && STAGE == 2 && v == 5) src = uold + (size_t)5 * g.plane + off;
+                __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)upd[v], 16, 0, 0);
+            }
 #pragma unroll
             for (int k = 0; k < (STAGE == 2 ? 6 : 0); ++k) {
                 const int v = k < 4 ? k : k + 2;
@@ -254,6 +260,14 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
     if (DMA) stage_col(i0 - 2);
 
     double bx_prev = Bx[(size_t)(i0 - 3 + dcol) * g.pitch + jl];
+    // multi-GPU scheme, stage 2 with the update staged: U^n's Bx of the two
+    // columns before the next update's staged one (s-1 and s)
+    constexpr bool OLDB_LDS = MULTI && STAGE == 2 && UPD_DMA;
+    double obx_m = 0.0, obx_s = 0.0;
+    if (OLDB_LDS) {
+        obx_m = PL(uold, 4)[(size_t)(i0 - 1) * g.pitch + jl];
+        obx_s = PL(uold, 4)[(size_t)i0 * g.pitch + jl];
+    }
     Cell A{}, B{};                                   // cells c-2 and c-1
     Face7 Lx{};                                      // left state of face c-2+1/2
     Flux8 Fprev{}, Gprev{};                          // F(c-3+1/2), G(c-2)
@@ -474,10 +488,26 @@ __device__ __forceinline__ void stage_item(int item, const double *__restrict__ 
                     double obx = 0.0, oby = 0.0;
                     if (STAGE == 2) {
                         // face-smoothed B of U^n at cell s
-                        const size_t on = (size_t)s * g.pitch + jl;
-                        const double bxs = PL(uold, 4)[on], bxm = PL(uold, 4)[on - g.pitch];
-                        const double bxp = PL(uold, 4)[on + g.pitch];
-                        const double bys = PL(uold, 5)[on], bym1 = PL(uold, 5)[on - 1];
+                        double bxs, bxm, bxp, bys, bym1;
+                        if (OLDB_LDS) {
+                            // s_waitcnt vmcnt(0): the staging loads of cell s have landed
+                            __builtin_amdgcn_s_waitcnt(0x0F70);
+                            const int r = lane + 1;
+                            bxm = obx_m;
+                            bxs = obx_s;
+                            bxp = upd[4][r];
+                            bys = upd[5][r];
+                            bym1 = upd[5][r - 1];
+                            obx_m = obx_s;
+                            obx_s = bxp;
+                        } else {
+                            const size_t on = (size_t)s * g.pitch + jl;
+                            bxs = PL(uold, 4)[on];
+                            bxm = PL(uold, 4)[on - g.pitch];
+                            bxp = PL(uold, 4)[on + g.pitch];
+                            bys = PL(uold, 5)[on];
+                            bym1 = PL(uold, 5)[on - 1];
+                        }
                         const double byp = lane_next(bys);
                         const double cxs = 0.5 * (bxs + bxm), cxp = 0.5 * (bxp + bxs);
                         const double cys = 0.5 * (bys + bym1), cyp = 0.5 * (byp + bys);
