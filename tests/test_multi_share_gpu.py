@@ -91,6 +91,16 @@ def test_multi_share_turbulence_ragged(numerics):
     _compare(cfg, U0, 60, numerics, sweep=16, tag="m-turb")
 
 
+def test_multi_share_column_walk_remainders():
+    """The CT-side kernels walk columns in segments (32 for the corner EMF
+    with the CT, 8 for the smoothed cell EMF) and take 63 rows per
+    wavefront: 100 interior columns leave a short last segment of each, and
+    130 rows a last wavefront of 4 rows."""
+    cfg = mhd.default_config(100, 130, 0.0, 2.0 * np.pi, 0.0, 2.6 * np.pi, eta=1e-4)
+    U0 = _halo(cfg, mhd.fill_turbulence(cfg, seed=11, kmax=6, rms_v=0.5, rms_b=0.5))
+    _compare(cfg, U0, 40, "exact", tag="m-turb-remainders")
+
+
 def _periodic_eta_planes(cfg, seed, base):
     """getEta of a smooth periodic resistivity (a random Fourier sum around
     `base`, positive) sampled at the MPI library's x-face, y-face and cell
