@@ -479,10 +479,11 @@ __global__ __launch_bounds__(64) void face_adj_tail_kernel(const AdjDesc *runs, 
 // wave 1 the y faces of each column, in step.  Everything a column's faces
 // read sits in LDS rings (rows j0-2 .. j0+65; state columns i-2 .. i+2,
 // adjoint and resistivity columns as the stencils need).  The columns the
-// next step adds arrive by direct-to-LDS loads into a separate staging
-// array while this step computes (a separate object, so the compiler need
-// not wait for them before reading the rings), and are copied into the
-// rings at the next step.  Cell and resistivity adjoints accumulate in an
+// next step adds arrive by direct-to-LDS loads while this step computes,
+// straight into ring slots that hold one column more than a step reads, so
+// the slot each load fills is one no wave reads at this step; the loop-top
+// wait and barrier make them visible to the next step.  Cell and
+// resistivity adjoints accumulate in an
 // LDS ring of five columns and leave for HBM once per column when
 // complete, instead of with 38 atomics per face.
 constexpr int kSwRows = 68;   // LDS rows: global rows j0-2 .. j0+65
