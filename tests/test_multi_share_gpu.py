@@ -101,6 +101,15 @@ def test_multi_share_column_walk_remainders():
     _compare(cfg, U0, 40, "exact", tag="m-turb-remainders")
 
 
+def test_multi_share_turbulence_512():
+    """Bitwise at a size where every kernel of the step runs many work items:
+    the stage kernels several strips and segments, the walking CT kernels
+    16 (and 64) column segments and 9 row groups."""
+    cfg = mhd.turbulence_config(512, eta=1e-4)
+    U0 = _halo(cfg, mhd.fill_turbulence(cfg, seed=5, kmax=8, rms_v=0.5, rms_b=0.5))
+    _compare(cfg, U0, 30, "exact", tag="m-turb512")
+
+
 def _periodic_eta_planes(cfg, seed, base):
     """getEta of a smooth periodic resistivity (a random Fourier sum around
     `base`, positive) sampled at the MPI library's x-face, y-face and cell
