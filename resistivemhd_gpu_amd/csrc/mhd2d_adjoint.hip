@@ -795,6 +795,8 @@ __device__ __forceinline__ double lam_bstar(const AdjDesc &d, int v, int i, int 
     return d.lamBar[o] + d.lam[o];
 }
 
+// Also clears the cell's lamC for the stage-1 face adjoints that follow
+// (nothing reads lamC between the stage-2 cell adjoint and them).
 __global__ __launch_bounds__(256) void ct_adj_kernel(const AdjDesc *runs, TileMap tm, int step)
 {
     int r, i, j;
@@ -803,6 +805,8 @@ __global__ __launch_bounds__(256) void ct_adj_kernel(const AdjDesc *runs, TileMa
     const Grid &g = d.g;
     if (j >= g.ny) return;
     const size_t o = (size_t)i * g.pitch + j;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) d.lamC[v * g.plane + o] = 0.0;
     const double dt = d.dts[step];
     const double cy = dt / d.ph.dy, cx = dt / d.ph.dx;
     double le = 0.0;
@@ -1711,8 +1715,7 @@ int mhd_adjoint_backward(mhd_adjoint *a)
             if ((rc = launch_face_adj(a, 2, st, slot, c))) return rc;
             hipLaunchKernelGGL(cell2_kernel, tg, blk, 0, s, a->d_desc, a->tiles);
             if ((rc = bc_transpose(a, kLamBar))) return rc;
-            hipLaunchKernelGGL(ct_adj_kernel, tg, blk, 0, s, a->d_desc, a->tiles, st);
-            hipLaunchKernelGGL(zero_kernel, tg, blk, 0, s, a->d_desc, a->tiles, 1);
+            hipLaunchKernelGGL(ct_adj_kernel, tg, blk, 0, s, a->d_desc, a->tiles, st);   // and clears lamC
             if ((rc = launch_face_adj(a, 1, st, slot, c))) return rc;
             hipLaunchKernelGGL(cell1_kernel, tg, blk, 0, s, a->d_desc, a->tiles);
             AHIP(a, hipGetLastError());
