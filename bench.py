@@ -213,7 +213,7 @@ def batch_bench(nruns, steps, plain_steps, run_steps, numerics):
     }
 
 
-def adjoint_bench(nruns, steps, run_steps, batched_only=False):
+def adjoint_bench(nruns, steps, run_steps, batched_only=False, stride=0):
     """Runs/s of batched forward+adjoint against a loop of single-run
     forward+adjoint objects, on the batched workload.  States stay on the
     GPU: dL/dU_N = U_N as the seed, moved with the device-pointer transfers
@@ -224,7 +224,8 @@ def adjoint_bench(nruns, steps, run_steps, batched_only=False):
     uploads the per-slot descriptors, which costs a single-run object about
     a millisecond of host round trips).  runs/s = runs * (steps / run_steps)
     / seconds.  batched_only: the batched leg alone (for profiling it;
-    no plain loop, no ratio)."""
+    no plain loop, no ratio).  stride: the checkpoint stride of both legs
+    (0: ceil(sqrt(steps)); -1: the largest that fits in memory)."""
     import numpy as np
     import torch
     import resistivemhd_gpu_amd as mhd
@@ -235,7 +236,7 @@ def adjoint_bench(nruns, steps, run_steps, batched_only=False):
         obj.setU0(r, U0)
 
     def fwd_adj(obj, buf):
-        obj.forward(steps)
+        obj.forward(steps, stride=stride)
         obj.getU_all_device(buf)
         obj.set_seed_all_device(buf)
         obj.backward()
@@ -278,6 +279,7 @@ def adjoint_bench(nruns, steps, run_steps, batched_only=False):
         "speedup_vs_plain": t_plain / t_batch,
         "batched_s": t_batch, "plain_s": t_plain, "gradients_finite": bool(finite),
         "autotuned_sweep_columns": cols,
+        "checkpoint_stride": "sqrt" if stride == 0 else ("fit" if stride < 0 else stride),
         "note": "forward, seed transfer on the GPU and backward timed in both legs, after a warm-up of the same "
                 "step count (steady state)",
     }
@@ -597,6 +599,8 @@ def main():
     ap.add_argument("--workload", choices=["harris", "turbulence", "batch", "adjoint", "gradcheck"],
                     default="harris")
     ap.add_argument("--adj-steps", type=int, default=20, help="timed forward+adjoint steps per run")
+    ap.add_argument("--adj-stride", type=int, default=0,
+                    help="adjoint workload: checkpoint stride of both legs (0: ceil(sqrt(steps)), -1: fit in memory)")
     ap.add_argument("--adj-batched-only", action="store_true",
                     help="adjoint workload: time the batched leg only (no plain loop, no ratio)")
     ap.add_argument("--check-nx", type=int, default=1024)
@@ -640,7 +644,7 @@ def main():
     import resistivemhd_gpu_amd as mhd
 
     if args.workload == "adjoint":
-        res = adjoint_bench(args.batch_runs, args.adj_steps, 500, args.adj_batched_only)
+        res = adjoint_bench(args.batch_runs, args.adj_steps, 500, args.adj_batched_only, args.adj_stride)
         print(json.dumps({"metric": "batched forward+adjoint runs/s vs plain per-instance loop",
                           "value": res["batched_fwd_adj_runs_per_s"], "unit": "runs/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f64", "data": "synthetic",

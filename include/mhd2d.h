@@ -340,7 +340,13 @@ int mhd_batch_autotune(mhd_batch *b, int steps, int *columns);
  * reference, which has no differentiation), for a batch of runs as in
  * mhd_batch_*.  A rollout U_N = S_{N-1}(... S_0(BC(U0))) is run by
  * mhd_adjoint_forward, which records every step's dt and keeps a checkpoint
- * every `stride` steps (0: ceil(sqrt(nsteps))).  mhd_adjoint_backward takes
+ * every `stride` steps (0: ceil(sqrt(nsteps)); MHD_ADJ_STRIDE_FIT: the largest
+ * stride up to nsteps whose storage, 2 stride + ceil(nsteps / stride) states
+ * of every run, fits in half of the free device memory plus what the object
+ * already holds, so that with memory to spare the whole rollout stays
+ * resident and the backward pass recomputes no segment; never below
+ * ceil(sqrt(nsteps)); mhd_adjoint_get_stride reports the stride used).
+ * mhd_adjoint_backward takes
  * the seed dL/dU_N of each run and returns dL/dU0 (of the state passed to
  * mhd_adjoint_set_state, before the boundary fill) and dL/deta (per cell,
  * ghosts included), with every step's dt held at its recorded value
@@ -359,7 +365,9 @@ int mhd_adjoint_set_eta_field(mhd_adjoint *a, int run, const double *host_eta); 
 int mhd_adjoint_set_state(mhd_adjoint *a, int run, const double *host_aos);       /* U0 */
 int mhd_adjoint_set_dts(mhd_adjoint *a, int run, const double *host_dts, int nsteps);
 int mhd_adjoint_get_dts(mhd_adjoint *a, int run, double *host_dts, int nsteps);
+#define MHD_ADJ_STRIDE_FIT (-1)
 int mhd_adjoint_forward(mhd_adjoint *a, int nsteps, int stride, int frozen_dt);
+int mhd_adjoint_get_stride(const mhd_adjoint *a, int *stride);   /* the last forward's checkpoint stride */
 int mhd_adjoint_get_state(mhd_adjoint *a, int run, double *host_aos);   /* U_N (until the next forward or tangent) */
 int mhd_adjoint_set_seed(mhd_adjoint *a, int run, const double *host_aos);   /* dL/dU_N */
 int mhd_adjoint_backward(mhd_adjoint *a);

@@ -1312,9 +1312,32 @@ int ensure(mhd_adjoint *a, double **p, size_t *cap, size_t n)
     return MHD_OK;
 }
 
+// Checkpoint stride: explicit (> 0), ceil(sqrt(nsteps)) (0), or for
+// MHD_ADJ_STRIDE_FIT the largest stride up to nsteps whose history and
+// checkpoint storage, (2 K + ceil(nsteps / K)) states of every run, fits in
+// half of the device memory free now plus what this object already holds
+// (ensure() frees it before it allocates more); at nsteps the whole rollout
+// stays in the history and the backward pass recomputes nothing.
+int choose_stride(const mhd_adjoint *a, int nsteps, int stride)
+{
+    const int kmin = std::max(1, (int)std::ceil(std::sqrt((double)nsteps)));
+    if (stride > 0) return stride;
+    if (stride == 0) return kmin;
+    double state = 0.0;   // bytes of one state of every run
+    for (const AdjDesc &d : a->desc) state += 8.0 * 8.0 * (double)d.g.plane;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return kmin;
+    const double budget = 0.5 * (double)free_b + 8.0 * (double)(a->hist_cap + a->hists_cap + a->ckpt_cap);
+    for (int K = nsteps; K > kmin; --K) {
+        const int nck = (nsteps + K - 1) / K;
+        if ((2.0 * K + nck) * state <= budget) return K;
+    }
+    return kmin;
+}
+
 int layout_storage(mhd_adjoint *a, int nsteps, int stride)
 {
-    const int K = stride > 0 ? stride : std::max(1, (int)std::ceil(std::sqrt((double)nsteps)));
+    const int K = choose_stride(a, nsteps, stride);
     const int nck = (nsteps + K - 1) / K;
     size_t h = 0, c = 0;
     for (const AdjDesc &d : a->desc) {
@@ -1635,6 +1658,8 @@ int mhd_adjoint_forward(mhd_adjoint *a, int nsteps, int stride, int frozen_dt)
 {
     if (!a) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint");
     if (nsteps < 1) return afail(a, MHD_ERR_INVALID, "nsteps must be >= 1");
+    if (stride < MHD_ADJ_STRIDE_FIT)
+        return afail(a, MHD_ERR_INVALID, "stride must be > 0, 0 (ceil(sqrt(nsteps))) or MHD_ADJ_STRIDE_FIT");
     if (frozen_dt && per_run_steps(a) < nsteps) return afail(a, MHD_ERR_STATE, "frozen dt: set nsteps dt values first");
     int rc;
     if ((rc = ensure_dts(a, nsteps))) return rc;
@@ -1663,6 +1688,14 @@ int mhd_adjoint_forward(mhd_adjoint *a, int nsteps, int stride, int frozen_dt)
     AHIP(a, hipStreamSynchronize(stream_of(a)));
     a->nsteps = nsteps;
     a->forward_done = true;
+    return MHD_OK;
+}
+
+int mhd_adjoint_get_stride(const mhd_adjoint *a, int *stride)
+{
+    if (!a || !stride) return afail(nullptr, MHD_ERR_INVALID, "NULL adjoint or stride");
+    if (!a->forward_done) return afail(const_cast<mhd_adjoint *>(a), MHD_ERR_STATE, "run mhd_adjoint_forward first");
+    *stride = a->stride;
     return MHD_OK;
 }
 

@@ -37,6 +37,7 @@ EXPORTS = (
     "mhd_batch_get_packed_items",
     "mhd_adjoint_create", "mhd_adjoint_destroy", "mhd_adjoint_last_error", "mhd_adjoint_set_eta_field",
     "mhd_adjoint_set_state", "mhd_adjoint_set_dts", "mhd_adjoint_get_dts", "mhd_adjoint_forward",
+    "mhd_adjoint_get_stride",
     "mhd_adjoint_get_state", "mhd_adjoint_set_seed", "mhd_adjoint_backward", "mhd_adjoint_get_grad_state",
     "mhd_adjoint_get_grad_eta", "mhd_adjoint_synchronize", "mhd_adjoint_set_direction", "mhd_adjoint_tangent",
     "mhd_adjoint_get_tangent", "mhd_adjoint_set_states", "mhd_adjoint_get_states", "mhd_adjoint_set_seeds",
@@ -167,6 +168,7 @@ def load_library(path=LIB_PATH):
         "mhd_adjoint_set_dts": (I, [P, I, D, I]),
         "mhd_adjoint_get_dts": (I, [P, I, D, I]),
         "mhd_adjoint_forward": (I, [P, I, I, I]),
+        "mhd_adjoint_get_stride": (I, [P, ctypes.POINTER(I)]),
         "mhd_adjoint_get_state": (I, [P, I, D]),
         "mhd_adjoint_set_seed": (I, [P, I, D]),
         "mhd_adjoint_backward": (I, [P]),
@@ -706,9 +708,21 @@ class ResistiveMHD2DAdjoint:
         self._ok(self._lib.mhd_adjoint_get_dts(self._h, run, _dptr(out), n))
         return out
 
+    STRIDE_FIT = -1
+
     def forward(self, nsteps, stride=0, frozen_dt=False):
+        """Rollout of nsteps steps with a checkpoint every `stride` steps
+        (0: ceil(sqrt(nsteps)); STRIDE_FIT: the largest stride up to nsteps
+        whose storage fits in half of the free device memory, so that the
+        backward pass recomputes nothing when the whole rollout fits)."""
         self._ok(self._lib.mhd_adjoint_forward(self._h, int(nsteps), int(stride), 1 if frozen_dt else 0))
         self.nsteps = int(nsteps)
+
+    def stride(self):
+        """Checkpoint stride of the last forward."""
+        k = ctypes.c_int(0)
+        self._ok(self._lib.mhd_adjoint_get_stride(self._h, ctypes.byref(k)))
+        return k.value
 
     def getU(self, run):
         out = np.empty(self.shape(run))

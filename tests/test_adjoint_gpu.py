@@ -437,3 +437,40 @@ def test_eta_fields_device_upload_and_stale_rollout():
         assert np.array_equal(res[0][r][0], res[1][r][0])
         for q in (1, 2):
             assert _rel_l2(res[1][r][q], res[0][r][q]) <= (1e-11 if r == 0 else 1e-8)
+
+
+def test_stride_fit_keeps_the_whole_rollout():
+    """MHD_ADJ_STRIDE_FIT on a small batch: the stride becomes the step count
+    (the whole rollout stays in the history, nothing is recomputed), the
+    rollout is the sqrt(N)-checkpointed one bit for bit and the gradients
+    agree to the rounding of the face adjoints' atomic sums; a stride below
+    it is refused."""
+    c0, U0 = _ot(32)
+    c1, U1 = _ot(24)
+    U1 = _generic(U1, 3, 0.05)
+    cases = [(c0, U0), (c1, U1)]
+    nsteps = 12
+    out = {}
+    for stride in (0, mhd.ResistiveMHD2DAdjoint.STRIDE_FIT):
+        a = mhd.ResistiveMHD2DAdjoint([c for c, _ in cases])
+        for r, (c, U) in enumerate(cases):
+            a.setU0(r, U)
+            a.setEtaField(r, _eta_field(c, r))
+        a.forward(nsteps, stride=stride)
+        for r, (c, _) in enumerate(cases):
+            a.set_seed(r, _weights(c, 40 + r))
+        a.backward()
+        out[stride] = (a.stride(), [a.getU(r) for r in range(2)], [a.grad_state(r) for r in range(2)],
+                       [a.grad_eta(r) for r in range(2)])
+    k0, U_sq, g_sq, e_sq = out[0]
+    kf, U_f, g_f, e_f = out[mhd.ResistiveMHD2DAdjoint.STRIDE_FIT]
+    print(f"strides: sqrt {k0}, fit {kf}", flush=True)
+    assert k0 == 4 and kf == nsteps
+    for r in range(2):
+        assert np.array_equal(U_sq[r], U_f[r])
+        assert _rel_l2(g_sq[r], g_f[r]) <= 1e-12, r
+        assert _rel_l2(e_sq[r], e_f[r]) <= 1e-12, r
+    a = mhd.ResistiveMHD2DAdjoint([c0])
+    a.setU0(0, U0)
+    with pytest.raises(mhd.MHDError):
+        a.forward(nsteps, stride=-2)
