@@ -76,5 +76,8 @@ class MHDRollout(torch.autograd.Function):
 
 
 def rollout(U0, eta, cfgs, nsteps, stride=0, degenerate_tol=0.0):
-    """Final states after nsteps RK2 steps, differentiable in U0 and eta."""
+    """Final states after nsteps RK2 steps, differentiable in U0 and eta.
+    stride: checkpoint stride (0: ceil(sqrt(nsteps)); -1: the largest whose
+    storage fits in half of the free device memory, so that the backward
+    pass recomputes nothing when the whole rollout fits)."""
     return MHDRollout.apply(U0, eta, cfgs, nsteps, stride, float(degenerate_tol))
